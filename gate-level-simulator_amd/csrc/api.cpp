@@ -1,0 +1,426 @@
+// C ABI of libgls (include/gls.h).
+#include "../../include/gls.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <unordered_map>
+
+#include "csr.h"
+#include "engine.h"
+#include "formats.h"
+#include "gen.h"
+
+using namespace gls;
+
+static thread_local std::string g_err;
+
+struct gls_design {
+  Library lib;
+  Netlist nl;
+  std::string vlib_text, gv_text, sdf_text;  // empty when generated
+};
+struct gls_stim {
+  Stimulus st;
+};
+struct gls_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+};
+struct gls_run {
+  std::unique_ptr<Job> job;
+  Program pg;
+  std::vector<int> stim_nets;
+  std::vector<std::string> names;  // all net names incl. stimulus-only signals
+  std::vector<uint8_t> in_saif;
+  std::vector<uint8_t> is_gate;
+  uint64_t start = 0, end = 0;
+  int timescale_ps = 1;
+  bool keep_waves = false;
+  bool executed = false;
+  std::vector<uint64_t> saif;  // 3 per net
+  std::vector<uint64_t> woff;
+  std::vector<uint32_t> wlen, pool;
+  std::unordered_map<std::string, int> by_name;
+};
+
+#define API_BEGIN try {
+#define API_END                                  \
+  }                                              \
+  catch (const GlsError& e) {                    \
+    g_err = e.what();                            \
+    return e.code;                               \
+  }                                              \
+  catch (const std::bad_alloc&) {                \
+    g_err = "host out of memory";                \
+    return GLS_E_OOM;                            \
+  }                                              \
+  catch (const std::exception& e) {              \
+    g_err = e.what();                            \
+    return GLS_E_FORMAT;                         \
+  }                                              \
+  g_err.clear();                                 \
+  return GLS_OK;
+
+extern "C" {
+
+const char* gls_last_error(void) { return g_err.c_str(); }
+const char* gls_version(void) { return "gls-mi355x 0.1"; }
+
+int gls_design_from_texts(const char* vlib, const char* gv, const char* sdf, gls_design** out) {
+  API_BEGIN
+  if (!vlib || !gv || !out) throw GlsError(GLS_E_ARG, "null argument");
+  auto d = std::make_unique<gls_design>();
+  d->vlib_text = vlib;
+  d->gv_text = gv;
+  d->sdf_text = sdf ? sdf : "";
+  d->lib = parse_vlib_text(d->vlib_text);
+  d->nl = parse_gv_text(d->gv_text, d->lib);
+  if (sdf) parse_sdf_text(d->sdf_text, d->nl);
+  *out = d.release();
+  API_END
+}
+
+int gls_design_from_files(const char* vlib, const char* gv, const char* sdf, gls_design** out) {
+  API_BEGIN
+  if (!vlib || !gv || !out) throw GlsError(GLS_E_ARG, "null argument");
+  std::string a = read_file(vlib), b = read_file(gv), c = sdf ? read_file(sdf) : std::string();
+  int rc = gls_design_from_texts(a.c_str(), b.c_str(), sdf ? c.c_str() : nullptr, out);
+  if (rc != GLS_OK) return rc;
+  API_END
+}
+
+int gls_design_generate(int config, uint64_t seed, int n_cells, int n_levels, int n_pis,
+                        gls_design** out) {
+  API_BEGIN
+  if (!out) throw GlsError(GLS_E_ARG, "null argument");
+  auto d = std::make_unique<gls_design>();
+  d->lib = parse_vlib_text(generic_vlib_text());
+  NetlistSpec sp = config_netlist_spec(config);
+  if (seed) sp.seed = seed;
+  if (n_cells > 0) sp.n_cells = n_cells;
+  if (n_levels > 0) sp.n_levels = n_levels;
+  if (n_pis > 0) sp.n_pis = n_pis;
+  d->nl = generate_netlist(sp, d->lib);
+  *out = d.release();
+  API_END
+}
+
+static void ensure_texts(const gls_design* d, std::string& v, std::string& g, std::string& s) {
+  if (!d->gv_text.empty()) {
+    v = d->vlib_text;
+    g = d->gv_text;
+    s = d->sdf_text;
+  } else {
+    v = generic_vlib_text();
+    g = gv_text(d->nl, d->lib);
+    s = sdf_text(d->nl, d->lib);
+  }
+}
+
+int gls_design_save(const gls_design* d, const char* path) {
+  API_BEGIN
+  if (!d || !path) throw GlsError(GLS_E_ARG, "null argument");
+  std::string v, g, s;
+  ensure_texts(d, v, g, s);
+  std::string o = "GLS-INTER 1\n";
+  o += "vlib " + std::to_string(v.size()) + "\n" + v + "\n";
+  o += "gv " + std::to_string(g.size()) + "\n" + g + "\n";
+  o += "sdf " + std::to_string(s.size()) + "\n" + s + "\n";
+  write_file(path, o);
+  API_END
+}
+
+int gls_design_load(const char* path, gls_design** out) {
+  API_BEGIN
+  if (!path || !out) throw GlsError(GLS_E_ARG, "null argument");
+  std::string a = read_file(path);
+  size_t p = 0;
+  auto line = [&]() {
+    size_t q = a.find('\n', p);
+    if (q == std::string::npos) throw GlsError(GLS_E_FORMAT, "inter.dat: truncated");
+    std::string r = a.substr(p, q - p);
+    p = q + 1;
+    return r;
+  };
+  if (line() != "GLS-INTER 1") throw GlsError(GLS_E_FORMAT, "inter.dat: bad magic");
+  std::string parts[3];
+  const char* keys[3] = {"vlib", "gv", "sdf"};
+  for (int i = 0; i < 3; ++i) {
+    std::string h = line();
+    size_t sp = h.find(' ');
+    if (sp == std::string::npos || h.substr(0, sp) != keys[i])
+      throw GlsError(GLS_E_FORMAT, "inter.dat: missing section");
+    size_t n = std::stoull(h.substr(sp + 1));
+    if (p + n + 1 > a.size()) throw GlsError(GLS_E_FORMAT, "inter.dat: truncated section");
+    parts[i] = a.substr(p, n);
+    p += n + 1;
+  }
+  int rc = gls_design_from_texts(parts[0].c_str(), parts[1].c_str(), parts[2].c_str(), out);
+  if (rc != GLS_OK) return rc;
+  API_END
+}
+
+int gls_design_write(const gls_design* d, const char* vlib, const char* gv, const char* sdf) {
+  API_BEGIN
+  if (!d) throw GlsError(GLS_E_ARG, "null argument");
+  std::string v, g, s;
+  ensure_texts(d, v, g, s);
+  if (vlib) write_file(vlib, v);
+  if (gv) write_file(gv, g);
+  if (sdf) write_file(sdf, s);
+  API_END
+}
+
+static std::vector<std::string> declared_inputs(const Netlist& nl) {
+  std::vector<std::string> r;
+  for (size_t n = 0; n < nl.net_names.size(); ++n)
+    if (nl.net_is_input[n]) r.push_back(nl.net_names[n]);
+  return r;
+}
+
+int gls_design_info_get(const gls_design* d, gls_design_info* info) {
+  API_BEGIN
+  if (!d || !info) throw GlsError(GLS_E_ARG, "null argument");
+  Program pg = compile_program(d->nl, d->lib, declared_inputs(d->nl));
+  info->n_instances = d->nl.insts.size();
+  info->n_nets = d->nl.net_names.size();
+  info->n_inputs = declared_inputs(d->nl).size();
+  info->n_gates = pg.gates.size();
+  info->n_levels = (uint64_t)pg.n_levels();
+  info->n_pins = pg.n_pins;
+  API_END
+}
+
+void gls_design_free(gls_design* d) { delete d; }
+
+int gls_stim_from_vcd(const char* path, gls_stim** out) {
+  API_BEGIN
+  if (!path || !out) throw GlsError(GLS_E_ARG, "null argument");
+  auto s = std::make_unique<gls_stim>();
+  s->st = parse_vcd_file(path);
+  *out = s.release();
+  API_END
+}
+
+int gls_stim_generate(const gls_design* d, uint64_t seed, uint32_t t_end, double mean_gap,
+                      double x_prob, gls_stim** out) {
+  API_BEGIN
+  if (!d || !out) throw GlsError(GLS_E_ARG, "null argument");
+  if (!(mean_gap >= 1.0)) throw GlsError(GLS_E_ARG, "mean_gap must be >= 1");
+  StimulusSpec sp;
+  sp.seed = seed;
+  sp.t_end = t_end;
+  sp.mean_gap = mean_gap;
+  sp.x_prob = x_prob;
+  auto s = std::make_unique<gls_stim>();
+  s->st = generate_stimulus(d->nl, sp);
+  *out = s.release();
+  API_END
+}
+
+int gls_stim_write_vcd(const gls_stim* s, const char* path) {
+  API_BEGIN
+  if (!s || !path) throw GlsError(GLS_E_ARG, "null argument");
+  write_file(path, vcd_text(s->st));
+  API_END
+}
+
+int gls_stim_info(const gls_stim* s, uint64_t* n_signals, uint64_t* n_events) {
+  API_BEGIN
+  if (!s) throw GlsError(GLS_E_ARG, "null argument");
+  uint64_t ev = 0;
+  for (auto& w : s->st.waves) ev += w.size();
+  if (n_signals) *n_signals = s->st.names.size();
+  if (n_events) *n_events = ev;
+  API_END
+}
+
+void gls_stim_free(gls_stim* s) { delete s; }
+
+int gls_ctx_create(int device, gls_ctx** out) {
+  API_BEGIN
+  if (!out) throw GlsError(GLS_E_ARG, "null argument");
+  int n = 0;
+  hipError_t e = hipGetDeviceCount(&n);
+  if (e != hipSuccess || n <= 0) throw GlsError(GLS_E_DEVICE, "no HIP device available");
+  if (device < 0 || device >= n) throw GlsError(GLS_E_ARG, "device index out of range");
+  auto c = std::make_unique<gls_ctx>();
+  c->device = device;
+  hip_check(hipSetDevice(device), "hipSetDevice");
+  hip_check(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking), "hipStreamCreate");
+  *out = c.release();
+  API_END
+}
+
+void gls_ctx_destroy(gls_ctx* c) {
+  if (!c) return;
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+}
+
+int gls_prepare(gls_ctx* c, const gls_design* d, const gls_stim* s, uint64_t dumpon,
+                uint64_t dumpoff, const gls_sim_opts* opts, gls_run** out) {
+  API_BEGIN
+  if (!c || !d || !s || !out) throw GlsError(GLS_E_ARG, "null argument");
+  auto r = std::make_unique<gls_run>();
+  r->pg = compile_program(d->nl, d->lib, s->st.names);
+  r->names = r->pg.net_names;
+  r->in_saif = r->pg.in_saif;
+  r->is_gate.assign(r->names.size(), 0);
+  for (auto& g : r->pg.gates) r->is_gate[g.out_net] = 1;
+  for (size_t i = 0; i < r->names.size(); ++i) r->by_name.emplace(r->names[i], (int)i);
+  for (auto& nm : s->st.names) {
+    int id = d->nl.net(nm);
+    if (id < 0) {  // a VCD signal the netlist does not use: SAIF record only
+      auto it = r->by_name.find(nm);
+      if (it != r->by_name.end()) id = it->second;
+      else {
+        id = (int)r->names.size();
+        r->names.push_back(nm);
+        r->in_saif.push_back(1);
+        r->is_gate.push_back(0);
+        r->by_name.emplace(nm, id);
+      }
+    }
+    r->stim_nets.push_back(id);
+  }
+  r->start = dumpon;
+  r->end = dumpoff;
+  r->timescale_ps = s->st.timescale_ps;
+  r->keep_waves = opts && opts->keep_waves;
+  r->job = std::make_unique<Job>(c->device, c->stream);
+  if (opts && opts->seg_events) r->job->seg_target = opts->seg_events;
+  r->job->prepare(r->pg, s->st, r->stim_nets, dumpon, dumpoff, opts ? opts->pool_bytes / 4 : 0);
+  *out = r.release();
+  API_END
+}
+
+int gls_run_execute(gls_run* r) {
+  API_BEGIN
+  if (!r) throw GlsError(GLS_E_ARG, "null argument");
+  r->job->run();
+  r->job->fetch_saif(r->saif);
+  if (r->keep_waves) r->job->fetch_waves(r->woff, r->wlen, r->pool);
+  r->executed = true;
+  API_END
+}
+
+int gls_simulate(gls_ctx* c, const gls_design* d, const gls_stim* s, uint64_t dumpon,
+                 uint64_t dumpoff, const gls_sim_opts* opts, gls_run** out) {
+  int rc = gls_prepare(c, d, s, dumpon, dumpoff, opts, out);
+  if (rc != GLS_OK) return rc;
+  rc = gls_run_execute(*out);
+  if (rc != GLS_OK) {
+    std::string msg = g_err;
+    gls_run_free(*out);
+    *out = nullptr;
+    g_err = msg;
+  }
+  return rc;
+}
+
+int gls_run_stats_get(const gls_run* r, gls_run_stats* st) {
+  API_BEGIN
+  if (!r || !st) throw GlsError(GLS_E_ARG, "null argument");
+  const JobStats& j = r->job->stats();
+  st->n_events = j.n_events;
+  st->in_words = j.in_words;
+  st->out_words = j.out_words;
+  st->alg_bytes = j.alg_bytes;
+  st->n_nets = j.n_nets;
+  st->n_gates = j.n_gates;
+  st->n_pins = j.n_pins;
+  st->kernel_ms = j.kernel_ms;
+  st->wall_ms = j.wall_ms;
+  API_END
+}
+
+static std::string make_saif(const gls_run* r) {
+  if (!r->executed) throw GlsError(GLS_E_STATE, "run not executed");
+  std::vector<SaifRecord> recs;
+  for (size_t n = 0; n < r->names.size(); ++n) {
+    if (!r->in_saif[n]) continue;
+    SaifRecord s;
+    s.name = r->names[n];
+    s.t0 = r->saif[3 * n + 0];
+    s.t1 = r->saif[3 * n + 1];
+    s.tx = r->saif[3 * n + 2];
+    recs.push_back(std::move(s));
+  }
+  return saif_text(std::move(recs), r->start, r->end, r->timescale_ps);
+}
+
+int gls_run_write_saif(const gls_run* r, const char* path) {
+  API_BEGIN
+  if (!r || !path) throw GlsError(GLS_E_ARG, "null argument");
+  write_file(path, make_saif(r));
+  API_END
+}
+
+int gls_run_saif_text(const gls_run* r, char* buf, size_t cap, size_t* len) {
+  API_BEGIN
+  if (!r || !len) throw GlsError(GLS_E_ARG, "null argument");
+  std::string s = make_saif(r);
+  *len = s.size();
+  if (buf) {
+    if (cap < s.size()) throw GlsError(GLS_E_ARG, "buffer too small");
+    std::memcpy(buf, s.data(), s.size());
+  }
+  API_END
+}
+
+int gls_run_net_saif(const gls_run* r, const char* net, uint64_t v[3]) {
+  API_BEGIN
+  if (!r || !net || !v) throw GlsError(GLS_E_ARG, "null argument");
+  if (!r->executed) throw GlsError(GLS_E_STATE, "run not executed");
+  auto it = r->by_name.find(net);
+  if (it == r->by_name.end()) throw GlsError(GLS_E_ARG, std::string("unknown net ") + net);
+  for (int i = 0; i < 3; ++i) v[i] = r->saif[3 * it->second + i];
+  API_END
+}
+
+int gls_run_waveform(const gls_run* r, const char* net, uint32_t* words, size_t cap, size_t* n) {
+  API_BEGIN
+  if (!r || !net || !n) throw GlsError(GLS_E_ARG, "null argument");
+  if (!r->executed || !r->keep_waves) throw GlsError(GLS_E_STATE, "waveforms not kept");
+  auto it = r->by_name.find(net);
+  if (it == r->by_name.end()) throw GlsError(GLS_E_ARG, std::string("unknown net ") + net);
+  int id = it->second;
+  *n = r->wlen[id];
+  if (words) {
+    if (cap < r->wlen[id]) throw GlsError(GLS_E_ARG, "buffer too small");
+    std::memcpy(words, r->pool.data() + r->woff[id], r->wlen[id] * 4ull);
+  }
+  API_END
+}
+
+// Per-net FNV-1a over the name bytes then the little-endian event words,
+// finalised with SplitMix64 and summed over gate-output nets.
+int gls_run_waves_hash(const gls_run* r, uint64_t* hash) {
+  API_BEGIN
+  if (!r || !hash) throw GlsError(GLS_E_ARG, "null argument");
+  if (!r->executed || !r->keep_waves) throw GlsError(GLS_E_STATE, "waveforms not kept");
+  uint64_t total = 0;
+  for (size_t n = 0; n < r->names.size(); ++n) {
+    if (!r->is_gate[n]) continue;
+    uint64_t h = 1469598103934665603ull;
+    for (unsigned char ch : r->names[n]) h = (h ^ ch) * 1099511628211ull;
+    const uint32_t* w = r->pool.data() + r->woff[n];
+    for (uint32_t i = 0; i < r->wlen[n]; ++i)
+      for (int b = 0; b < 4; ++b) h = (h ^ ((w[i] >> (8 * b)) & 0xFF)) * 1099511628211ull;
+    uint64_t z = h + 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    total += z ^ (z >> 31);
+  }
+  *hash = total;
+  API_END
+}
+
+void gls_run_free(gls_run* r) { delete r; }
+
+}  // extern "C"

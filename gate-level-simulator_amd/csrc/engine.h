@@ -1,0 +1,82 @@
+// Device engine: uploads a compiled program + stimulus into HBM and runs the
+// level kernels.  All device memory is owned here.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "csr.h"
+#include "device.h"
+#include "netlist.h"
+
+namespace gls {
+
+struct JobStats {
+  uint64_t n_events = 0;        // gate events (distinct input-event times per output)
+  uint64_t in_words = 0;        // sum over gates of input waveform lengths
+  uint64_t out_words = 0;       // sum over gates of output waveform lengths
+  uint64_t n_nets = 0, n_gates = 0, n_pins = 0;
+  uint64_t alg_bytes = 0;       // algorithmic bytes of the level kernels
+  double kernel_ms = 0;         // device time of the level loop (hipEvent)
+  double wall_ms = 0;           // host wall time of the level loop
+  uint64_t pool_words_used = 0;
+};
+
+// A prepared simulation: program, delays, stimulus resident on the device.
+class Job {
+ public:
+  Job(int device, hipStream_t stream);
+  ~Job();
+  Job(const Job&) = delete;
+  Job& operator=(const Job&) = delete;
+
+  // Compile-time inputs; stim_nets[i] is the net id of stimulus signal i.
+  void prepare(const Program& pg, const Stimulus& st, const std::vector<int>& stim_nets,
+               uint64_t start, uint64_t end, uint64_t pool_words);
+  // Run all levels (plain per-level launches). Returns GLS status.
+  void run();
+  // Results
+  void fetch_saif(std::vector<uint64_t>& t0t1tx) const;
+  void fetch_waves(std::vector<uint64_t>& off, std::vector<uint32_t>& len,
+                   std::vector<uint32_t>& pool_words) const;
+  const JobStats& stats() const { return stats_; }
+  uint32_t seg_target = 256;
+
+ private:
+  void ensure(void** p, size_t& cap, size_t need);
+  int device_;
+  hipStream_t stream_;
+  uint64_t start_ = 0, end_ = 0;
+  uint32_t n_nets_ = 0, n_gates_ = 0;
+  std::vector<uint32_t> level_start_;
+  std::vector<uint32_t> saif_net_list_;
+  uint64_t pool_words_ = 0, stim_words_ = 0;
+  // device buffers
+  GateDev* d_gates_ = nullptr;
+  uint8_t* d_luts_ = nullptr;
+  uint64_t* d_woff_ = nullptr;
+  uint32_t* d_wlen_ = nullptr;
+  uint32_t* d_pool_ = nullptr;
+  unsigned long long* d_saif_ = nullptr;
+  uint32_t* d_saif_nets_ = nullptr;
+  uint64_t* d_scal_ = nullptr;  // [0]=bump [1]=total_segs [2]=total_scr [3]=total_out [4]=ev_count [5]=overflow
+  // per-level temporaries
+  uint64_t *d_nseg_ = nullptr, *d_scr_ = nullptr, *d_segbase_ = nullptr, *d_scrbase_ = nullptr;
+  uint64_t *d_outlen_ = nullptr, *d_outoff_ = nullptr, *d_part_ = nullptr;
+  uint32_t* d_gnseg_ = nullptr;
+  size_t cap_segmap_ = 0, cap_sums_ = 0, cap_carry_ = 0, cap_scratch_ = 0;
+  uint32_t* d_segmap_ = nullptr;
+  SegSum* d_sums_ = nullptr;
+  SegCarry* d_carry_ = nullptr;
+  uint32_t* d_scratch_ = nullptr;
+  std::vector<uint64_t> h_woff_init_;
+  std::vector<uint32_t> h_wlen_init_;
+  JobStats stats_;
+  std::vector<std::vector<uint32_t>> gate_in_nets_;  // for stats
+};
+
+void hip_check(hipError_t e, const char* what);
+
+}  // namespace gls

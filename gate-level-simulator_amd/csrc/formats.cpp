@@ -1,0 +1,782 @@
+#include "formats.h"
+
+#include <algorithm>
+#include <cctype>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <sstream>
+
+#include "../../include/gls_status.h"
+
+namespace gls {
+
+std::string read_file(const std::string& path) {
+  FILE* f = std::fopen(path.c_str(), "rb");
+  if (!f) throw GlsError(GLS_E_IO, "cannot open " + path);
+  std::string s;
+  std::fseek(f, 0, SEEK_END);
+  long n = std::ftell(f);
+  std::fseek(f, 0, SEEK_SET);
+  if (n > 0) {
+    s.resize((size_t)n);
+    size_t got = std::fread(&s[0], 1, (size_t)n, f);
+    s.resize(got);
+  }
+  std::fclose(f);
+  return s;
+}
+
+void write_file(const std::string& path, const std::string& data) {
+  FILE* f = std::fopen(path.c_str(), "wb");
+  if (!f) throw GlsError(GLS_E_IO, "cannot write " + path);
+  size_t w = std::fwrite(data.data(), 1, data.size(), f);
+  std::fclose(f);
+  if (w != data.size()) throw GlsError(GLS_E_IO, "short write to " + path);
+}
+
+// strtok-style split: every char of `delims` separates, empty tokens dropped
+// (the reference's PreProcess::Split, src/PreProcess.cpp:1466-1484).
+static std::vector<std::string> split(const std::string& s, const char* delims) {
+  std::vector<std::string> r;
+  size_t i = 0, n = s.size();
+  while (i < n) {
+    while (i < n && std::strchr(delims, s[i]) && s[i] != 0) ++i;
+    size_t j = i;
+    while (j < n && !(std::strchr(delims, s[j]) && s[j] != 0)) ++j;
+    if (j > i) r.push_back(s.substr(i, j - i));
+    i = j;
+  }
+  return r;
+}
+
+static std::string lower(std::string s) {
+  for (auto& c : s) c = (char)std::tolower((unsigned char)c);
+  return s;
+}
+static std::string upper(std::string s) {
+  for (auto& c : s) c = (char)std::toupper((unsigned char)c);
+  return s;
+}
+static std::string trim_spaces(const std::string& s) {
+  size_t a = s.find_first_not_of(' ');
+  if (a == std::string::npos) return "";
+  size_t b = s.find_last_not_of(' ');
+  return s.substr(a, b - a + 1);
+}
+
+static std::vector<std::string> lines_of(const std::string& text) {
+  std::vector<std::string> out;
+  size_t p = 0;
+  while (p <= text.size()) {
+    size_t q = text.find('\n', p);
+    if (q == std::string::npos) {
+      if (p < text.size()) out.push_back(text.substr(p));
+      break;
+    }
+    out.push_back(text.substr(p, q - p));
+    p = q + 1;
+  }
+  return out;
+}
+
+// ---------------------------------------------------------------- vlib ----
+// src/PreProcess.cpp:42-648.
+Library parse_vlib_text(const std::string& text) {
+  Library lib;
+  auto lines = lines_of(text);
+  const char* D = " \t\r\n,;()";
+  int const_counter = 0;
+  for (size_t li = 0; li < lines.size(); ++li) {
+    auto tmp = split(lines[li], D);
+    if (tmp.empty()) continue;
+    if (lower(tmp[0]) != "module") continue;
+    if (tmp.size() < 2) throw GlsError(GLS_E_FORMAT, "vlib: module without a name");
+    CellType ct;
+    ct.name = tmp[1];
+    bool ignored = false, closed = false;
+    for (++li; li < lines.size(); ++li) {
+      auto t = split(lines[li], D);
+      if (t.empty()) continue;
+      std::string kw = lower(t[0]);
+      if (kw == "input" || kw == "output" || kw == "wire" || kw == "reg" || kw == "inout") {
+        if (t.size() < 2) throw GlsError(GLS_E_FORMAT, "vlib: empty declaration in " + ct.name);
+        std::vector<std::string> names;
+        if (t[1].find('[') != std::string::npos) {
+          if (t.size() < 3) throw GlsError(GLS_E_FORMAT, "vlib: bus declaration without a name");
+          names.push_back(t[2]);
+        } else {
+          names.assign(t.begin() + 1, t.end());
+        }
+        for (auto& n : names) {
+          if (kw == "input" || kw == "output" || kw == "inout") ct.ports.push_back(n);
+          if (kw == "input") ct.inputs.push_back(n);
+          if (kw == "output") ct.outputs.push_back(n);
+        }
+      } else if (kw == "and" || kw == "or" || kw == "xor" || kw == "xnor" || kw == "nor" ||
+                 kw == "nand" || kw == "buf" || kw == "not" || kw == "udp_xbuf" ||
+                 kw == "udp_mux2") {
+        Prim p;
+        static const char* ops[] = {"and", "or", "xor", "xnor", "nor", "nand",
+                                    "buf", "not", "udp_xbuf", "udp_mux2"};
+        for (int o = 0; o < 10; ++o)
+          if (kw == ops[o]) p.op = o;
+        size_t first = (p.op == P_XBUF || p.op == P_MUX2) ? 2 : 1;  // UDPs carry an instance name
+        if (t.size() < first + 2) throw GlsError(GLS_E_FORMAT, "vlib: primitive without pins in " + ct.name);
+        if (p.op == P_XBUF && t.size() != 5) throw GlsError(GLS_E_FORMAT, "vlib: udp_xbuf needs 2 inputs");
+        if (p.op == P_MUX2 && t.size() != 6) throw GlsError(GLS_E_FORMAT, "vlib: udp_mux2 needs 3 inputs");
+        p.out = t[first];
+        for (size_t i = first + 1; i < t.size(); ++i) {
+          if (t[i] == "1'b1" || t[i] == "1'b0") {
+            std::string nm = "zc_const" + std::to_string(const_counter++);
+            (t[i] == "1'b1" ? ct.supply1 : ct.supply0).push_back(nm);
+            p.ins.push_back(nm);
+          } else {
+            p.ins.push_back(t[i]);
+          }
+        }
+        ct.prims.push_back(p);
+      } else if (kw == "supply0") {
+        for (size_t i = 1; i < t.size(); ++i) ct.supply0.push_back(t[i]);
+      } else if (kw == "supply1") {
+        for (size_t i = 1; i < t.size(); ++i) ct.supply1.push_back(t[i]);
+      } else if (kw == "udp_dff" || kw == "udp_tlat" || kw == "always") {
+        ignored = true;
+        break;
+      } else if (kw == "endmodule") {
+        closed = true;
+        break;
+      }
+    }
+    if (ignored) {
+      lib.ignored.push_back(ct.name);
+      continue;
+    }
+    if (!closed) throw GlsError(GLS_E_FORMAT, "vlib: module " + ct.name + " has no endmodule");
+    lib.add(std::move(ct));
+  }
+  return lib;
+}
+
+Library parse_vlib_file(const std::string& path) { return parse_vlib_text(read_file(path)); }
+
+// ------------------------------------------------------------------ gv ----
+// src/PreProcess.cpp:650-1274, restricted to flat single-module netlists
+// with named .pin(net) connections, scalar and bus nets, {..}
+// concatenations, 1'b0/1'b1 ties and `assign`.
+static std::string strip_tail(const std::string& s) {
+  if (!s.empty() && (s.back() == ',' || s.back() == ';')) return s.substr(0, s.size() - 1);
+  return s;
+}
+
+static void parse_range(const std::string& tok, int& l, int& r) {
+  size_t a = tok.find('['), c = tok.find(':'), b = tok.find(']');
+  if (a == std::string::npos || c == std::string::npos || b == std::string::npos)
+    throw GlsError(GLS_E_FORMAT, "gv: bad range " + tok);
+  l = std::stoi(tok.substr(a + 1, c - a - 1));
+  r = std::stoi(tok.substr(c + 1, b - c - 1));
+}
+
+Netlist parse_gv_text(const std::string& text, const Library& lib) {
+  Netlist nl;
+  auto lines = lines_of(text);
+  bool in_module = false;
+  std::string stmt;
+  int hidden_const = 0;
+  auto declared = [&](const std::string& n) { return nl.net(n) >= 0; };
+  auto const_net = [&](const std::string& v) {
+    std::string nm = "gv_net_const" + std::to_string(hidden_const++);
+    int id = nl.add_net(nm);
+    nl.const_val[id] = (v == "1'b1") ? V1 : V0;
+    nl.const_hidden[id] = 1;
+    return id;
+  };
+  for (auto& raw : lines) {
+    std::string line = trim_spaces(raw);
+    if (line.empty()) continue;
+    auto tmp = split(line, " \t\r\n");
+    if (tmp.empty()) continue;
+    std::string kw = lower(tmp[0]);
+    if (kw == "module") {
+      in_module = true;
+      if (tmp.size() > 1) {
+        std::string nm = tmp[1];
+        size_t p = nm.find('(');
+        nl.top = p == std::string::npos ? nm : nm.substr(0, p);
+      }
+      // the header's port list may span lines; it declares nothing
+      if (line.back() != ';') stmt = "\x01";
+      continue;
+    }
+    if (kw == "endmodule") break;
+    if (!in_module) continue;
+    if (stmt == "\x01") {  // inside a multi-line module header
+      if (line.back() == ';') stmt.clear();
+      continue;
+    }
+    stmt += " " + line;
+    if (line.back() != ';') continue;
+    auto t = split(stmt, " \t\r\n");
+    stmt.clear();
+    if (t.empty()) continue;
+    std::string k0 = lower(t[0]);
+    if (k0 == "input" || k0 == "output" || k0 == "wire" || k0 == "inout") {
+      if (t.size() < 2) throw GlsError(GLS_E_FORMAT, "gv: empty declaration");
+      if (t[1].find('[') != std::string::npos) {
+        if (t.size() < 3) throw GlsError(GLS_E_FORMAT, "gv: bus declaration without a name");
+        int l, r;
+        parse_range(t[1], l, r);
+        // a bus declaration may list several names: input [3:0] a, b;
+        for (size_t ni = 2; ni < t.size(); ++ni) {
+          std::string nm = strip_tail(t[ni]);
+          if (nm.empty()) continue;
+          int step = l > r ? -1 : 1;
+          for (int i = l;; i += step) {
+            int id = nl.add_net(nm + "[" + std::to_string(i) + "]");
+            if (k0 == "input") nl.net_is_input[id] = 1;
+            if (i == r) break;
+          }
+        }
+      } else {
+        for (size_t i = 1; i < t.size(); ++i) {
+          std::string nm = strip_tail(t[i]);
+          if (nm.empty()) continue;
+          int id = nl.add_net(nm);
+          if (k0 == "input") nl.net_is_input[id] = 1;
+        }
+      }
+      continue;
+    }
+    if (k0 == "assign") {
+      // assign left = right;  (any spacing)
+      std::string body;
+      for (size_t i = 1; i < t.size(); ++i) body += t[i];
+      if (!body.empty() && body.back() == ';') body.pop_back();
+      size_t eq = body.find('=');
+      if (eq == std::string::npos) throw GlsError(GLS_E_FORMAT, "gv: assign without '='");
+      std::string left = body.substr(0, eq), right = body.substr(eq + 1);
+      int lid = nl.add_net(left);
+      if (right == "1'b1" || right == "1'b0") {
+        nl.const_val[lid] = right == "1'b1" ? V1 : V0;
+      } else {
+        int rid = nl.add_net(right);
+        nl.alias_of[lid] = rid;
+      }
+      continue;
+    }
+    // instance statement
+    int ty = lib.find(t[0]);
+    if (ty < 0) continue;  // the reference skips unknown modules with a warning
+    const CellType& ct = lib.types[ty];
+    std::string body;
+    for (size_t i = 1; i < t.size(); ++i) body += t[i];
+    size_t lp = body.find('(');
+    if (lp == std::string::npos) throw GlsError(GLS_E_FORMAT, "gv: instance without connections");
+    Instance inst;
+    inst.name = body.substr(0, lp);
+    inst.type = ty;
+    std::vector<std::string> used_ports;
+    // connections: .pin(net) or .pin({a,b[3:0],1'b0})
+    size_t p = lp + 1;
+    while ((p = body.find('.', p)) != std::string::npos) {
+      size_t po = body.find('(', p);
+      if (po == std::string::npos) throw GlsError(GLS_E_FORMAT, "gv: bad connection in " + inst.name);
+      std::string pin = body.substr(p + 1, po - p - 1);
+      int depth = 1;
+      size_t q = po + 1;
+      while (q < body.size() && depth > 0) {
+        if (body[q] == '(') depth++;
+        else if (body[q] == ')') depth--;
+        if (depth > 0) ++q;
+      }
+      std::string conn = body.substr(po + 1, q - po - 1);
+      p = q;
+      auto it = std::find(ct.ports.begin(), ct.ports.end(), pin);
+      if (it == ct.ports.end()) throw GlsError(GLS_E_FORMAT, "gv: " + ct.name + " has no pin " + pin);
+      bool is_in = std::find(ct.inputs.begin(), ct.inputs.end(), pin) != ct.inputs.end();
+      bool is_out = std::find(ct.outputs.begin(), ct.outputs.end(), pin) != ct.outputs.end();
+      if (!is_in && !is_out) throw GlsError(GLS_E_FORMAT, "gv: pin " + pin + " must be input or output");
+      used_ports.push_back(pin);
+      std::vector<std::string> nets;
+      if (!conn.empty() && conn[0] == '{') {
+        std::string inner = conn.substr(1, conn.size() - 2);
+        for (auto& part : split(inner, ",")) {
+          if (part.find('[') != std::string::npos && part.find(':') != std::string::npos) {
+            int l, r;
+            parse_range(part, l, r);
+            std::string base = part.substr(0, part.find('['));
+            int step = l > r ? -1 : 1;
+            for (int i = l;; i += step) {
+              nets.push_back(base + "[" + std::to_string(i) + "]");
+              if (i == r) break;
+            }
+          } else {
+            nets.push_back(part);
+          }
+        }
+      } else {
+        nets.push_back(conn);
+      }
+      for (size_t bi = 0; bi < nets.size(); ++bi) {
+        const std::string& nn = nets[bi];
+        std::string pin_bit = nets.size() > 1 ? pin + "[" + std::to_string(bi) + "]" : pin;
+        int id;
+        if (nn == "1'b1" || nn == "1'b0") id = const_net(nn);
+        else {
+          if (!declared(nn) && nn.find('[') == std::string::npos)
+            throw GlsError(GLS_E_FORMAT, "gv: undeclared net " + nn + " on " + inst.name);
+          id = nl.add_net(nn);
+        }
+        if (is_in) inst.in_pins.push_back({pin_bit, id});
+        else inst.out_pins.push_back({pin_bit, id});
+      }
+    }
+    for (auto& op : ct.outputs)
+      if (std::find(used_ports.begin(), used_ports.end(), op) == used_ports.end())
+        inst.unknown_out.push_back(op);
+    nl.insts.push_back(std::move(inst));
+  }
+  return nl;
+}
+
+Netlist parse_gv_file(const std::string& path, const Library& lib) {
+  return parse_gv_text(read_file(path), lib);
+}
+
+// ----------------------------------------------------------------- SDF ----
+// src/PreProcess.cpp:1276-1464: TIMESCALE factor (default 1ns = 1e3),
+// INSTANCE blocks closed by the first line that trims to ")", typ value of
+// each triple times the factor.
+void parse_sdf_text(const std::string& text, Netlist& nl) {
+  std::unordered_map<std::string, int> by_name;
+  for (size_t i = 0; i < nl.insts.size(); ++i) by_name[nl.insts[i].name] = (int)i;
+  auto lines = lines_of(text);
+  double factor = 1e3;
+  const char* D = " \t\r\n\"'()";
+  auto typ = [&](const std::string& tok) -> double {
+    auto v = split(tok, ":");
+    const std::string& s = v.size() == 3 ? v[1] : (v.empty() ? tok : v[0]);
+    char* end = nullptr;
+    double x = std::strtod(s.c_str(), &end);
+    if (end == s.c_str()) throw GlsError(GLS_E_FORMAT, "sdf: bad delay value " + tok);
+    return x * factor;
+  };
+  for (size_t li = 0; li < lines.size(); ++li) {
+    std::string line = trim_spaces(lines[li]);
+    if (line.empty()) continue;
+    auto tmp = split(line, D);
+    if (tmp.empty()) continue;
+    std::string k0 = upper(tmp[0]);
+    if (k0 == "TIMESCALE" && tmp.size() > 1) {
+      std::string ts = tmp[1];
+      if (ts == "1ns") factor = 1e3;
+      else if (ts == "1ps") factor = 1.0;
+      else if (ts == "1us") factor = 1e6;
+      else if (ts == "1ms") factor = 1e9;
+      else if (ts == "1s") factor = 1e12;
+      // other spellings keep the current factor, as the reference does
+      continue;
+    }
+    if (k0 != "INSTANCE" || tmp.size() < 2) continue;
+    auto it = by_name.find(tmp[1]);
+    if (it == by_name.end()) continue;
+    Instance& inst = nl.insts[it->second];
+    for (++li; li < lines.size(); ++li) {
+      std::string l2 = trim_spaces(lines[li]);
+      if (l2.empty()) continue;
+      if (l2 == ")") break;
+      auto t = split(l2, D);
+      if (t.empty() || upper(t[0]) != "IOPATH") continue;
+      IoPath d;
+      size_t ip, op, vp;
+      std::string e = upper(t.size() > 1 ? t[1] : "");
+      if (e == "POSEDGE" || e == "NEGEDGE") {
+        if (t.size() < 5) throw GlsError(GLS_E_FORMAT, "sdf: short IOPATH in " + inst.name);
+        d.edge = e == "POSEDGE" ? 0 : 1;
+        ip = 2; op = 3; vp = 4;
+      } else {
+        if (t.size() < 4) throw GlsError(GLS_E_FORMAT, "sdf: short IOPATH in " + inst.name);
+        d.edge = 2;
+        ip = 1; op = 2; vp = 3;
+      }
+      d.in_net = -1;
+      for (auto& pp : inst.in_pins)
+        if (pp.first == t[ip]) { d.in_net = pp.second; break; }
+      d.out_net = -1000000;
+      for (auto& pp : inst.out_pins)
+        if (pp.first == t[op]) { d.out_net = pp.second; break; }
+      if (d.out_net == -1000000) {
+        for (size_t k = 0; k < inst.unknown_out.size(); ++k)
+          if (inst.unknown_out[k] == t[op]) { d.out_net = -2 - (int)k; break; }
+      }
+      if (d.out_net == -1000000)
+        throw GlsError(GLS_E_FORMAT, "sdf: instance " + inst.name + " has no output pin " + t[op]);
+      d.rise = typ(t[vp]);
+      d.fall = t.size() > vp + 1 ? typ(t[vp + 1]) : d.rise;
+      inst.delays.push_back(d);
+    }
+  }
+}
+
+void parse_sdf_file(const std::string& path, Netlist& nl) { parse_sdf_text(read_file(path), nl); }
+
+// ----------------------------------------------------------------- VCD ----
+static int vcd_code(char c) {
+  switch (c) {
+    case '0': return V0;
+    case '1': return V1;
+    case 'x': case 'X': return VX;
+    case 'z': case 'Z': return VZ;
+    default: return -1;
+  }
+}
+
+static const uint64_t kMaxTime = (1ull << 30) - (1ull << 17);
+
+Stimulus parse_vcd_text(const std::string& s) {
+  Stimulus st;
+  // header
+  size_t p = 0, n = s.size();
+  auto skip_ws = [&]() { while (p < n && std::isspace((unsigned char)s[p])) ++p; };
+  auto word = [&]() {
+    skip_ws();
+    size_t a = p;
+    while (p < n && !std::isspace((unsigned char)s[p])) ++p;
+    return s.substr(a, p - a);
+  };
+  struct Var { std::vector<int> sig; int lindex = -1, rindex = -1; bool vec = false; };
+  std::unordered_map<std::string, std::vector<Var>> by_id;
+  bool scope_seen = false;
+  std::vector<uint8_t> is_vec_bit;
+  std::vector<uint8_t> last_bit;  // per signal, for bus-bit dedup (initial Z)
+  for (;;) {
+    std::string w = word();
+    if (w.empty()) throw GlsError(GLS_E_FORMAT, "vcd: missing $enddefinitions");
+    if (w == "$enddefinitions") {
+      while (!(w = word()).empty() && w != "$end") {}
+      break;
+    }
+    if (w == "$timescale") {
+      std::string acc;
+      while (!(w = word()).empty() && w != "$end") acc += w;
+      size_t k = 0;
+      while (k < acc.size() && std::isdigit((unsigned char)acc[k])) ++k;
+      int mult = k ? std::stoi(acc.substr(0, k)) : 1;
+      std::string unit = acc.substr(k);
+      long long ps = 1;
+      if (unit == "ps") ps = 1;
+      else if (unit == "ns") ps = 1000;
+      else if (unit == "us") ps = 1000000;
+      else if (unit == "fs") throw GlsError(GLS_E_UNSUPPORTED, "vcd: fs timescale unsupported");
+      else throw GlsError(GLS_E_FORMAT, "vcd: bad timescale " + acc);
+      st.timescale_ps = (int)(mult * ps);
+      continue;
+    }
+    if (w == "$scope") {
+      std::string type = word(), name = word();
+      if (!scope_seen) { st.scope = name; scope_seen = true; }
+      while (!(w = word()).empty() && w != "$end") {}
+      continue;
+    }
+    if (w == "$var") {
+      std::vector<std::string> f;
+      while (!(w = word()).empty() && w != "$end") f.push_back(w);
+      if (f.size() < 4) throw GlsError(GLS_E_FORMAT, "vcd: short $var");
+      int size = std::stoi(f[1]);
+      std::string id = f[2], ref = f[3];
+      Var v;
+      std::string range = f.size() > 4 ? f[4] : "";
+      if (ref.find('[') != std::string::npos && range.empty()) {
+        range = ref.substr(ref.find('['));
+        ref = ref.substr(0, ref.find('['));
+      }
+      if (!range.empty()) {
+        std::string inner = range.substr(1, range.size() - 2);
+        size_t c = inner.find(':');
+        if (c != std::string::npos) {
+          v.lindex = std::stoi(inner.substr(0, c));
+          v.rindex = std::stoi(inner.substr(c + 1));
+          v.vec = true;
+        } else {
+          // single bit of a bus: its own scalar signal named ref[i]
+          ref = ref + "[" + inner + "]";
+        }
+      } else if (size > 1) {
+        v.lindex = size - 1;
+        v.rindex = 0;
+        v.vec = true;
+      }
+      if (v.vec) {
+        int step = v.lindex > v.rindex ? -1 : 1;
+        for (int i = v.lindex;; i += step) {
+          v.sig.push_back((int)st.names.size());
+          st.names.push_back(ref + "[" + std::to_string(i) + "]");
+          st.waves.emplace_back();
+          is_vec_bit.push_back(1);
+          last_bit.push_back(VZ);
+          if (i == v.rindex) break;
+        }
+      } else {
+        v.sig.push_back((int)st.names.size());
+        st.names.push_back(ref);
+        st.waves.emplace_back();
+        is_vec_bit.push_back(0);
+        last_bit.push_back(VZ);
+      }
+      by_id[id].push_back(v);
+      continue;
+    }
+    if (w[0] == '$') {  // $date $version $comment $upscope ...
+      if (w != "$upscope" && w != "$end")
+        while (!(w = word()).empty() && w != "$end") {}
+      continue;
+    }
+    throw GlsError(GLS_E_FORMAT, "vcd: unexpected token in header: " + w);
+  }
+  // body
+  uint64_t cur_t = 0;
+  auto put = [&](int sig, uint8_t v) {
+    Wave& wv = st.waves[sig];
+    if (is_vec_bit[sig]) {  // bus bits are de-duplicated, starting from Z
+      if (v == last_bit[sig]) return;
+      last_bit[sig] = v;
+    }
+    if (!wv.t.empty() && wv.t.back() == (uint32_t)cur_t) {
+      wv.v.back() = v;  // several changes at one timestamp: the last wins
+      return;
+    }
+    wv.push((uint32_t)cur_t, v);
+  };
+  while (p < n) {
+    skip_ws();
+    if (p >= n) break;
+    char c = s[p];
+    if (c == '#') {
+      ++p;
+      uint64_t t = 0;
+      size_t a = p;
+      while (p < n && std::isdigit((unsigned char)s[p])) t = t * 10 + (uint64_t)(s[p++] - '0');
+      if (p == a) throw GlsError(GLS_E_FORMAT, "vcd: bad timestamp");
+      if (t > kMaxTime) throw GlsError(GLS_E_RANGE, "vcd: time beyond the 30-bit event word");
+      if (t < cur_t) throw GlsError(GLS_E_FORMAT, "vcd: timestamps go backwards");
+      cur_t = t;
+      continue;
+    }
+    if (c == '$') {
+      std::string w = word();  // $dumpvars / $dumpall / $dumpon / $dumpoff / $end / $comment
+      if (w == "$comment")
+        while (!(w = word()).empty() && w != "$end") {}
+      continue;
+    }
+    if (c == 'b' || c == 'B') {
+      ++p;
+      size_t a = p;
+      while (p < n && !std::isspace((unsigned char)s[p])) ++p;
+      std::string bits = s.substr(a, p - a);
+      std::string id = word();
+      auto it = by_id.find(id);
+      if (it == by_id.end()) throw GlsError(GLS_E_FORMAT, "vcd: unknown identifier " + id);
+      for (auto& v : it->second) {
+        size_t width = v.sig.size();
+        std::string b = bits;
+        if (b.size() < width) {
+          char pad = (b[0] == 'x' || b[0] == 'X' || b[0] == 'z' || b[0] == 'Z') ? b[0] : '0';
+          b = std::string(width - b.size(), pad) + b;
+        }
+        for (size_t k = 0; k < width; ++k) {
+          int code = vcd_code(b[b.size() - width + k]);
+          if (code < 0) throw GlsError(GLS_E_FORMAT, "vcd: bad vector value");
+          put(v.sig[k], (uint8_t)code);
+        }
+      }
+      continue;
+    }
+    if (c == 'r' || c == 'R') {  // real values are ignored
+      word();
+      word();
+      continue;
+    }
+    int code = vcd_code(c);
+    if (code < 0) throw GlsError(GLS_E_FORMAT, std::string("vcd: unexpected character '") + c + "'");
+    ++p;
+    size_t a = p;
+    while (p < n && !std::isspace((unsigned char)s[p])) ++p;
+    std::string id = s.substr(a, p - a);
+    auto it = by_id.find(id);
+    if (it == by_id.end()) throw GlsError(GLS_E_FORMAT, "vcd: unknown identifier " + id);
+    for (auto& v : it->second) {
+      if (v.vec) {
+        for (int sig : v.sig) put(sig, (uint8_t)code);
+      } else {
+        put(v.sig[0], (uint8_t)code);
+      }
+    }
+  }
+  return st;
+}
+
+Stimulus parse_vcd_file(const std::string& path) { return parse_vcd_text(read_file(path)); }
+
+// ------------------------------------------------------------- writers ----
+std::string gv_text(const Netlist& nl, const Library& lib) {
+  std::string o;
+  o.reserve(nl.insts.size() * 64 + nl.net_names.size() * 12);
+  o += "module " + nl.top + " ( );\n";
+  std::string ins, wires;
+  for (size_t n = 0; n < nl.net_names.size(); ++n) {
+    if (nl.const_hidden[n]) continue;
+    std::string& dst = nl.net_is_input[n] ? ins : wires;
+    dst += dst.empty() ? "" : ",\n    ";
+    dst += nl.net_names[n];
+  }
+  if (!ins.empty()) o += "  input " + ins + ";\n";
+  if (!wires.empty()) o += "  wire " + wires + ";\n";
+  for (auto& inst : nl.insts) {
+    o += "  " + lib.types[inst.type].name + " " + inst.name + " (";
+    bool first = true;
+    for (auto& pp : inst.in_pins) {
+      o += first ? "" : ", ";
+      first = false;
+      o += "." + pp.first + "(" + nl.net_names[pp.second] + ")";
+    }
+    for (auto& pp : inst.out_pins) {
+      o += first ? "" : ", ";
+      first = false;
+      o += "." + pp.first + "(" + nl.net_names[pp.second] + ")";
+    }
+    o += ");\n";
+  }
+  o += "endmodule\n";
+  return o;
+}
+
+static std::string fmt_delay(double v) {
+  char b[64];
+  if (v == (double)(long long)v) std::snprintf(b, sizeof b, "%lld", (long long)v);
+  else std::snprintf(b, sizeof b, "%.17g", v);
+  return b;
+}
+
+std::string sdf_text(const Netlist& nl, const Library& lib) {
+  std::string o;
+  o += "(DELAYFILE\n  (SDFVERSION \"3.0\")\n  (DESIGN \"" + nl.top + "\")\n  (TIMESCALE 1ps)\n";
+  for (auto& inst : nl.insts) {
+    o += "  (CELL\n    (CELLTYPE \"" + lib.types[inst.type].name + "\")\n    (INSTANCE " + inst.name +
+         ")\n    (DELAY\n      (ABSOLUTE\n";
+    for (auto& d : inst.delays) {
+      std::string in_pin, out_pin;
+      for (auto& pp : inst.in_pins)
+        if (pp.second == d.in_net) { in_pin = pp.first; break; }
+      for (auto& pp : inst.out_pins)
+        if (pp.second == d.out_net) { out_pin = pp.first; break; }
+      if (d.out_net <= -2) out_pin = inst.unknown_out[-2 - d.out_net];
+      std::string r = fmt_delay(d.rise), f = fmt_delay(d.fall);
+      o += "        (IOPATH ";
+      if (d.edge == 0) o += "(posedge " + in_pin + ")";
+      else if (d.edge == 1) o += "(negedge " + in_pin + ")";
+      else o += in_pin;
+      o += " " + out_pin + " (" + r + ":" + r + ":" + r + ") (" + f + ":" + f + ":" + f + "))\n";
+    }
+    o += "      )\n    )\n  )\n";
+  }
+  o += ")\n";
+  return o;
+}
+
+static std::string vcd_id(size_t i) {
+  std::string s;
+  do {
+    s += (char)(33 + i % 94);
+    i /= 94;
+  } while (i);
+  return s;
+}
+
+std::string vcd_text(const Stimulus& st) {
+  std::string o;
+  size_t total = 0;
+  for (auto& w : st.waves) total += w.size();
+  o.reserve(total * 12 + st.names.size() * 32 + 256);
+  o += "$date\n  synthetic\n$end\n$version\n  glsgen\n$end\n";
+  o += "$timescale " + std::to_string(st.timescale_ps) + "ps $end\n";
+  o += "$scope module " + st.scope + " $end\n";
+  for (size_t i = 0; i < st.names.size(); ++i)
+    o += "$var wire 1 " + vcd_id(i) + " " + st.names[i] + " $end\n";
+  o += "$upscope $end\n$enddefinitions $end\n";
+  // k-way merge of all signals by time
+  std::vector<size_t> pos(st.waves.size(), 0);
+  std::vector<std::pair<uint32_t, uint32_t>> heap;  // (time, signal)
+  for (size_t i = 0; i < st.waves.size(); ++i)
+    if (st.waves[i].size()) heap.push_back({st.waves[i].t[0], (uint32_t)i});
+  auto cmp = [](const std::pair<uint32_t, uint32_t>& a, const std::pair<uint32_t, uint32_t>& b) {
+    return a > b;
+  };
+  std::make_heap(heap.begin(), heap.end(), cmp);
+  long long last_t = -1;
+  static const char vc[4] = {'0', 'x', 'z', '1'};
+  char buf[32];
+  while (!heap.empty()) {
+    std::pop_heap(heap.begin(), heap.end(), cmp);
+    auto [t, sig] = heap.back();
+    heap.pop_back();
+    if ((long long)t != last_t) {
+      int k = std::snprintf(buf, sizeof buf, "#%u\n", t);
+      o.append(buf, (size_t)k);
+      last_t = t;
+    }
+    o += vc[st.waves[sig].v[pos[sig]] & 3];
+    o += vcd_id(sig);
+    o += '\n';
+    if (++pos[sig] < st.waves[sig].size()) {
+      heap.push_back({st.waves[sig].t[pos[sig]], sig});
+      std::push_heap(heap.begin(), heap.end(), cmp);
+    }
+  }
+  return o;
+}
+
+// ---------------------------------------------------------------- SAIF ----
+// Record layout of src/simulate.cpp:85-88,162-163,303-304,373-378 with the
+// header of src/simulate.cu:457-481; nets in lexicographic order.
+std::string saif_text(std::vector<SaifRecord> recs, uint64_t start, uint64_t end, int timescale_ps) {
+  std::sort(recs.begin(), recs.end(),
+            [](const SaifRecord& a, const SaifRecord& b) { return a.name < b.name; });
+  std::string o;
+  o.reserve(recs.size() * 64 + 128);
+  const char* unit = "ps";
+  long long res = timescale_ps;
+  if (timescale_ps % 1000000 == 0) { unit = "us"; res = timescale_ps / 1000000; }
+  else if (timescale_ps % 1000 == 0) { unit = "ns"; res = timescale_ps / 1000; }
+  char buf[160];
+  std::snprintf(buf, sizeof buf, "(TIMESCALE %lld %s)\n(DURATION %llu)\n(INSTANCE root\n   (NET\n",
+                res, unit, (unsigned long long)(end - start));
+  o += buf;
+  std::vector<const SaifRecord*> hier;
+  for (auto& r : recs) {
+    size_t lb = r.name.find('[');
+    if (lb != std::string::npos && lb + 1 < r.name.size() &&
+        !(r.name[lb + 1] >= '0' && r.name[lb + 1] <= '9')) {
+      hier.push_back(&r);
+      continue;
+    }
+    o += "      (" + r.name + "\n";
+    std::snprintf(buf, sizeof buf, "         (T0 %llu) (T1 %llu) (TX %llu)\n      )\n",
+                  (unsigned long long)r.t0, (unsigned long long)r.t1, (unsigned long long)r.tx);
+    o += buf;
+  }
+  o += "   )\n\n";
+  for (auto* r : hier) {
+    size_t lb = r->name.find('['), rb = r->name.find(']');
+    std::string inst = r->name.substr(0, lb);
+    std::string net = r->name.substr(lb + 1, rb - lb - 1);
+    o += "   (INSTANCE " + inst + "\n      (NET\n         (" + net + "\n";
+    std::snprintf(buf, sizeof buf,
+                  "            (T0 %llu) (T1 %llu) (TX %llu)\n         )\n      )\n   )\n",
+                  (unsigned long long)r->t0, (unsigned long long)r->t1, (unsigned long long)r->tx);
+    o += buf;
+  }
+  o += ")\n";
+  return o;
+}
+
+}  // namespace gls

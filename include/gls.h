@@ -1,0 +1,126 @@
+/* libgls — C ABI of the MI355X gate-level waveform simulator.
+ *
+ * The reference exposes no library interface: its boundary is the pair of
+ * command-line tools
+ *     process  vlib gv sdf inter.dat               (src/process.cpp:178-194)
+ *     simulate inter.dat vcd dumpon dumpoff saif    (src/simulate.cu:1786-1812)
+ * with the kernels called straight from main (src/simulate.cu:2027-3254).
+ * Each entry point below replaces one step of those tools; the file-level
+ * mapping is given per function.  All functions return GLS_OK (0) or a
+ * negative GLS_E_* status (gls_status.h) and never exit the process; the
+ * message of the last failure on the calling thread is gls_last_error().
+ *
+ * Handles are opaque.  All device memory is owned by the library.
+ */
+#ifndef GLS_H
+#define GLS_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#include "gls_status.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct gls_design gls_design; /* parsed + levelized netlist */
+typedef struct gls_stim gls_stim;     /* stimulus waveforms */
+typedef struct gls_ctx gls_ctx;       /* one device */
+typedef struct gls_run gls_run;       /* one prepared / finished simulation */
+
+/* Message of the last failing call on this thread ("" if none). */
+const char* gls_last_error(void);
+/* Library version string. */
+const char* gls_version(void);
+
+/* ------------------------------------------------------------ designs --- */
+/* Parse vlib + gv + SDF (PreProcess::parse_vlib/parse_gv/parse_sdf,
+ * src/PreProcess.cpp:42,650,1276) — the first half of `process`. */
+int gls_design_from_files(const char* vlib_path, const char* gv_path, const char* sdf_path,
+                          gls_design** out);
+/* Same, from in-memory texts. */
+int gls_design_from_texts(const char* vlib_text, const char* gv_text, const char* sdf_text,
+                          gls_design** out);
+/* Seeded synthetic netlist of one benchmark configuration (0, 1, 2) with the
+ * generic cell library; n_cells/n_levels/n_pis override when > 0. */
+int gls_design_generate(int config, uint64_t seed, int n_cells, int n_levels, int n_pis,
+                        gls_design** out);
+/* inter.dat writer / reader — replaces the Boost archive hand-off between
+ * `process` and `simulate` (src/process.cpp:581-608, src/simulate.cu:1835-1838). */
+int gls_design_save(const gls_design* d, const char* inter_path);
+int gls_design_load(const char* inter_path, gls_design** out);
+/* Write the design as vlib / gv / SDF text files (generator output). */
+int gls_design_write(const gls_design* d, const char* vlib_path, const char* gv_path,
+                     const char* sdf_path);
+typedef struct {
+  uint64_t n_instances, n_nets, n_inputs; /* netlist */
+  uint64_t n_gates, n_levels, n_pins;     /* compiled program (after levelization) */
+} gls_design_info;
+/* Counts; levelizes against the design's declared inputs. */
+int gls_design_info_get(const gls_design* d, gls_design_info* info);
+void gls_design_free(gls_design* d);
+
+/* ---------------------------------------------------------- stimulus --- */
+/* Parse a VCD (VCDParser::init/parse/process, src/VCDParser.cpp:244-717). */
+int gls_stim_from_vcd(const char* vcd_path, gls_stim** out);
+/* Seeded Poisson stimulus for the design's primary inputs: initial value
+ * U{0,1} at t = 0, mean toggle gap `mean_gap` ps, toggles in (0, t_end],
+ * a toggle goes to X with probability x_prob. */
+int gls_stim_generate(const gls_design* d, uint64_t seed, uint32_t t_end, double mean_gap,
+                      double x_prob, gls_stim** out);
+int gls_stim_write_vcd(const gls_stim* s, const char* vcd_path);
+/* Number of signals and total events. */
+int gls_stim_info(const gls_stim* s, uint64_t* n_signals, uint64_t* n_events);
+void gls_stim_free(gls_stim* s);
+
+/* ------------------------------------------------------------ device --- */
+int gls_ctx_create(int device, gls_ctx** out);
+void gls_ctx_destroy(gls_ctx* c);
+
+typedef struct {
+  uint64_t pool_bytes;   /* event pool size (0: 70% of free HBM) */
+  uint32_t seg_events;   /* target input events per time segment (0: 256) */
+  uint32_t keep_waves;   /* 1: keep waveforms for gls_run_waveform */
+} gls_sim_opts;
+
+/* Upload program + stimulus into HBM (the per-level H2D copies of
+ * src/simulate.cu:2434-2482 happen once here).  dumpon/dumpoff are the SAIF
+ * window (src/simulate.cu:1811-1812). */
+int gls_prepare(gls_ctx* c, const gls_design* d, const gls_stim* s, uint64_t dumpon,
+                uint64_t dumpoff, const gls_sim_opts* opts, gls_run** out);
+/* Run every level on the device (the level loop of src/simulate.cu:2027-3254
+ * and the SAIF reduction of src/simulate.cu:1563-1784).  May be called again
+ * on the same run; each call recomputes from the resident stimulus. */
+int gls_run_execute(gls_run* r);
+/* prepare + execute */
+int gls_simulate(gls_ctx* c, const gls_design* d, const gls_stim* s, uint64_t dumpon,
+                 uint64_t dumpoff, const gls_sim_opts* opts, gls_run** out);
+
+typedef struct {
+  uint64_t n_events;   /* gate events: distinct input-event times per output */
+  uint64_t in_words;   /* input waveform words read by gates */
+  uint64_t out_words;  /* output waveform words written */
+  uint64_t alg_bytes;  /* algorithmic HBM bytes of the level kernels */
+  uint64_t n_nets, n_gates, n_pins;
+  double kernel_ms;    /* device time of the level loop */
+  double wall_ms;      /* host wall time of the level loop */
+} gls_run_stats;
+int gls_run_stats_get(const gls_run* r, gls_run_stats* st);
+/* SAIF writer (DumpSaif, src/simulate.cu:113-481 / src/simulate.cpp:37-380). */
+int gls_run_write_saif(const gls_run* r, const char* saif_path);
+/* SAIF text into a caller buffer; *len receives the size (call with
+ * buf == NULL to query). */
+int gls_run_saif_text(const gls_run* r, char* buf, size_t cap, size_t* len);
+/* T0/T1/TX durations of one net. */
+int gls_run_net_saif(const gls_run* r, const char* net, uint64_t t0t1tx[3]);
+/* Waveform of one net as (time << 2 | value) words (needs keep_waves). */
+int gls_run_waveform(const gls_run* r, const char* net, uint32_t* words, size_t cap, size_t* n);
+/* Order-independent 64-bit hash over every net's waveform (needs keep_waves). */
+int gls_run_waves_hash(const gls_run* r, uint64_t* hash);
+void gls_run_free(gls_run* r);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
