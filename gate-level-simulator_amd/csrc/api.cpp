@@ -336,6 +336,8 @@ int gls_run_stats_get(const gls_run* r, gls_run_stats* st) {
   st->n_pins = j.n_pins;
   st->kernel_ms = j.kernel_ms;
   st->wall_ms = j.wall_ms;
+  st->eval_ms = j.eval_ms;
+  st->eval_launches = j.eval_launches;
   API_END
 }
 

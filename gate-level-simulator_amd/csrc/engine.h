@@ -21,6 +21,8 @@ struct JobStats {
   uint64_t alg_bytes = 0;       // algorithmic bytes of the level kernels
   double kernel_ms = 0;         // device time of the level loop (hipEvent)
   double wall_ms = 0;           // host wall time of the level loop
+  double eval_ms = 0;           // summed device time of the k_eval launches
+  uint64_t eval_launches = 0;
   uint64_t pool_words_used = 0;
 };
 

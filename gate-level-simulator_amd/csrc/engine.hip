@@ -191,6 +191,8 @@ void Job::run() {
   uint64_t* tot_scr = d_scal_ + 2;
   uint64_t* tot_out = d_scal_ + 3;
   unsigned long long* evc = (unsigned long long*)(d_scal_ + 4);
+  // HIP events around every k_eval launch, on the stream it runs on
+  std::vector<hipEvent_t> ev_eval;
   for (size_t l = 0; l + 1 < level_start_.size(); ++l) {
     uint32_t g0 = level_start_[l], ng = level_start_[l + 1] - g0;
     if (ng == 0) continue;
@@ -211,8 +213,15 @@ void Job::run() {
       throw GlsError(GLS_E_OOM, "event pool exhausted at level " + std::to_string(l) +
                                     " (raise the pool size or shorten the window)");
     HC(launch_segmap(ng, d_segbase_, d_gnseg_, d_segmap_, stream_));
+    hipEvent_t a, b;
+    HC(hipEventCreate(&a));
+    HC(hipEventCreate(&b));
+    ev_eval.push_back(a);
+    ev_eval.push_back(b);
+    HC(hipEventRecord(a, stream_));
     HC(launch_eval(gates, d_segbase_, d_scrbase_, d_gnseg_, d_segmap_, nsegs, d_woff_, d_wlen_,
                    d_pool_, d_luts_, d_scratch_, d_sums_, evc, stream_));
+    HC(hipEventRecord(b, stream_));
     HC(launch_gate_scan(gates, ng, d_segbase_, d_gnseg_, d_sums_, d_carry_, d_outlen_, d_saif_,
                         start_, end_, stream_));
     HC(launch_scan(d_outlen_, d_outoff_, ng, d_part_, tot_out, bump, stream_));
@@ -226,6 +235,15 @@ void Job::run() {
   HC(hipEventElapsedTime(&ms, e0, e1));
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
+  double eval_ms = 0;
+  for (size_t i = 0; i + 1 < ev_eval.size(); i += 2) {
+    float x = 0;
+    HC(hipEventElapsedTime(&x, ev_eval[i], ev_eval[i + 1]));
+    eval_ms += x;
+  }
+  for (hipEvent_t e : ev_eval) (void)hipEventDestroy(e);
+  stats_.eval_ms = eval_ms;
+  stats_.eval_launches = ev_eval.size() / 2;
   uint64_t h[8];
   HC(hipMemcpy(h, d_scal_, sizeof h, hipMemcpyDeviceToHost));
   auto w1 = std::chrono::steady_clock::now();

@@ -1,0 +1,272 @@
+"""Python binding of libgls (ctypes over the C ABI in include/gls.h).
+
+The simulator itself is native: parsing, levelization and the HIP kernels
+all live in libgls.so.  This module only marshals handles and buffers so the
+tests and the benchmark can drive the same entry points a C/C++ caller uses.
+Loading fails loudly when the library has not been built.
+"""
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(os.path.dirname(_HERE), "build", "libgls.so")
+
+GLS_OK = 0
+STATUS = {
+    -1: "GLS_E_ARG", -2: "GLS_E_FORMAT", -3: "GLS_E_OOM", -4: "GLS_E_DEVICE",
+    -5: "GLS_E_UNSUPPORTED", -6: "GLS_E_CANCELLED", -7: "GLS_E_RANGE",
+    -8: "GLS_E_STATE", -9: "GLS_E_IO", -10: "GLS_E_CYCLE",
+}
+
+
+class GlsError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"{STATUS.get(code, code)}: {msg}")
+        self.code = code
+        self.status = STATUS.get(code, str(code))
+
+
+class DesignInfo(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_uint64) for n in
+                ("n_instances", "n_nets", "n_inputs", "n_gates", "n_levels", "n_pins")]
+
+
+class SimOpts(ctypes.Structure):
+    _fields_ = [("pool_bytes", ctypes.c_uint64), ("seg_events", ctypes.c_uint32),
+                ("keep_waves", ctypes.c_uint32)]
+
+
+class RunStats(ctypes.Structure):
+    _fields_ = [("n_events", ctypes.c_uint64), ("in_words", ctypes.c_uint64),
+                ("out_words", ctypes.c_uint64), ("alg_bytes", ctypes.c_uint64),
+                ("n_nets", ctypes.c_uint64), ("n_gates", ctypes.c_uint64),
+                ("n_pins", ctypes.c_uint64), ("kernel_ms", ctypes.c_double),
+                ("wall_ms", ctypes.c_double), ("eval_ms", ctypes.c_double),
+                ("eval_launches", ctypes.c_uint64)]
+
+
+# Every symbol include/gls.h declares: (name, restype, argtypes)
+_P = ctypes.c_void_p
+_PP = ctypes.POINTER(ctypes.c_void_p)
+_S = ctypes.c_char_p
+_U64 = ctypes.c_uint64
+_I = ctypes.c_int
+SIGNATURES = [
+    ("gls_last_error", _S, []),
+    ("gls_version", _S, []),
+    ("gls_design_from_files", _I, [_S, _S, _S, _PP]),
+    ("gls_design_from_texts", _I, [_S, _S, _S, _PP]),
+    ("gls_design_generate", _I, [_I, _U64, _I, _I, _I, _PP]),
+    ("gls_design_save", _I, [_P, _S]),
+    ("gls_design_load", _I, [_S, _PP]),
+    ("gls_design_write", _I, [_P, _S, _S, _S]),
+    ("gls_design_info_get", _I, [_P, ctypes.POINTER(DesignInfo)]),
+    ("gls_design_free", None, [_P]),
+    ("gls_stim_from_vcd", _I, [_S, _PP]),
+    ("gls_stim_generate", _I, [_P, _U64, ctypes.c_uint32, ctypes.c_double, ctypes.c_double, _PP]),
+    ("gls_stim_write_vcd", _I, [_P, _S]),
+    ("gls_stim_info", _I, [_P, ctypes.POINTER(_U64), ctypes.POINTER(_U64)]),
+    ("gls_stim_free", None, [_P]),
+    ("gls_ctx_create", _I, [_I, _PP]),
+    ("gls_ctx_destroy", None, [_P]),
+    ("gls_prepare", _I, [_P, _P, _P, _U64, _U64, ctypes.POINTER(SimOpts), _PP]),
+    ("gls_run_execute", _I, [_P]),
+    ("gls_simulate", _I, [_P, _P, _P, _U64, _U64, ctypes.POINTER(SimOpts), _PP]),
+    ("gls_run_stats_get", _I, [_P, ctypes.POINTER(RunStats)]),
+    ("gls_run_write_saif", _I, [_P, _S]),
+    ("gls_run_saif_text", _I, [_P, ctypes.c_char_p, ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t)]),
+    ("gls_run_net_saif", _I, [_P, _S, ctypes.POINTER(_U64)]),
+    ("gls_run_waveform", _I, [_P, _S, ctypes.POINTER(ctypes.c_uint32), ctypes.c_size_t,
+                              ctypes.POINTER(ctypes.c_size_t)]),
+    ("gls_run_waves_hash", _I, [_P, ctypes.POINTER(_U64)]),
+    ("gls_run_free", None, [_P]),
+]
+
+_lib = None
+
+
+def load(path=None):
+    """Load libgls.so (raises OSError if it was not built)."""
+    global _lib
+    if _lib is not None and path is None:
+        return _lib
+    p = path or LIB_PATH
+    if not os.path.exists(p):
+        raise OSError(f"libgls.so not built at {p}; run __graft_entry__.build()")
+    lib = ctypes.CDLL(p)
+    for name, res, args in SIGNATURES:
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    if path is None:
+        _lib = lib
+    return lib
+
+
+def _b(s):
+    return None if s is None else (s.encode() if isinstance(s, str) else s)
+
+
+def _check(rc):
+    if rc != GLS_OK:
+        raise GlsError(rc, load().gls_last_error().decode(errors="replace"))
+
+
+class _Handle:
+    _free = None
+
+    def __init__(self, ptr):
+        self._ptr = ptr
+
+    @property
+    def ptr(self):
+        if not self._ptr:
+            raise GlsError(-8, "handle already freed")
+        return self._ptr
+
+    def close(self):
+        if self._ptr and self._free:
+            getattr(load(), self._free)(self._ptr)
+        self._ptr = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+
+class Design(_Handle):
+    _free = "gls_design_free"
+
+    @classmethod
+    def from_files(cls, vlib, gv, sdf):
+        h = ctypes.c_void_p()
+        _check(load().gls_design_from_files(_b(vlib), _b(gv), _b(sdf), ctypes.byref(h)))
+        return cls(h.value)
+
+    @classmethod
+    def from_texts(cls, vlib, gv, sdf):
+        h = ctypes.c_void_p()
+        _check(load().gls_design_from_texts(_b(vlib), _b(gv), _b(sdf), ctypes.byref(h)))
+        return cls(h.value)
+
+    @classmethod
+    def generate(cls, config, seed=0, n_cells=0, n_levels=0, n_pis=0):
+        h = ctypes.c_void_p()
+        _check(load().gls_design_generate(config, seed, n_cells, n_levels, n_pis, ctypes.byref(h)))
+        return cls(h.value)
+
+    @classmethod
+    def load_inter(cls, path):
+        h = ctypes.c_void_p()
+        _check(load().gls_design_load(_b(path), ctypes.byref(h)))
+        return cls(h.value)
+
+    def save_inter(self, path):
+        _check(load().gls_design_save(self.ptr, _b(path)))
+
+    def write(self, vlib, gv, sdf):
+        _check(load().gls_design_write(self.ptr, _b(vlib), _b(gv), _b(sdf)))
+
+    def info(self):
+        i = DesignInfo()
+        _check(load().gls_design_info_get(self.ptr, ctypes.byref(i)))
+        return {n: getattr(i, n) for n, _ in DesignInfo._fields_}
+
+
+class Stimulus(_Handle):
+    _free = "gls_stim_free"
+
+    @classmethod
+    def from_vcd(cls, path):
+        h = ctypes.c_void_p()
+        _check(load().gls_stim_from_vcd(_b(path), ctypes.byref(h)))
+        return cls(h.value)
+
+    @classmethod
+    def generate(cls, design, seed, t_end, mean_gap, x_prob=0.01):
+        h = ctypes.c_void_p()
+        _check(load().gls_stim_generate(design.ptr, seed, t_end, mean_gap, x_prob, ctypes.byref(h)))
+        return cls(h.value)
+
+    def write_vcd(self, path):
+        _check(load().gls_stim_write_vcd(self.ptr, _b(path)))
+
+    def info(self):
+        a, b = _U64(), _U64()
+        _check(load().gls_stim_info(self.ptr, ctypes.byref(a), ctypes.byref(b)))
+        return {"n_signals": a.value, "n_events": b.value}
+
+
+class Run(_Handle):
+    _free = "gls_run_free"
+
+    def execute(self):
+        _check(load().gls_run_execute(self.ptr))
+        return self
+
+    def stats(self):
+        s = RunStats()
+        _check(load().gls_run_stats_get(self.ptr, ctypes.byref(s)))
+        return {n: getattr(s, n) for n, _ in RunStats._fields_}
+
+    def write_saif(self, path):
+        _check(load().gls_run_write_saif(self.ptr, _b(path)))
+
+    def saif_text(self):
+        n = ctypes.c_size_t()
+        _check(load().gls_run_saif_text(self.ptr, None, 0, ctypes.byref(n)))
+        buf = ctypes.create_string_buffer(n.value)
+        _check(load().gls_run_saif_text(self.ptr, buf, n.value, ctypes.byref(n)))
+        return buf.raw[:n.value].decode()
+
+    def net_saif(self, net):
+        v = (_U64 * 3)()
+        _check(load().gls_run_net_saif(self.ptr, _b(net), v))
+        return tuple(v)
+
+    def waveform(self, net):
+        n = ctypes.c_size_t()
+        _check(load().gls_run_waveform(self.ptr, _b(net), None, 0, ctypes.byref(n)))
+        buf = (ctypes.c_uint32 * max(1, n.value))()
+        _check(load().gls_run_waveform(self.ptr, _b(net), buf, n.value, ctypes.byref(n)))
+        return [(w >> 2, w & 3) for w in buf[:n.value]]
+
+    def waves_hash(self):
+        h = _U64()
+        _check(load().gls_run_waves_hash(self.ptr, ctypes.byref(h)))
+        return h.value
+
+
+class Context(_Handle):
+    _free = "gls_ctx_destroy"
+
+    def __init__(self, device=0):
+        h = ctypes.c_void_p()
+        _check(load().gls_ctx_create(device, ctypes.byref(h)))
+        super().__init__(h.value)
+
+    @staticmethod
+    def _opts(pool_bytes, seg_events, keep_waves):
+        return SimOpts(pool_bytes, seg_events, 1 if keep_waves else 0)
+
+    def prepare(self, design, stim, dumpon, dumpoff, pool_bytes=0, seg_events=0, keep_waves=False):
+        h = ctypes.c_void_p()
+        o = self._opts(pool_bytes, seg_events, keep_waves)
+        _check(load().gls_prepare(self.ptr, design.ptr, stim.ptr, dumpon, dumpoff, ctypes.byref(o),
+                                  ctypes.byref(h)))
+        return Run(h.value)
+
+    def simulate(self, design, stim, dumpon, dumpoff, pool_bytes=0, seg_events=0, keep_waves=False):
+        h = ctypes.c_void_p()
+        o = self._opts(pool_bytes, seg_events, keep_waves)
+        _check(load().gls_simulate(self.ptr, design.ptr, stim.ptr, dumpon, dumpoff, ctypes.byref(o),
+                                   ctypes.byref(h)))
+        return Run(h.value)

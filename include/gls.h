@@ -105,6 +105,8 @@ typedef struct {
   uint64_t n_nets, n_gates, n_pins;
   double kernel_ms;    /* device time of the level loop */
   double wall_ms;      /* host wall time of the level loop */
+  double eval_ms;      /* summed device time of the gate-evaluation kernel */
+  uint64_t eval_launches;
 } gls_run_stats;
 int gls_run_stats_get(const gls_run* r, gls_run_stats* st);
 /* SAIF writer (DumpSaif, src/simulate.cu:113-481 / src/simulate.cpp:37-380). */

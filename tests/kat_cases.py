@@ -1,0 +1,248 @@
+"""Known-answer cases: tiny netlists whose output waveforms are worked out by
+hand from the reference's event rules (src/simulate.cpp:919-1201).  Each case
+is (name, gv instances, SDF IOPATH lines, stimulus, expected waveforms).
+
+Values: 0, 1, 'x', 'z'; waveforms are lists of (time, value) with value codes
+0 -> 0, X -> 1, Z -> 2, 1 -> 3.
+"""
+
+VLIB = """`timescale 1ps/1ps
+primitive udp_mux2 (z, i0, i1, s);
+   output z;
+   input  i0, i1, s;
+   table
+    1  ?  0 :  1;
+    0  ?  0 :  0;
+    ?  1  1 :  1;
+    ?  0  1 :  0;
+    0  0  x :  0;
+    1  1  x :  1;
+   endtable
+endprimitive
+module GEN_INV_D1 (i,zn);
+  input i;
+  output zn;
+  not (zn, i);
+endmodule
+module GEN_BUF_D1 (i,z);
+  input i;
+  output z;
+  buf (z, i);
+endmodule
+module GEN_AND2_D1 (a1,a2,z);
+  input a1;
+  input a2;
+  output z;
+  and (z, a1, a2);
+endmodule
+module GEN_XOR2_D1 (a1,a2,z);
+  input a1;
+  input a2;
+  output z;
+  xor (z, a1, a2);
+endmodule
+module GEN_MUX2_D1 (i0,i1,s,z);
+  input i0;
+  input i1;
+  input s;
+  output z;
+  udp_mux2 udpi0 (z, i0, i1, s);
+endmodule
+module GEN_AOI21_D1 (a1,a2,b,zn);
+  input a1;
+  input a2;
+  input b;
+  output zn;
+  wire net0, net1;
+  and (net0, a1, a2);
+  or (net1, net0, b);
+  not (zn, net1);
+endmodule
+module GEN_TIE_D1 (a,z);
+  input a;
+  output z;
+  and (z, a, 1'b1);
+endmodule
+"""
+
+C = {"0": 0, "x": 1, "z": 2, "1": 3}
+X, Z = 1, 2
+
+
+def V(c):
+    return C[str(c)]
+
+
+CASES = [
+    # INV, rise 10 / fall 20: every input change gives one output change.
+    dict(name="inv_basic",
+         insts=["GEN_INV_D1 U0 (.i(a), .zn(y));"],
+         sdf={"U0": ["(IOPATH i zn (10:10:10) (20:20:20))"]},
+         stim={"a": [(0, 0), (100, 1), (200, 0)]},
+         expect={"y": [(0, X), (10, 3), (120, 0), (210, 3)]}),
+    # BUF rise 50 / fall 5: a 20 ps high pulse is swallowed (fall at 125
+    # pre-empts the rise scheduled at 150; the value equals the queue tail).
+    dict(name="buf_preempt",
+         insts=["GEN_BUF_D1 U0 (.i(a), .z(y));"],
+         sdf={"U0": ["(IOPATH i z (50:50:50) (5:5:5))"]},
+         stim={"a": [(0, 0), (100, 1), (120, 0)]},
+         expect={"y": [(0, X), (5, 0)]}),
+    # BUF rise 5 / fall 50: the same pulse passes.
+    dict(name="buf_pass",
+         insts=["GEN_BUF_D1 U0 (.i(a), .z(y));"],
+         sdf={"U0": ["(IOPATH i z (5:5:5) (50:50:50))"]},
+         stim={"a": [(0, 0), (100, 1), (120, 0)]},
+         expect={"y": [(0, X), (50, 0), (105, 3), (170, 0)]}),
+    # AND2, both inputs switch together: delay = min over the related IOPATHs.
+    dict(name="and2_min",
+         insts=["GEN_AND2_D1 U0 (.a1(a), .a2(b), .z(y));"],
+         sdf={"U0": ["(IOPATH a1 z (10:10:10) (30:30:30))", "(IOPATH a2 z (20:20:20) (15:15:15))"]},
+         stim={"a": [(0, 0), (100, 1)], "b": [(0, 0), (100, 1)]},
+         expect={"y": [(0, X), (15, 0), (110, 3)]}),
+    # AND2 with an X on one input while the other is 0: no output event;
+    # then the other input rises: X out (rise class from 0), back to 1.
+    dict(name="and2_x",
+         insts=["GEN_AND2_D1 U0 (.a1(a), .a2(b), .z(y));"],
+         sdf={"U0": ["(IOPATH a1 z (10:10:10) (30:30:30))", "(IOPATH a2 z (20:20:20) (15:15:15))"]},
+         stim={"a": [(0, 0), (50, "x"), (200, 1)], "b": [(0, 0), (100, 1)]},
+         # t0: f=0 fall, num_in=2 -> min(30,15)=15.  t50: a->X, b=0: f=0, no event.
+         # t100: b->1, a=X: f=X, 0->X is class 1 (rise), num_in=1, one IOPATH -> 20 -> (120,X).
+         # t200: a->1: f=1, X->1 rise, num_in=1 -> a1 rise 10 -> (210,1).
+         expect={"y": [(0, X), (15, 0), (120, X), (210, 3)]}),
+    # XOR2 with posedge/negedge IOPATHs per input.
+    dict(name="xor2_edges",
+         insts=["GEN_XOR2_D1 U0 (.a1(a), .a2(b), .z(y));"],
+         sdf={"U0": ["(IOPATH (posedge a1) z (10:10:10) (11:11:11))",
+                     "(IOPATH (negedge a1) z (12:12:12) (13:13:13))",
+                     "(IOPATH (posedge a2) z (20:20:20) (21:21:21))",
+                     "(IOPATH (negedge a2) z (22:22:22) (23:23:23))"]},
+         stim={"a": [(0, 0), (100, 1), (300, 0)], "b": [(0, 1), (200, 0)]},
+         # t0: f=1 rise; a: X->0 is a fall -> negedge a1 (rise 12); b: X->1 rise -> posedge a2 (20); min 12
+         # t100: a 0->1 (posedge) f=0 fall -> 11.  t200: b 1->0 (negedge) f=1 rise -> 22.
+         # t300: a 1->0 (negedge) f=0 fall -> 13.
+         expect={"y": [(0, X), (12, 3), (111, 0), (222, 3), (313, 0)]}),
+    # MUX2 select going X with different data: output X; an X select edge
+    # counts as posedge.
+    dict(name="mux2_xsel",
+         insts=["GEN_MUX2_D1 U0 (.i0(d0), .i1(d1), .s(s), .z(y));"],
+         sdf={"U0": ["(IOPATH i0 z (30:30:30) (31:31:31))",
+                     "(IOPATH i1 z (40:40:40) (41:41:41))",
+                     "(IOPATH (posedge s) z (7:7:7) (8:8:8))",
+                     "(IOPATH (negedge s) z (9:9:9) (10:10:10))"]},
+         stim={"d0": [(0, 0)], "d1": [(0, 1)], "s": [(0, 0), (100, "x"), (200, 1), (300, "x")]},
+         # t0: f=d0=0, class fall, num_in=3: i0 31, i1 41, s X->0 fall -> negedge fall 10 -> 10
+         # t100: s 0->X: f=X, class rise (0->X); num_in=1, two IOPATHs, input edge 0->X = rise -> posedge rise 7
+         # t200: s X->1: f=1, class rise; input edge X->1 rise -> posedge rise 7 -> 207
+         # t300: s 1->X: f=X, class fall (1->X); input edge 1->X fall -> negedge fall 10 -> 310
+         expect={"y": [(0, X), (10, 0), (107, X), (207, 3), (310, X)]}),
+    # MUX2 with select X at t = 0 and equal data: f = data; the select's
+    # X->X "edge" is regarded as posedge (num_in = 3, min over i0/i1/posedge s).
+    dict(name="mux2_x_at_zero",
+         insts=["GEN_MUX2_D1 U0 (.i0(d0), .i1(d1), .s(s), .z(y));"],
+         sdf={"U0": ["(IOPATH i0 z (30:30:30) (31:31:31))",
+                     "(IOPATH i1 z (40:40:40) (41:41:41))",
+                     "(IOPATH (posedge s) z (7:7:7) (8:8:8))",
+                     "(IOPATH (negedge s) z (9:9:9) (10:10:10))"]},
+         stim={"d0": [(0, 1)], "d1": [(0, 1)], "s": [(0, "x")]},
+         expect={"y": [(0, X), (7, 3)]}),
+    # AOI21 chain with internal wires; an input with no IOPATH uses delay 1.
+    dict(name="aoi21_default_delay",
+         insts=["GEN_AOI21_D1 U0 (.a1(a), .a2(b), .b(c), .zn(y));"],
+         sdf={"U0": ["(IOPATH a1 zn (10:10:10) (20:20:20))", "(IOPATH a2 zn (11:11:11) (21:21:21))"]},
+         stim={"a": [(0, 1)], "b": [(0, 1)], "c": [(0, 0), (100, 1), (200, 0)]},
+         # t0: a1&a2=1 -> zn=0, class fall; num_in=3, related a1,a2 -> min(20,21)=20
+         # t100: c->1: zn stays 0: no event.  t200: no event.
+         expect={"y": [(0, X), (20, 0)]}),
+    dict(name="aoi21_no_iopath",
+         insts=["GEN_AOI21_D1 U0 (.a1(a), .a2(b), .b(c), .zn(y));"],
+         sdf={"U0": ["(IOPATH a1 zn (10:10:10) (20:20:20))"]},
+         stim={"a": [(0, 0)], "b": [(0, 0)], "c": [(0, 0), (100, 1), (200, 0)]},
+         # t0: zn = 1 rise, num_in=3, related {a1} -> 10.  t100: c->1: zn=0 fall,
+         # num_in=1, no related IOPATH -> delay 1 -> (101,0).  t200: c->0 -> (201,1).
+         expect={"y": [(0, X), (10, 3), (101, 0), (201, 3)]}),
+    # Cell with a supply tie inside (and with 1'b1): z = a.
+    dict(name="tie_supply",
+         insts=["GEN_TIE_D1 U0 (.a(a), .z(y));"],
+         sdf={"U0": ["(IOPATH a z (3:3:3) (4:4:4))"]},
+         stim={"a": [(0, 1), (50, 0)]},
+         expect={"y": [(0, X), (3, 3), (54, 0)]}),
+    # Two levels: INV feeding BUF; the second level sees the first's (0,X).
+    dict(name="two_levels",
+         insts=["GEN_INV_D1 U0 (.i(a), .zn(m));", "GEN_BUF_D1 U1 (.i(m), .z(y));"],
+         sdf={"U0": ["(IOPATH i zn (10:10:10) (20:20:20))"], "U1": ["(IOPATH i z (5:5:5) (6:6:6))"]},
+         stim={"a": [(0, 0), (100, 1)]},
+         # m: (0,X),(10,1),(120,0).  y: t0 f=X (buf of X) no event; t10: f=1 rise 5 -> 15;
+         # t120: f=0 fall 6 -> 126
+         expect={"m": [(0, X), (10, 3), (120, 0)], "y": [(0, X), (15, 3), (126, 0)]}),
+    # Gate tied to a constant through the netlist (1'b0 on an input pin).
+    dict(name="const_input",
+         insts=["GEN_AND2_D1 U0 (.a1(a), .a2(1'b1), .z(y));"],
+         sdf={"U0": ["(IOPATH a1 z (10:10:10) (20:20:20))", "(IOPATH a2 z (1:1:1) (2:2:2))"]},
+         stim={"a": [(0, 0), (100, 1)]},
+         # t0: a and const both have entries -> num_in=2, f=0 fall: min(20, 2)=2
+         # t100: f=1 rise, num_in=1 -> a1 rise 10 -> 110
+         expect={"y": [(0, X), (2, 0), (110, 3)]}),
+]
+
+
+def vcd_id(i):
+    s = ""
+    while True:
+        s += chr(33 + i % 94)
+        i //= 94
+        if i == 0:
+            return s
+
+
+def texts(case):
+    """Return (vlib, gv, sdf, vcd) texts of a case."""
+    nets_in = list(case["stim"].keys())
+    wires = set()
+    for inst in case["insts"]:
+        body = inst[inst.index("(") + 1:]
+        for part in body.split(".")[1:]:
+            net = part[part.index("(") + 1:part.index(")")]
+            if net not in nets_in and not net.startswith("1'b"):
+                wires.add(net)
+    gv = "module top ( );\n"
+    gv += "  input " + ", ".join(nets_in) + ";\n"
+    if wires:
+        gv += "  wire " + ", ".join(sorted(wires)) + ";\n"
+    for inst in case["insts"]:
+        gv += "  " + inst + "\n"
+    gv += "endmodule\n"
+    sdf = '(DELAYFILE\n  (SDFVERSION "3.0")\n  (TIMESCALE 1ps)\n'
+    for iname, lines in case["sdf"].items():
+        sdf += f"  (CELL\n    (CELLTYPE \"X\")\n    (INSTANCE {iname})\n    (DELAY\n      (ABSOLUTE\n"
+        for l in lines:
+            sdf += "        " + l + "\n"
+        sdf += "      )\n    )\n  )\n"
+    sdf += ")\n"
+    vcd = "$timescale 1ps $end\n$scope module top $end\n"
+    ids = {}
+    for i, n in enumerate(nets_in):
+        ids[n] = vcd_id(i)
+        vcd += f"$var wire 1 {ids[n]} {n} $end\n"
+    vcd += "$upscope $end\n$enddefinitions $end\n"
+    events = []
+    for n, w in case["stim"].items():
+        for t, v in w:
+            events.append((t, n, str(v)))
+    events.sort(key=lambda e: (e[0], nets_in.index(e[1])))
+    last = None
+    for t, n, v in events:
+        if t != last:
+            vcd += f"#{t}\n"
+            last = t
+        vcd += f"{v}{ids[n]}\n"
+    return VLIB, gv, sdf, vcd
+
+
+def write_case(case, d):
+    import os
+    vlib, gv, sdf, vcd = texts(case)
+    paths = [os.path.join(d, n) for n in ("lib.vlib", "top.gv", "top.sdf", "stim.vcd")]
+    for p, t in zip(paths, (vlib, gv, sdf, vcd)):
+        with open(p, "w") as f:
+            f.write(t)
+    return paths

@@ -1,0 +1,134 @@
+"""Parity of the HIP path (through the C ABI) with the oracle.
+
+Bit-exact bar: every gate-output waveform (hash over all nets, and the
+waveforms themselves for the known-answer cases) and the SAIF text must be
+identical to the oracle's on the same generated inputs.
+"""
+import json
+import os
+import random
+import subprocess
+import sys
+
+import pytest
+
+import gls
+import golden_cases
+import kat_cases
+import oracle_bind
+
+pytestmark = pytest.mark.gpu
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+GOLDEN = json.load(open(os.path.join(ROOT, "tests", "golden", "oracle_golden.json")))
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = gls.Context(0)
+    yield c
+    c.close()
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.mark.parametrize("case", kat_cases.CASES, ids=[c["name"] for c in kat_cases.CASES])
+def test_known_answers(ctx, case):
+    say("kat", case["name"])
+    vlib, gv, sdf, vcd = kat_cases.texts(case)
+    d = gls.Design.from_texts(vlib, gv, sdf)
+    import tempfile
+    with tempfile.TemporaryDirectory() as t:
+        p = os.path.join(t, "s.vcd")
+        open(p, "w").write(vcd)
+        s = gls.Stimulus.from_vcd(p)
+        r = ctx.simulate(d, s, 0, 1000, keep_waves=True)
+    for net, want in case["expect"].items():
+        assert r.waveform(net) == want, net
+
+
+@pytest.mark.parametrize("seed", [1, 2, 3, 4, 5, 6])
+def test_random_designs_against_oracle(ctx, seed, tmp_path):
+    rng = random.Random(seed)
+    cfg = seed % 2
+    cells = rng.randint(50, 400)
+    levels = rng.randint(2, 25)
+    pis = rng.randint(4, 40)
+    t_end = rng.randint(2000, 60000)
+    gap = rng.choice([15.0, 40.0, 200.0, 900.0])
+    say("random", dict(seed=seed, cfg=cfg, cells=cells, levels=levels, pis=pis, t_end=t_end, gap=gap))
+    d = gls.Design.generate(cfg, 1000 + seed, cells, levels, pis)
+    s = gls.Stimulus.generate(d, 77 + seed, t_end, gap, 0.05)
+    paths = [str(tmp_path / n) for n in ("lib.vlib", "n.gv", "n.sdf", "s.vcd")]
+    d.write(*paths[:3])
+    s.write_vcd(paths[3])
+    start, end = rng.randint(0, t_end // 3), t_end + rng.randint(1, 500)
+    want = oracle_bind.run(*paths, start, end, str(tmp_path))
+    for seg in (0, 3, 17):
+        r = ctx.simulate(d, s, start, end, seg_events=seg, keep_waves=True)
+        assert r.stats()["n_events"] == want["n_events"]
+        assert r.waves_hash() == want["waves_hash"], seg
+        assert r.saif_text() == want["saif"], seg
+
+
+@pytest.mark.parametrize("name", ["c0_small", "c1_small", "c0_full"])
+def test_golden_workloads(ctx, name):
+    say("golden", name)
+    d, s = golden_cases.make(name)
+    start, end = golden_cases.CASES[name]["window"]
+    r = ctx.simulate(d, s, start, end, keep_waves=True)
+    g = GOLDEN[name]
+    assert r.stats()["n_events"] == g["n_events"]
+    assert "%016x" % r.waves_hash() == g["waves_hash"]
+    assert oracle_bind.sha256(r.saif_text()) == g["saif_sha256"]
+
+
+def test_segment_size_invariance(ctx):
+    d, s = golden_cases.make("c1_small")
+    start, end = golden_cases.CASES["c1_small"]["window"]
+    ref = None
+    for seg in (1, 8, 64, 256, 5000, 1 << 20):
+        say("segments", seg)
+        r = ctx.simulate(d, s, start, end, seg_events=seg, keep_waves=True)
+        cur = (r.waves_hash(), r.saif_text())
+        if ref is None:
+            ref = cur
+        assert cur == ref, seg
+    assert "%016x" % ref[0] == GOLDEN["c1_small"]["waves_hash"]
+
+
+def test_cli_dropin(tmp_path):
+    say("cli")
+    paths = golden_cases.write_inputs("c0_small", str(tmp_path))
+    b = os.path.join(ROOT, "gate-level-simulator_amd", "build")
+    inter = str(tmp_path / "inter.dat")
+    saif = str(tmp_path / "out.saif")
+    subprocess.run([os.path.join(b, "process"), paths[0], paths[1], paths[2], inter], check=True,
+                   timeout=120)
+    start, end = golden_cases.CASES["c0_small"]["window"]
+    subprocess.run([os.path.join(b, "simulate"), inter, paths[3], str(start), str(end), saif],
+                   check=True, timeout=120)
+    assert oracle_bind.sha256(open(saif).read()) == GOLDEN["c0_small"]["saif_sha256"]
+
+
+def test_repeated_execute_is_identical(ctx):
+    d, s = golden_cases.make("c0_small")
+    r = ctx.prepare(d, s, 0, 200001, keep_waves=True)
+    hashes = set()
+    for _ in range(3):
+        r.execute()
+        hashes.add(r.waves_hash())
+    assert hashes == {int(GOLDEN["c0_small"]["waves_hash"], 16)}
+
+
+def test_pool_exhaustion_is_reported(ctx):
+    # the pool is clamped to the stimulus plus 1024 words: level 0 cannot fit
+    d, s = golden_cases.make("c0_small")
+    with pytest.raises(gls.GlsError) as e:
+        ctx.simulate(d, s, 0, 200001, pool_bytes=4096)
+    assert e.value.status == "GLS_E_OOM"
+    # the context stays usable after the failure
+    r = ctx.simulate(d, s, 0, 200001, keep_waves=True)
+    assert "%016x" % r.waves_hash() == GOLDEN["c0_small"]["waves_hash"]
