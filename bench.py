@@ -64,6 +64,7 @@ def main():
     ap.add_argument("--cells", type=int, default=200000)
     ap.add_argument("--t-end", type=int, default=20000000)
     ap.add_argument("--seg-events", type=int, default=0)
+    ap.add_argument("--path", choices=["fused", "two-pass"], default="fused")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-events", type=int, default=1500000)
     args = ap.parse_args()
@@ -85,7 +86,8 @@ def main():
     log(f"bench: generated {info['n_instances']} cells / {info['n_levels']} levels, "
         f"{s.info()['n_events']} stimulus events in {time.time() - t0:.1f}s")
     ctx = gls.Context(0)
-    run = ctx.prepare(d, s, 0, args.t_end + 1, seg_events=args.seg_events)
+    path = gls.PATH_FUSED if args.path == "fused" else gls.PATH_TWO_PASS
+    run = ctx.prepare(d, s, 0, args.t_end + 1, seg_events=args.seg_events, path=path)
     log("bench: prepared (program + stimulus resident in HBM)")
     for i in range(args.warmup):
         run.execute()
@@ -128,7 +130,7 @@ def main():
                                "mean toggle gap 500 ps, Pareto(1.5) fanout; plain per-level HIP path",
                    "cells": info["n_instances"], "levels": info["n_levels"], "inputs": info["n_inputs"],
                    "t_end_ps": args.t_end, "mean_gap_ps": 500, "seed": "0x5EED0001",
-                   "path": "plain per-level", "parallelism": "single GPU"},
+                   "path": "plain per-level (" + args.path + ")", "parallelism": "single GPU"},
         "gate_events": st["n_events"],
         "alg_bytes": st["alg_bytes"],
         "device_ms_per_step": round(sum(x["kernel_ms"] for x in stats) / len(stats), 3),

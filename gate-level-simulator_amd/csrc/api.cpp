@@ -294,6 +294,10 @@ int gls_prepare(gls_ctx* c, const gls_design* d, const gls_stim* s, uint64_t dum
   r->keep_waves = opts && opts->keep_waves;
   r->job = std::make_unique<Job>(c->device, c->stream);
   if (opts && opts->seg_events) r->job->seg_target = opts->seg_events;
+  if (opts) {
+    if (opts->path > GLS_PATH_TWO_PASS) throw GlsError(GLS_E_ARG, "unknown path");
+    r->job->mode = (int)opts->path;
+  }
   r->job->prepare(r->pg, s->st, r->stim_nets, dumpon, dumpoff, opts ? opts->pool_bytes / 4 : 0);
   *out = r.release();
   API_END

@@ -23,6 +23,48 @@ struct alignas(16) GateDev {
 };
 static_assert(sizeof(GateDev) == 80, "GateDev layout");
 
+// Gate record of the fused level kernel, 160 bytes.  Delays are folded into
+// two lookup tables per input j, indexed by the output edge class s (1..4)
+// and by whether input j fell at this time:
+//   tmin[j][s-1][fall]: contribution of j to the minimum taken when several
+//                       inputs have an entry (0xFFFF: no related IOPATH),
+//   tone[j][s-1][fall]: the delay when j is the only input with an entry.
+struct alignas(16) GateRec {
+  uint32_t in_net[4];
+  uint32_t out_net;
+  uint16_t lut;
+  uint8_t k;
+  uint8_t pad0;
+  uint16_t dmax;
+  uint16_t pad1[3];
+  uint16_t tmin[4][4][2];
+  uint16_t tone[4][4][2];
+};
+static_assert(sizeof(GateRec) == 160, "GateRec layout");
+
+// Arguments of the fused level kernel.
+struct LevelArgs {
+  const GateRec* gates;       // gates of this level
+  const uint32_t* tilemap;    // tile -> level-local gate index
+  const uint64_t* tile_base;  // per gate: first tile
+  const uint32_t* gnseg;      // per gate: segments
+  const uint64_t* out_base;   // per gate: output region (pool word offset)
+  uint64_t ntiles;
+  uint64_t* woff;
+  uint32_t* wlen;
+  uint32_t* pool;
+  const uint8_t* luts;
+  unsigned long long* tstate;  // 2 words per tile: aggregate, inclusive prefix
+  unsigned int* ticket;
+  uint32_t* ovf;               // stacks of tiles that do not fit LDS
+  unsigned long long ovf_cap;
+  unsigned long long* ovf_used;
+  unsigned long long* saif;
+  unsigned long long* ev_count;
+  unsigned int* err;
+  unsigned long long start, end;
+};
+
 // Per time-segment summary written by the evaluation kernel.
 struct SegSum {
   uint64_t scr_off;   // scratch offset of the segment's kept candidates

@@ -44,10 +44,13 @@ class Job {
   void fetch_waves(std::vector<uint64_t>& off, std::vector<uint32_t>& len,
                    std::vector<uint32_t>& pool_words) const;
   const JobStats& stats() const { return stats_; }
-  uint32_t seg_target = 256;
+  uint32_t seg_target = 0;  // input events per time segment (0: mode default)
+  int mode = 0;             // 0: fused LDS-tiled level kernel, 1: two-pass kernels
 
  private:
   void ensure(void** p, size_t& cap, size_t need);
+  void run_fused(std::vector<hipEvent_t>& ev_eval);
+  void run_two_pass(std::vector<hipEvent_t>& ev_eval);
   int device_;
   hipStream_t stream_;
   uint64_t start_ = 0, end_ = 0;
@@ -57,6 +60,7 @@ class Job {
   uint64_t pool_words_ = 0, stim_words_ = 0;
   // device buffers
   GateDev* d_gates_ = nullptr;
+  GateRec* d_grec_ = nullptr;
   uint8_t* d_luts_ = nullptr;
   uint64_t* d_woff_ = nullptr;
   uint32_t* d_wlen_ = nullptr;
@@ -73,6 +77,14 @@ class Job {
   SegSum* d_sums_ = nullptr;
   SegCarry* d_carry_ = nullptr;
   uint32_t* d_scratch_ = nullptr;
+  // fused path
+  uint64_t *d_ntile_ = nullptr, *d_bound_ = nullptr, *d_tilebase_ = nullptr;
+  size_t cap_tilemap_ = 0, cap_tstate_ = 0;
+  uint32_t* d_tilemap_ = nullptr;
+  unsigned long long* d_tstate_ = nullptr;
+  uint32_t* d_ovf_ = nullptr;
+  uint64_t ovf_words_ = 0;
+  int level_blocks_ = 0;
   std::vector<uint64_t> h_woff_init_;
   std::vector<uint32_t> h_wlen_init_;
   JobStats stats_;

@@ -8,7 +8,7 @@
 
 namespace gls {
 
-// launch helpers (kernels.hip)
+// launch helpers (kernels.hip: two-pass path and scans)
 hipError_t launch_scan(const uint64_t* in, uint64_t* out, uint64_t n, uint64_t* part,
                        uint64_t* total, const uint64_t* base, hipStream_t st);
 uint64_t scan_part_size(uint64_t n);
@@ -35,6 +35,16 @@ hipError_t launch_saif_nets(const uint32_t* nets, uint32_t count, const uint64_t
                             const uint32_t* wlen, const uint32_t* pool, unsigned long long* saif,
                             uint64_t start, uint64_t end, hipStream_t st);
 
+// level.hip: fused path
+hipError_t launch_plan2(const GateRec* gates, uint32_t ng, const uint32_t* wlen,
+                        uint32_t seg_target, uint64_t* ntile, uint64_t* bound, uint32_t* gnseg,
+                        hipStream_t st);
+hipError_t launch_tilemap(uint32_t ng, const uint64_t* tile_base, const uint64_t* ntile,
+                          uint32_t* tilemap, hipStream_t st);
+hipError_t launch_fill_u64(unsigned long long* p, uint64_t n, unsigned long long v, hipStream_t st);
+hipError_t launch_level(const LevelArgs& a, unsigned nblocks, hipStream_t st);
+int level_max_blocks();
+
 void hip_check(hipError_t e, const char* what) {
   if (e == hipSuccess) return;
   int code = (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation) ? GLS_E_OOM : GLS_E_DEVICE;
@@ -43,12 +53,17 @@ void hip_check(hipError_t e, const char* what) {
 
 #define HC(x) hip_check((x), #x)
 
+// scalar slots in d_scal_
+enum { S_BUMP = 0, S_TOT_A = 1, S_TOT_B = 2, S_TOT_OUT = 3, S_EVC = 4, S_TICKET = 5,
+       S_OVF_USED = 6, S_ERR = 7 };
+
 Job::Job(int device, hipStream_t stream) : device_(device), stream_(stream) {}
 
 Job::~Job() {
-  void* ptrs[] = {d_gates_, d_luts_, d_woff_, d_wlen_, d_pool_, d_saif_, d_saif_nets_, d_scal_,
-                  d_nseg_, d_scr_, d_segbase_, d_scrbase_, d_outlen_, d_outoff_, d_part_, d_gnseg_,
-                  d_segmap_, d_sums_, d_carry_, d_scratch_};
+  void* ptrs[] = {d_gates_, d_grec_, d_luts_, d_woff_, d_wlen_, d_pool_, d_saif_, d_saif_nets_,
+                  d_scal_, d_nseg_, d_scr_, d_segbase_, d_scrbase_, d_outlen_, d_outoff_, d_part_,
+                  d_gnseg_, d_segmap_, d_sums_, d_carry_, d_scratch_, d_ntile_, d_bound_,
+                  d_tilebase_, d_tilemap_, d_tstate_, d_ovf_};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
 }
@@ -63,6 +78,35 @@ void Job::ensure(void** p, size_t& cap, size_t need) {
   cap = n;
 }
 
+static uint32_t sel_delay(uint32_t r, uint32_t f, int s) {
+  return s == 1 ? r : (s == 2 ? f : (s == 3 ? std::max(r, f) : std::min(r, f)));
+}
+
+// Fold AddDelay (src/simulate.cpp:1012-1176) into per-input tables indexed
+// by the output edge class and by whether the input fell at this time.
+static void fill_delay_tables(const GateDelays& D, int k, GateRec& r) {
+  const uint16_t NO = kNoDelay;
+  for (int j = 0; j < 4; ++j)
+    for (int s = 1; s <= 4; ++s)
+      for (int fall = 0; fall < 2; ++fall) {
+        uint32_t m = 0xFFFF, one = 1;
+        if (j < k) {
+          const uint16_t* both = D.v[j][DK_BOTH];
+          const uint16_t* kind = D.v[j][fall ? DK_NEG : DK_POS];
+          if (both[0] != NO) m = std::min(m, sel_delay(both[0], both[1], s));
+          if (kind[0] != NO) m = std::min(m, sel_delay(kind[0], kind[1], s));
+          if (D.nrec[j] == 1) {
+            for (int kd = 0; kd < 3; ++kd)
+              if (D.v[j][kd][0] != NO) one = sel_delay(D.v[j][kd][0], D.v[j][kd][1], s);
+          } else if (D.nrec[j] >= 2) {
+            one = kind[0] != NO ? sel_delay(kind[0], kind[1], s) : 1;
+          }
+        }
+        r.tmin[j][s - 1][fall] = (uint16_t)m;
+        r.tone[j][s - 1][fall] = (uint16_t)one;
+      }
+}
+
 void Job::prepare(const Program& pg, const Stimulus& st, const std::vector<int>& stim_nets,
                   uint64_t start, uint64_t end, uint64_t pool_words) {
   HC(hipSetDevice(device_));
@@ -74,16 +118,21 @@ void Job::prepare(const Program& pg, const Stimulus& st, const std::vector<int>&
   n_nets_ = (uint32_t)pg.net_names.size();
   for (int sn : stim_nets) n_nets_ = std::max<uint32_t>(n_nets_, (uint32_t)sn + 1);
 
-  // gates
+  // gate records for both paths
   std::vector<GateDev> gd(n_gates_);
+  std::vector<GateRec> gr(n_gates_);
   gate_in_nets_.assign(n_gates_, {});
   for (uint32_t i = 0; i < n_gates_; ++i) {
     const CompiledGate& g = pg.gates[i];
+    if (g.k < 1) throw GlsError(GLS_E_UNSUPPORTED, "gate without inputs");
     GateDelays D = gate_delays(g, st.timescale_ps);
     GateDev& r = gd[i];
     std::memset(&r, 0, sizeof r);
+    GateRec& q = gr[i];
+    std::memset(&q, 0, sizeof q);
     for (int j = 0; j < 4; ++j) {
       r.in_net[j] = j < g.k ? (uint32_t)g.in_net[j] : 0u;
+      q.in_net[j] = r.in_net[j];
       r.nrec[j] = D.nrec[j];
       for (int kd = 0; kd < 3; ++kd) {
         r.dly[j][kd][0] = D.v[j][kd][0];
@@ -91,11 +140,11 @@ void Job::prepare(const Program& pg, const Stimulus& st, const std::vector<int>&
       }
       if (j < g.k) gate_in_nets_[i].push_back((uint32_t)g.in_net[j]);
     }
-    r.out_net = (uint32_t)g.out_net;
-    r.lut = (uint16_t)g.lut;
-    r.k = (uint8_t)g.k;
-    r.dmax = D.dmax;
-    if (g.k < 1) throw GlsError(GLS_E_UNSUPPORTED, "gate without inputs");
+    r.out_net = q.out_net = (uint32_t)g.out_net;
+    r.lut = q.lut = (uint16_t)g.lut;
+    r.k = q.k = (uint8_t)g.k;
+    r.dmax = q.dmax = D.dmax;
+    fill_delay_tables(D, g.k, q);
   }
   std::vector<uint8_t> luts(pg.luts.size() * 256, 1);
   for (size_t i = 0; i < pg.luts.size(); ++i)
@@ -127,34 +176,36 @@ void Job::prepare(const Program& pg, const Stimulus& st, const std::vector<int>&
   }
   stim_words_ = words.size();
 
-  size_t free_b = 0, total_b = 0;
-  HC(hipMemGetInfo(&free_b, &total_b));
-  if (pool_words == 0) pool_words = (uint64_t)(free_b * 0.70) / 4;
-  if (pool_words < stim_words_ + 1024) pool_words = stim_words_ + 1024;
-  pool_words_ = pool_words;
-
   HC(hipMalloc((void**)&d_gates_, std::max<size_t>(1, gd.size()) * sizeof(GateDev)));
+  HC(hipMalloc((void**)&d_grec_, std::max<size_t>(1, gr.size()) * sizeof(GateRec)));
   HC(hipMalloc((void**)&d_luts_, std::max<size_t>(256, luts.size())));
   HC(hipMalloc((void**)&d_woff_, n_nets_ * sizeof(uint64_t)));
   HC(hipMalloc((void**)&d_wlen_, n_nets_ * sizeof(uint32_t)));
-  HC(hipMalloc((void**)&d_pool_, pool_words_ * sizeof(uint32_t)));
   HC(hipMalloc((void**)&d_saif_, (size_t)n_nets_ * 3 * sizeof(unsigned long long)));
   HC(hipMalloc((void**)&d_saif_nets_, std::max<size_t>(1, saif_net_list_.size()) * 4));
   HC(hipMalloc((void**)&d_scal_, 16 * sizeof(uint64_t)));
   uint32_t maxw = 1;
   for (size_t l = 0; l + 1 < level_start_.size(); ++l)
     maxw = std::max(maxw, level_start_[l + 1] - level_start_[l]);
-  HC(hipMalloc((void**)&d_nseg_, maxw * 8ull));
-  HC(hipMalloc((void**)&d_scr_, maxw * 8ull));
-  HC(hipMalloc((void**)&d_segbase_, maxw * 8ull));
-  HC(hipMalloc((void**)&d_scrbase_, maxw * 8ull));
-  HC(hipMalloc((void**)&d_outlen_, maxw * 8ull));
-  HC(hipMalloc((void**)&d_outoff_, maxw * 8ull));
+  for (uint64_t** p : {&d_nseg_, &d_scr_, &d_segbase_, &d_scrbase_, &d_outlen_, &d_outoff_,
+                       &d_ntile_, &d_bound_, &d_tilebase_})
+    HC(hipMalloc((void**)p, maxw * 8ull));
   HC(hipMalloc((void**)&d_part_, scan_part_size(maxw) * 8ull));
   HC(hipMalloc((void**)&d_gnseg_, maxw * 4ull));
+  ovf_words_ = 1ull << 24;  // stacks of tiles that do not fit LDS (64 MiB)
+  HC(hipMalloc((void**)&d_ovf_, ovf_words_ * 4));
 
-  if (!gd.empty())
+  size_t free_b = 0, total_b = 0;
+  HC(hipMemGetInfo(&free_b, &total_b));
+  if (pool_words == 0) pool_words = (uint64_t)(free_b * 0.85) / 4;
+  if (pool_words < stim_words_ + 1024) pool_words = stim_words_ + 1024;
+  pool_words_ = pool_words;
+  HC(hipMalloc((void**)&d_pool_, pool_words_ * sizeof(uint32_t)));
+
+  if (!gd.empty()) {
     HC(hipMemcpyAsync(d_gates_, gd.data(), gd.size() * sizeof(GateDev), hipMemcpyHostToDevice, stream_));
+    HC(hipMemcpyAsync(d_grec_, gr.data(), gr.size() * sizeof(GateRec), hipMemcpyHostToDevice, stream_));
+  }
   if (!luts.empty())
     HC(hipMemcpyAsync(d_luts_, luts.data(), luts.size(), hipMemcpyHostToDevice, stream_));
   if (!words.empty())
@@ -163,47 +214,86 @@ void Job::prepare(const Program& pg, const Stimulus& st, const std::vector<int>&
     HC(hipMemcpyAsync(d_saif_nets_, saif_net_list_.data(), saif_net_list_.size() * 4,
                       hipMemcpyHostToDevice, stream_));
   HC(hipStreamSynchronize(stream_));
+  level_blocks_ = level_max_blocks();
   stats_ = JobStats();
   stats_.n_nets = n_nets_;
   stats_.n_gates = n_gates_;
   stats_.n_pins = pg.n_pins;
 }
 
-void Job::run() {
-  HC(hipSetDevice(device_));
-  auto w0 = std::chrono::steady_clock::now();
-  hipEvent_t e0, e1;
-  HC(hipEventCreate(&e0));
-  HC(hipEventCreate(&e1));
-  uint64_t scal[16] = {0};
-  scal[0] = stim_words_;
-  HC(hipMemcpyAsync(d_woff_, h_woff_init_.data(), n_nets_ * 8ull, hipMemcpyHostToDevice, stream_));
-  HC(hipMemcpyAsync(d_wlen_, h_wlen_init_.data(), n_nets_ * 4ull, hipMemcpyHostToDevice, stream_));
-  HC(hipMemcpyAsync(d_scal_, scal, sizeof scal, hipMemcpyHostToDevice, stream_));
-  HC(hipMemsetAsync(d_saif_, 0, (size_t)n_nets_ * 3 * 8, stream_));
-  HC(hipStreamSynchronize(stream_));
+void Job::run_fused(std::vector<hipEvent_t>& ev_eval) {
+  uint64_t* bump = d_scal_ + S_BUMP;
+  unsigned long long* evc = (unsigned long long*)(d_scal_ + S_EVC);
+  const uint32_t seg = seg_target ? seg_target : 32;
+  for (size_t l = 0; l + 1 < level_start_.size(); ++l) {
+    uint32_t g0 = level_start_[l], ng = level_start_[l + 1] - g0;
+    if (ng == 0) continue;
+    const GateRec* gates = d_grec_ + g0;
+    HC(launch_plan2(gates, ng, d_wlen_, seg, d_ntile_, d_bound_, d_gnseg_, stream_));
+    HC(launch_scan(d_ntile_, d_tilebase_, ng, d_part_, d_scal_ + S_TOT_A, nullptr, stream_));
+    HC(launch_scan(d_bound_, d_outoff_, ng, d_part_, d_scal_ + S_TOT_B, bump, stream_));
+    uint64_t h[3];
+    HC(hipMemcpyAsync(h, d_scal_, 3 * sizeof(uint64_t), hipMemcpyDeviceToHost, stream_));
+    HC(hipStreamSynchronize(stream_));
+    const uint64_t ntiles = h[S_TOT_A], need = h[S_TOT_B], used = h[S_BUMP];
+    if (used + need > pool_words_)
+      throw GlsError(GLS_E_OOM, "event pool exhausted at level " + std::to_string(l) +
+                                    " (raise the pool size or shorten the window)");
+    if (ntiles >= 0xFFFFFFF0ull) throw GlsError(GLS_E_RANGE, "too many tiles in one level");
+    ensure((void**)&d_tilemap_, cap_tilemap_, ntiles * 4);
+    ensure((void**)&d_tstate_, cap_tstate_, ntiles * 16);
+    HC(launch_tilemap(ng, d_tilebase_, d_ntile_, d_tilemap_, stream_));
+    HC(launch_fill_u64(d_tstate_, 2 * ntiles, 0ull, stream_));
+    HC(launch_fill_u64((unsigned long long*)(d_scal_ + S_TICKET), 2, 0ull, stream_));  // ticket, ovf
+    LevelArgs a;
+    a.gates = gates;
+    a.tilemap = d_tilemap_;
+    a.tile_base = d_tilebase_;
+    a.gnseg = d_gnseg_;
+    a.out_base = d_outoff_;
+    a.ntiles = ntiles;
+    a.woff = d_woff_;
+    a.wlen = d_wlen_;
+    a.pool = d_pool_;
+    a.luts = d_luts_;
+    a.tstate = d_tstate_;
+    a.ticket = (unsigned int*)(d_scal_ + S_TICKET);
+    a.ovf = d_ovf_;
+    a.ovf_cap = ovf_words_;
+    a.ovf_used = (unsigned long long*)(d_scal_ + S_OVF_USED);
+    a.saif = d_saif_;
+    a.ev_count = evc;
+    a.err = (unsigned int*)(d_scal_ + S_ERR);
+    a.start = start_;
+    a.end = end_;
+    unsigned nb = (unsigned)std::min<uint64_t>(ntiles, (uint64_t)std::max(1, level_blocks_));
+    hipEvent_t e_a, e_b;
+    HC(hipEventCreate(&e_a));
+    HC(hipEventCreate(&e_b));
+    ev_eval.push_back(e_a);
+    ev_eval.push_back(e_b);
+    HC(hipEventRecord(e_a, stream_));
+    HC(launch_level(a, nb, stream_));
+    HC(hipEventRecord(e_b, stream_));
+    HC(launch_bump(bump, d_scal_ + S_TOT_B, stream_));
+  }
+}
 
-  HC(hipEventRecord(e0, stream_));
-  HC(launch_saif_nets(d_saif_nets_, (uint32_t)saif_net_list_.size(), d_woff_, d_wlen_, d_pool_,
-                      d_saif_, start_, end_, stream_));
-  uint64_t* bump = d_scal_ + 0;
-  uint64_t* tot_segs = d_scal_ + 1;
-  uint64_t* tot_scr = d_scal_ + 2;
-  uint64_t* tot_out = d_scal_ + 3;
-  unsigned long long* evc = (unsigned long long*)(d_scal_ + 4);
-  // HIP events around every k_eval launch, on the stream it runs on
-  std::vector<hipEvent_t> ev_eval;
+void Job::run_two_pass(std::vector<hipEvent_t>& ev_eval) {
+  uint64_t* bump = d_scal_ + S_BUMP;
+  unsigned long long* evc = (unsigned long long*)(d_scal_ + S_EVC);
+  const uint32_t seg = seg_target ? seg_target : 256;
   for (size_t l = 0; l + 1 < level_start_.size(); ++l) {
     uint32_t g0 = level_start_[l], ng = level_start_[l + 1] - g0;
     if (ng == 0) continue;
     const GateDev* gates = d_gates_ + g0;
-    HC(launch_plan(gates, ng, d_wlen_, seg_target, d_nseg_, d_scr_, d_gnseg_, stream_));
-    HC(launch_scan(d_nseg_, d_segbase_, ng, d_part_, tot_segs, nullptr, stream_));
-    HC(launch_scan(d_scr_, d_scrbase_, ng, d_part_, tot_scr, nullptr, stream_));
-    uint64_t h[4];
-    HC(hipMemcpyAsync(h, d_scal_, 4 * sizeof(uint64_t), hipMemcpyDeviceToHost, stream_));
+    HC(launch_plan(gates, ng, d_wlen_, seg, d_nseg_, d_scr_, d_gnseg_, stream_));
+    HC(launch_scan(d_nseg_, d_segbase_, ng, d_part_, d_scal_ + S_TOT_A, nullptr, stream_));
+    HC(launch_scan(d_scr_, d_scrbase_, ng, d_part_, d_scal_ + S_TOT_B, nullptr, stream_));
+    uint64_t h[3];
+    HC(hipMemcpyAsync(h, d_scal_, 3 * sizeof(uint64_t), hipMemcpyDeviceToHost, stream_));
     HC(hipStreamSynchronize(stream_));
-    uint64_t nsegs = h[1], nscr = h[2], used = h[0];
+    uint64_t nsegs = h[S_TOT_A], nscr = h[S_TOT_B], used = h[S_BUMP];
     ensure((void**)&d_segmap_, cap_segmap_, nsegs * 4);
     ensure((void**)&d_sums_, cap_sums_, nsegs * sizeof(SegSum));
     ensure((void**)&d_carry_, cap_carry_, nsegs * sizeof(SegCarry));
@@ -213,21 +303,51 @@ void Job::run() {
       throw GlsError(GLS_E_OOM, "event pool exhausted at level " + std::to_string(l) +
                                     " (raise the pool size or shorten the window)");
     HC(launch_segmap(ng, d_segbase_, d_gnseg_, d_segmap_, stream_));
-    hipEvent_t a, b;
-    HC(hipEventCreate(&a));
-    HC(hipEventCreate(&b));
-    ev_eval.push_back(a);
-    ev_eval.push_back(b);
-    HC(hipEventRecord(a, stream_));
+    hipEvent_t e_a, e_b;
+    HC(hipEventCreate(&e_a));
+    HC(hipEventCreate(&e_b));
+    ev_eval.push_back(e_a);
+    ev_eval.push_back(e_b);
+    HC(hipEventRecord(e_a, stream_));
     HC(launch_eval(gates, d_segbase_, d_scrbase_, d_gnseg_, d_segmap_, nsegs, d_woff_, d_wlen_,
                    d_pool_, d_luts_, d_scratch_, d_sums_, evc, stream_));
-    HC(hipEventRecord(b, stream_));
+    HC(hipEventRecord(e_b, stream_));
     HC(launch_gate_scan(gates, ng, d_segbase_, d_gnseg_, d_sums_, d_carry_, d_outlen_, d_saif_,
                         start_, end_, stream_));
-    HC(launch_scan(d_outlen_, d_outoff_, ng, d_part_, tot_out, bump, stream_));
+    HC(launch_scan(d_outlen_, d_outoff_, ng, d_part_, d_scal_ + S_TOT_OUT, bump, stream_));
     HC(launch_emit(gates, d_segbase_, d_segmap_, nsegs, d_sums_, d_carry_, d_outoff_, d_outlen_,
                    d_scratch_, d_pool_, d_woff_, d_wlen_, d_saif_, start_, end_, stream_));
-    HC(launch_bump(bump, tot_out, stream_));
+    HC(launch_bump(bump, d_scal_ + S_TOT_OUT, stream_));
+  }
+}
+
+void Job::run() {
+  HC(hipSetDevice(device_));
+  auto w0 = std::chrono::steady_clock::now();
+  hipEvent_t e0, e1;
+  HC(hipEventCreate(&e0));
+  HC(hipEventCreate(&e1));
+  uint64_t scal[16] = {0};
+  scal[S_BUMP] = stim_words_;
+  HC(hipMemcpyAsync(d_woff_, h_woff_init_.data(), n_nets_ * 8ull, hipMemcpyHostToDevice, stream_));
+  HC(hipMemcpyAsync(d_wlen_, h_wlen_init_.data(), n_nets_ * 4ull, hipMemcpyHostToDevice, stream_));
+  HC(hipMemcpyAsync(d_scal_, scal, sizeof scal, hipMemcpyHostToDevice, stream_));
+  HC(launch_fill_u64(d_saif_, (uint64_t)n_nets_ * 3, 0ull, stream_));
+  HC(hipStreamSynchronize(stream_));
+
+  HC(hipEventRecord(e0, stream_));
+  HC(launch_saif_nets(d_saif_nets_, (uint32_t)saif_net_list_.size(), d_woff_, d_wlen_, d_pool_,
+                      d_saif_, start_, end_, stream_));
+  std::vector<hipEvent_t> ev_eval;
+  try {
+    if (mode == 1) run_two_pass(ev_eval);
+    else run_fused(ev_eval);
+  } catch (...) {
+    (void)hipStreamSynchronize(stream_);
+    for (hipEvent_t e : ev_eval) (void)hipEventDestroy(e);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    throw;
   }
   HC(hipEventRecord(e1, stream_));
   HC(hipEventSynchronize(e1));
@@ -244,22 +364,23 @@ void Job::run() {
   for (hipEvent_t e : ev_eval) (void)hipEventDestroy(e);
   stats_.eval_ms = eval_ms;
   stats_.eval_launches = ev_eval.size() / 2;
-  uint64_t h[8];
+  uint64_t h[16];
   HC(hipMemcpy(h, d_scal_, sizeof h, hipMemcpyDeviceToHost));
+  if (h[S_ERR] & 1u) throw GlsError(GLS_E_DEVICE, "tile look-back did not complete (spin limit)");
+  if (h[S_ERR] & 2u)
+    throw GlsError(GLS_E_UNSUPPORTED, "a tile exceeded the overflow stack buffer (event density too skewed)");
   auto w1 = std::chrono::steady_clock::now();
   stats_.kernel_ms = ms;
   stats_.wall_ms = std::chrono::duration<double, std::milli>(w1 - w0).count();
-  stats_.n_events = h[4];
-  stats_.pool_words_used = h[0];
+  stats_.n_events = h[S_EVC];
+  stats_.pool_words_used = h[S_BUMP];
   // algorithmic bytes: every input waveform read once per reading gate, every
   // output written once, 3 u64 SAIF counters per net, CSR metadata
   std::vector<uint32_t> wlen(n_nets_);
   HC(hipMemcpy(wlen.data(), d_wlen_, n_nets_ * 4ull, hipMemcpyDeviceToHost));
   uint64_t in_w = 0, out_w = 0;
-  std::vector<uint8_t> is_out(n_nets_, 0);
   for (uint32_t g = 0; g < n_gates_; ++g)
     for (uint32_t n : gate_in_nets_[g]) in_w += wlen[n];
-  // output nets: every gate's out_net
   std::vector<GateDev> gd(n_gates_);
   if (n_gates_) HC(hipMemcpy(gd.data(), d_gates_, n_gates_ * sizeof(GateDev), hipMemcpyDeviceToHost));
   for (auto& g : gd) out_w += wlen[g.out_net];

@@ -33,7 +33,12 @@ class DesignInfo(ctypes.Structure):
 
 class SimOpts(ctypes.Structure):
     _fields_ = [("pool_bytes", ctypes.c_uint64), ("seg_events", ctypes.c_uint32),
-                ("keep_waves", ctypes.c_uint32)]
+                ("keep_waves", ctypes.c_uint32), ("path", ctypes.c_uint32),
+                ("reserved", ctypes.c_uint32)]
+
+
+PATH_FUSED = 0
+PATH_TWO_PASS = 1
 
 
 class RunStats(ctypes.Structure):
@@ -254,19 +259,21 @@ class Context(_Handle):
         super().__init__(h.value)
 
     @staticmethod
-    def _opts(pool_bytes, seg_events, keep_waves):
-        return SimOpts(pool_bytes, seg_events, 1 if keep_waves else 0)
+    def _opts(pool_bytes, seg_events, keep_waves, path):
+        return SimOpts(pool_bytes, seg_events, 1 if keep_waves else 0, path, 0)
 
-    def prepare(self, design, stim, dumpon, dumpoff, pool_bytes=0, seg_events=0, keep_waves=False):
+    def prepare(self, design, stim, dumpon, dumpoff, pool_bytes=0, seg_events=0, keep_waves=False,
+                path=PATH_FUSED):
         h = ctypes.c_void_p()
-        o = self._opts(pool_bytes, seg_events, keep_waves)
+        o = self._opts(pool_bytes, seg_events, keep_waves, path)
         _check(load().gls_prepare(self.ptr, design.ptr, stim.ptr, dumpon, dumpoff, ctypes.byref(o),
                                   ctypes.byref(h)))
         return Run(h.value)
 
-    def simulate(self, design, stim, dumpon, dumpoff, pool_bytes=0, seg_events=0, keep_waves=False):
+    def simulate(self, design, stim, dumpon, dumpoff, pool_bytes=0, seg_events=0, keep_waves=False,
+                 path=PATH_FUSED):
         h = ctypes.c_void_p()
-        o = self._opts(pool_bytes, seg_events, keep_waves)
+        o = self._opts(pool_bytes, seg_events, keep_waves, path)
         _check(load().gls_simulate(self.ptr, design.ptr, stim.ptr, dumpon, dumpoff, ctypes.byref(o),
                                    ctypes.byref(h)))
         return Run(h.value)

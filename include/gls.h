@@ -79,10 +79,17 @@ int gls_ctx_create(int device, gls_ctx** out);
 void gls_ctx_destroy(gls_ctx* c);
 
 typedef struct {
-  uint64_t pool_bytes;   /* event pool size (0: 70% of free HBM) */
-  uint32_t seg_events;   /* target input events per time segment (0: 256) */
+  uint64_t pool_bytes;   /* event pool size (0: 85% of free HBM) */
+  uint32_t seg_events;   /* target input events per time segment (0: path default) */
   uint32_t keep_waves;   /* 1: keep waveforms for gls_run_waveform */
+  uint32_t path;         /* GLS_PATH_FUSED (default) or GLS_PATH_TWO_PASS */
+  uint32_t reserved;
 } gls_sim_opts;
+
+/* Level-kernel variants: one fused LDS-tiled kernel per level, or the
+ * two-pass form (evaluation into scratch, then compaction). */
+#define GLS_PATH_FUSED 0
+#define GLS_PATH_TWO_PASS 1
 
 /* Upload program + stimulus into HBM (the per-level H2D copies of
  * src/simulate.cu:2434-2482 happen once here).  dumpon/dumpoff are the SAIF

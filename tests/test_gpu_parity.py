@@ -66,19 +66,21 @@ def test_random_designs_against_oracle(ctx, seed, tmp_path):
     s.write_vcd(paths[3])
     start, end = rng.randint(0, t_end // 3), t_end + rng.randint(1, 500)
     want = oracle_bind.run(*paths, start, end, str(tmp_path))
-    for seg in (0, 3, 17):
-        r = ctx.simulate(d, s, start, end, seg_events=seg, keep_waves=True)
+    for path, seg in ((gls.PATH_FUSED, 0), (gls.PATH_FUSED, 2), (gls.PATH_FUSED, 5000),
+                      (gls.PATH_TWO_PASS, 0), (gls.PATH_TWO_PASS, 3)):
+        r = ctx.simulate(d, s, start, end, seg_events=seg, keep_waves=True, path=path)
         assert r.stats()["n_events"] == want["n_events"]
-        assert r.waves_hash() == want["waves_hash"], seg
-        assert r.saif_text() == want["saif"], seg
+        assert r.waves_hash() == want["waves_hash"], (path, seg)
+        assert r.saif_text() == want["saif"], (path, seg)
 
 
+@pytest.mark.parametrize("path", [0, 1], ids=["fused", "two_pass"])
 @pytest.mark.parametrize("name", ["c0_small", "c1_small", "c0_full"])
-def test_golden_workloads(ctx, name):
-    say("golden", name)
+def test_golden_workloads(ctx, name, path):
+    say("golden", name, path)
     d, s = golden_cases.make(name)
     start, end = golden_cases.CASES[name]["window"]
-    r = ctx.simulate(d, s, start, end, keep_waves=True)
+    r = ctx.simulate(d, s, start, end, keep_waves=True, path=path)
     g = GOLDEN[name]
     assert r.stats()["n_events"] == g["n_events"]
     assert "%016x" % r.waves_hash() == g["waves_hash"]
@@ -89,9 +91,12 @@ def test_segment_size_invariance(ctx):
     d, s = golden_cases.make("c1_small")
     start, end = golden_cases.CASES["c1_small"]["window"]
     ref = None
-    for seg in (1, 8, 64, 256, 5000, 1 << 20):
-        say("segments", seg)
-        r = ctx.simulate(d, s, start, end, seg_events=seg, keep_waves=True)
+    # seg 1: many tiles per gate (long look-back chains); 1<<20: every tile
+    # overflows LDS and takes the HBM fallback
+    for path, seg in ((0, 1), (0, 8), (0, 32), (0, 256), (0, 5000), (0, 1 << 20),
+                      (1, 1), (1, 64), (1, 1 << 20)):
+        say("segments", path, seg)
+        r = ctx.simulate(d, s, start, end, seg_events=seg, keep_waves=True, path=path)
         cur = (r.waves_hash(), r.saif_text())
         if ref is None:
             ref = cur
