@@ -65,6 +65,7 @@ def main():
     ap.add_argument("--t-end", type=int, default=20000000)
     ap.add_argument("--seg-events", type=int, default=0)
     ap.add_argument("--path", choices=["fused", "two-pass"], default="fused")
+    ap.add_argument("--level-blocks", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-events", type=int, default=1500000)
     args = ap.parse_args()
@@ -87,8 +88,10 @@ def main():
         f"{s.info()['n_events']} stimulus events in {time.time() - t0:.1f}s")
     ctx = gls.Context(0)
     path = gls.PATH_FUSED if args.path == "fused" else gls.PATH_TWO_PASS
-    run = ctx.prepare(d, s, 0, args.t_end + 1, seg_events=args.seg_events, path=path)
-    log("bench: prepared (program + stimulus resident in HBM)")
+    run = ctx.prepare(d, s, 0, args.t_end + 1, seg_events=args.seg_events, path=path,
+                      level_blocks=args.level_blocks)
+    log(f"bench: prepared (program + stimulus resident in HBM), "
+        f"{run.stats()['level_blocks']} workgroups per level launch")
     for i in range(args.warmup):
         run.execute()
         log(f"bench: warmup {i} done ({run.stats()['kernel_ms']:.1f} ms device)")

@@ -297,6 +297,7 @@ int gls_prepare(gls_ctx* c, const gls_design* d, const gls_stim* s, uint64_t dum
   if (opts) {
     if (opts->path > GLS_PATH_TWO_PASS) throw GlsError(GLS_E_ARG, "unknown path");
     r->job->mode = (int)opts->path;
+    r->job->level_blocks_opt = opts->level_blocks;
   }
   r->job->prepare(r->pg, s->st, r->stim_nets, dumpon, dumpoff, opts ? opts->pool_bytes / 4 : 0);
   *out = r.release();
@@ -342,6 +343,7 @@ int gls_run_stats_get(const gls_run* r, gls_run_stats* st) {
   st->wall_ms = j.wall_ms;
   st->eval_ms = j.eval_ms;
   st->eval_launches = j.eval_launches;
+  st->level_blocks = j.level_blocks;
   API_END
 }
 

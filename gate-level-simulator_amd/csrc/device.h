@@ -42,12 +42,27 @@ struct alignas(16) GateRec {
 };
 static_assert(sizeof(GateRec) == 160, "GateRec layout");
 
+// Per-gate segmentation of one level: segment c covers the leader input's
+// events [c*stride, (c+1)*stride).
+struct GatePlan {
+  uint32_t nseg, stride, lead, pad;
+};
+
+// Per-tile input ranges: L[j] = first event of input j at or after the
+// tile's start time, E[j] = end of the range including the look-ahead halo.
+struct TileDesc {
+  uint32_t g, ti;
+  uint32_t L[4], E[4];
+  uint32_t pad[2];
+};
+static_assert(sizeof(TileDesc) == 48, "TileDesc layout");
+
 // Arguments of the fused level kernel.
 struct LevelArgs {
   const GateRec* gates;       // gates of this level
-  const uint32_t* tilemap;    // tile -> level-local gate index
-  const uint64_t* tile_base;  // per gate: first tile
-  const uint32_t* gnseg;      // per gate: segments
+  const GatePlan* plan;       // per gate
+  const TileDesc* tiles;      // per tile
+  const uint32_t* lutp;       // truth tables packed 2 bits per entry, 16 words each
   const uint64_t* out_base;   // per gate: output region (pool word offset)
   uint64_t ntiles;
   uint64_t* woff;
@@ -63,7 +78,13 @@ struct LevelArgs {
   unsigned long long* ev_count;
   unsigned int* err;
   unsigned long long start, end;
+  uint32_t cap;                // LDS input capacity per tile (words)
+  uint32_t pad;
 };
+
+// LDS bytes of one fused-level workgroup for an input capacity of `cap`
+// words: inputs (cap + 64), stacks (cap + 64), truth table, delay tables.
+inline size_t level_lds_bytes(uint32_t cap) { return (2ull * (cap + 64)) * 4 + 256 + 128; }
 
 // Per time-segment summary written by the evaluation kernel.
 struct SegSum {

@@ -23,6 +23,7 @@ struct JobStats {
   double wall_ms = 0;           // host wall time of the level loop
   double eval_ms = 0;           // summed device time of the k_eval launches
   uint64_t eval_launches = 0;
+  uint64_t level_blocks = 0;    // workgroups per fused level launch
   uint64_t pool_words_used = 0;
 };
 
@@ -46,6 +47,7 @@ class Job {
   const JobStats& stats() const { return stats_; }
   uint32_t seg_target = 0;  // input events per time segment (0: mode default)
   int mode = 0;             // 0: fused LDS-tiled level kernel, 1: two-pass kernels
+  uint32_t level_blocks_opt = 0;  // workgroups per fused launch (0: occupancy limit)
 
  private:
   void ensure(void** p, size_t& cap, size_t need);
@@ -79,12 +81,16 @@ class Job {
   uint32_t* d_scratch_ = nullptr;
   // fused path
   uint64_t *d_ntile_ = nullptr, *d_bound_ = nullptr, *d_tilebase_ = nullptr;
-  size_t cap_tilemap_ = 0, cap_tstate_ = 0;
-  uint32_t* d_tilemap_ = nullptr;
+  size_t cap_tiles_ = 0, cap_tstate_ = 0;
+  GatePlan* d_plan_ = nullptr;
+  TileDesc* d_tiles_ = nullptr;
+  uint32_t* d_lutp_ = nullptr;
   unsigned long long* d_tstate_ = nullptr;
   uint32_t* d_ovf_ = nullptr;
   uint64_t ovf_words_ = 0;
   int level_blocks_ = 0;
+  uint32_t cap_ = 0;                          // LDS input words per fused tile
+  static constexpr uint32_t kDefaultSeg = 64;  // input events per time segment
   std::vector<uint64_t> h_woff_init_;
   std::vector<uint32_t> h_wlen_init_;
   JobStats stats_;

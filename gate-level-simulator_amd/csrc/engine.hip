@@ -37,13 +37,23 @@ hipError_t launch_saif_nets(const uint32_t* nets, uint32_t count, const uint64_t
 
 // level.hip: fused path
 hipError_t launch_plan2(const GateRec* gates, uint32_t ng, const uint32_t* wlen,
-                        uint32_t seg_target, uint64_t* ntile, uint64_t* bound, uint32_t* gnseg,
-                        hipStream_t st);
-hipError_t launch_tilemap(uint32_t ng, const uint64_t* tile_base, const uint64_t* ntile,
-                          uint32_t* tilemap, hipStream_t st);
+                        uint32_t seg_target, GatePlan* plan, uint64_t* ntile, uint64_t* bound,
+                        unsigned int* err, hipStream_t st);
+hipError_t launch_tile_desc(const GateRec* gates, uint32_t ng, const GatePlan* plan,
+                            const uint64_t* tile_base, const uint64_t* ntile, const uint64_t* woff,
+                            const uint32_t* wlen, const uint32_t* pool, TileDesc* tiles,
+                            uint64_t ntiles, hipStream_t st);
 hipError_t launch_fill_u64(unsigned long long* p, uint64_t n, unsigned long long v, hipStream_t st);
 hipError_t launch_level(const LevelArgs& a, unsigned nblocks, hipStream_t st);
-int level_max_blocks();
+int level_max_blocks(uint32_t cap);
+
+// LDS input capacity of a fused tile: 64 segments of `seg` input events,
+// plus variance and halo headroom.
+static uint32_t tile_cap(uint32_t seg) {
+  uint64_t c = (uint64_t)seg * 64 * 13 / 10 + 256;
+  c = (c + 63) / 64 * 64;
+  return (uint32_t)std::min<uint64_t>(std::max<uint64_t>(c, 1024), 16384);
+}
 
 void hip_check(hipError_t e, const char* what) {
   if (e == hipSuccess) return;
@@ -63,7 +73,7 @@ Job::~Job() {
   void* ptrs[] = {d_gates_, d_grec_, d_luts_, d_woff_, d_wlen_, d_pool_, d_saif_, d_saif_nets_,
                   d_scal_, d_nseg_, d_scr_, d_segbase_, d_scrbase_, d_outlen_, d_outoff_, d_part_,
                   d_gnseg_, d_segmap_, d_sums_, d_carry_, d_scratch_, d_ntile_, d_bound_,
-                  d_tilebase_, d_tilemap_, d_tstate_, d_ovf_};
+                  d_tilebase_, d_plan_, d_tiles_, d_lutp_, d_tstate_, d_ovf_};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
 }
@@ -192,8 +202,16 @@ void Job::prepare(const Program& pg, const Stimulus& st, const std::vector<int>&
     HC(hipMalloc((void**)p, maxw * 8ull));
   HC(hipMalloc((void**)&d_part_, scan_part_size(maxw) * 8ull));
   HC(hipMalloc((void**)&d_gnseg_, maxw * 4ull));
+  HC(hipMalloc((void**)&d_plan_, maxw * sizeof(GatePlan)));
   ovf_words_ = 1ull << 24;  // stacks of tiles that do not fit LDS (64 MiB)
   HC(hipMalloc((void**)&d_ovf_, ovf_words_ * 4));
+  // truth tables packed 2 bits per entry (16 words each) for register lookups
+  std::vector<uint32_t> lutp(std::max<size_t>(1, pg.luts.size()) * 16, 0);
+  for (size_t i = 0; i < pg.luts.size(); ++i)
+    for (size_t e = 0; e < pg.luts[i].size() && e < 256; ++e)
+      lutp[i * 16 + e / 16] |= (uint32_t)(pg.luts[i][e] & 3u) << (2 * (e % 16));
+  HC(hipMalloc((void**)&d_lutp_, lutp.size() * 4));
+  HC(hipMemcpyAsync(d_lutp_, lutp.data(), lutp.size() * 4, hipMemcpyHostToDevice, stream_));
 
   size_t free_b = 0, total_b = 0;
   HC(hipMemGetInfo(&free_b, &total_b));
@@ -214,8 +232,10 @@ void Job::prepare(const Program& pg, const Stimulus& st, const std::vector<int>&
     HC(hipMemcpyAsync(d_saif_nets_, saif_net_list_.data(), saif_net_list_.size() * 4,
                       hipMemcpyHostToDevice, stream_));
   HC(hipStreamSynchronize(stream_));
-  level_blocks_ = level_max_blocks();
+  cap_ = tile_cap(seg_target ? seg_target : kDefaultSeg);
+  level_blocks_ = level_blocks_opt ? (int)level_blocks_opt : level_max_blocks(cap_);
   stats_ = JobStats();
+  stats_.level_blocks = (uint64_t)level_blocks_;
   stats_.n_nets = n_nets_;
   stats_.n_gates = n_gates_;
   stats_.n_pins = pg.n_pins;
@@ -224,12 +244,13 @@ void Job::prepare(const Program& pg, const Stimulus& st, const std::vector<int>&
 void Job::run_fused(std::vector<hipEvent_t>& ev_eval) {
   uint64_t* bump = d_scal_ + S_BUMP;
   unsigned long long* evc = (unsigned long long*)(d_scal_ + S_EVC);
-  const uint32_t seg = seg_target ? seg_target : 32;
+  const uint32_t seg = seg_target ? seg_target : kDefaultSeg;
   for (size_t l = 0; l + 1 < level_start_.size(); ++l) {
     uint32_t g0 = level_start_[l], ng = level_start_[l + 1] - g0;
     if (ng == 0) continue;
     const GateRec* gates = d_grec_ + g0;
-    HC(launch_plan2(gates, ng, d_wlen_, seg, d_ntile_, d_bound_, d_gnseg_, stream_));
+    HC(launch_plan2(gates, ng, d_wlen_, seg, d_plan_, d_ntile_, d_bound_,
+                    (unsigned int*)(d_scal_ + S_ERR), stream_));
     HC(launch_scan(d_ntile_, d_tilebase_, ng, d_part_, d_scal_ + S_TOT_A, nullptr, stream_));
     HC(launch_scan(d_bound_, d_outoff_, ng, d_part_, d_scal_ + S_TOT_B, bump, stream_));
     uint64_t h[3];
@@ -240,16 +261,17 @@ void Job::run_fused(std::vector<hipEvent_t>& ev_eval) {
       throw GlsError(GLS_E_OOM, "event pool exhausted at level " + std::to_string(l) +
                                     " (raise the pool size or shorten the window)");
     if (ntiles >= 0xFFFFFFF0ull) throw GlsError(GLS_E_RANGE, "too many tiles in one level");
-    ensure((void**)&d_tilemap_, cap_tilemap_, ntiles * 4);
+    ensure((void**)&d_tiles_, cap_tiles_, ntiles * sizeof(TileDesc));
     ensure((void**)&d_tstate_, cap_tstate_, ntiles * 16);
-    HC(launch_tilemap(ng, d_tilebase_, d_ntile_, d_tilemap_, stream_));
+    HC(launch_tile_desc(gates, ng, d_plan_, d_tilebase_, d_ntile_, d_woff_, d_wlen_, d_pool_,
+                        d_tiles_, ntiles, stream_));
     HC(launch_fill_u64(d_tstate_, 2 * ntiles, 0ull, stream_));
     HC(launch_fill_u64((unsigned long long*)(d_scal_ + S_TICKET), 2, 0ull, stream_));  // ticket, ovf
     LevelArgs a;
     a.gates = gates;
-    a.tilemap = d_tilemap_;
-    a.tile_base = d_tilebase_;
-    a.gnseg = d_gnseg_;
+    a.plan = d_plan_;
+    a.tiles = d_tiles_;
+    a.lutp = d_lutp_;
     a.out_base = d_outoff_;
     a.ntiles = ntiles;
     a.woff = d_woff_;
@@ -266,6 +288,8 @@ void Job::run_fused(std::vector<hipEvent_t>& ev_eval) {
     a.err = (unsigned int*)(d_scal_ + S_ERR);
     a.start = start_;
     a.end = end_;
+    a.cap = cap_;
+    a.pad = 0;
     unsigned nb = (unsigned)std::min<uint64_t>(ntiles, (uint64_t)std::max(1, level_blocks_));
     hipEvent_t e_a, e_b;
     HC(hipEventCreate(&e_a));
@@ -369,6 +393,8 @@ void Job::run() {
   if (h[S_ERR] & 1u) throw GlsError(GLS_E_DEVICE, "tile look-back did not complete (spin limit)");
   if (h[S_ERR] & 2u)
     throw GlsError(GLS_E_UNSUPPORTED, "a tile exceeded the overflow stack buffer (event density too skewed)");
+  if (h[S_ERR] & 4u)
+    throw GlsError(GLS_E_RANGE, "a gate's inputs carry 2^30 or more events");
   auto w1 = std::chrono::steady_clock::now();
   stats_.kernel_ms = ms;
   stats_.wall_ms = std::chrono::duration<double, std::milli>(w1 - w0).count();

@@ -34,7 +34,7 @@ class DesignInfo(ctypes.Structure):
 class SimOpts(ctypes.Structure):
     _fields_ = [("pool_bytes", ctypes.c_uint64), ("seg_events", ctypes.c_uint32),
                 ("keep_waves", ctypes.c_uint32), ("path", ctypes.c_uint32),
-                ("reserved", ctypes.c_uint32)]
+                ("level_blocks", ctypes.c_uint32)]
 
 
 PATH_FUSED = 0
@@ -47,7 +47,7 @@ class RunStats(ctypes.Structure):
                 ("n_nets", ctypes.c_uint64), ("n_gates", ctypes.c_uint64),
                 ("n_pins", ctypes.c_uint64), ("kernel_ms", ctypes.c_double),
                 ("wall_ms", ctypes.c_double), ("eval_ms", ctypes.c_double),
-                ("eval_launches", ctypes.c_uint64)]
+                ("eval_launches", ctypes.c_uint64), ("level_blocks", ctypes.c_uint64)]
 
 
 # Every symbol include/gls.h declares: (name, restype, argtypes)
@@ -259,21 +259,21 @@ class Context(_Handle):
         super().__init__(h.value)
 
     @staticmethod
-    def _opts(pool_bytes, seg_events, keep_waves, path):
-        return SimOpts(pool_bytes, seg_events, 1 if keep_waves else 0, path, 0)
+    def _opts(pool_bytes, seg_events, keep_waves, path, level_blocks=0):
+        return SimOpts(pool_bytes, seg_events, 1 if keep_waves else 0, path, level_blocks)
 
     def prepare(self, design, stim, dumpon, dumpoff, pool_bytes=0, seg_events=0, keep_waves=False,
-                path=PATH_FUSED):
+                path=PATH_FUSED, level_blocks=0):
         h = ctypes.c_void_p()
-        o = self._opts(pool_bytes, seg_events, keep_waves, path)
+        o = self._opts(pool_bytes, seg_events, keep_waves, path, level_blocks)
         _check(load().gls_prepare(self.ptr, design.ptr, stim.ptr, dumpon, dumpoff, ctypes.byref(o),
                                   ctypes.byref(h)))
         return Run(h.value)
 
     def simulate(self, design, stim, dumpon, dumpoff, pool_bytes=0, seg_events=0, keep_waves=False,
-                 path=PATH_FUSED):
+                 path=PATH_FUSED, level_blocks=0):
         h = ctypes.c_void_p()
-        o = self._opts(pool_bytes, seg_events, keep_waves, path)
+        o = self._opts(pool_bytes, seg_events, keep_waves, path, level_blocks)
         _check(load().gls_simulate(self.ptr, design.ptr, stim.ptr, dumpon, dumpoff, ctypes.byref(o),
                                    ctypes.byref(h)))
         return Run(h.value)

@@ -83,7 +83,7 @@ typedef struct {
   uint32_t seg_events;   /* target input events per time segment (0: path default) */
   uint32_t keep_waves;   /* 1: keep waveforms for gls_run_waveform */
   uint32_t path;         /* GLS_PATH_FUSED (default) or GLS_PATH_TWO_PASS */
-  uint32_t reserved;
+  uint32_t level_blocks; /* workgroups per fused level launch (0: occupancy limit) */
 } gls_sim_opts;
 
 /* Level-kernel variants: one fused LDS-tiled kernel per level, or the
@@ -114,6 +114,7 @@ typedef struct {
   double wall_ms;      /* host wall time of the level loop */
   double eval_ms;      /* summed device time of the gate-evaluation kernel */
   uint64_t eval_launches;
+  uint64_t level_blocks; /* workgroups per fused level launch */
 } gls_run_stats;
 int gls_run_stats_get(const gls_run* r, gls_run_stats* st);
 /* SAIF writer (DumpSaif, src/simulate.cu:113-481 / src/simulate.cpp:37-380). */
