@@ -56,6 +56,24 @@ def cpu_baseline(budget_events):
                       f"oracle/cpu_ref, 1 thread, {r['sim_seconds']:.1f} s"}
 
 
+def measured_traffic(args, launches):
+    """HBM bytes per launch of the fused level kernel, from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE summary of this same default workload
+    (counters scaled by a plain copy of known size).  None for any other
+    workload or path: counters cannot be read from inside this process."""
+    if (args.path != "fused" or args.cells != 200000 or args.t_end != 20000000
+            or args.seg_events or args.level_blocks):
+        return None
+    p = os.path.join(ROOT, "profiles", "k_level_200k_cells_hbm_counters.txt")
+    try:
+        for line in open(p):
+            if line.startswith("traffic per launch (mean)"):
+                return round(float(line.split()[4]) * 1e9)
+    except OSError:
+        pass
+    return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -67,12 +85,25 @@ def main():
     ap.add_argument("--path", choices=["fused", "two-pass"], default="fused")
     ap.add_argument("--level-blocks", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-events", type=int, default=1500000)
+    ap.add_argument("--cpu-events", type=int, default=700000)
+    ap.add_argument("--copy-probe-mib", type=int, default=0,
+                    help="only run a plain device copy of this many MiB (3 times): the "
+                         "reference point that scales FETCH_SIZE/WRITE_SIZE counters to bytes")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
               "independent jobs would shard across GPUs with no communication.", file=sys.stderr)
         sys.exit(2)
+    if args.copy_probe_mib:
+        import torch
+        n = args.copy_probe_mib << 20
+        a = torch.ones(n, dtype=torch.uint8, device="cuda:0")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes": n, "copies": 3}), flush=True)
+        return
 
     import gls
     # CPU baseline first: the oracle runs as a child process before this
@@ -106,11 +137,15 @@ def main():
     st = stats[-1]
     ms_per_step = sum(walls) / len(walls) * 1e3
     value = st["n_events"] / (ms_per_step / 1e3)
-    # dominant kernel: k_eval. Algorithmic bytes = every input word read once
-    # per reading gate + every output word written once.
+    # dominant kernel: the level kernel (k_level fused / k_eval two-pass).
+    # Algorithmic bytes = every input word read once per reading gate + every
+    # output word written once; time = HIP events around each launch on the
+    # job's stream.
     eval_ms = sum(x["eval_ms"] for x in stats) / len(stats)
     eval_bytes = 4 * (st["in_words"] + st["out_words"])
     achieved = eval_bytes / (eval_ms / 1e3) / 1e9
+    launches = st["eval_launches"]
+    traffic = measured_traffic(args, launches)
     # saif self-check: T0+T1+TX = DURATION for every record
     saif = run.saif_text()
     dur = args.t_end + 1
@@ -138,9 +173,12 @@ def main():
         "alg_bytes": st["alg_bytes"],
         "device_ms_per_step": round(sum(x["kernel_ms"] for x in stats) / len(stats), 3),
         "alg_hbm_gbs_level_loop": round(st["alg_bytes"] / (ms_per_step / 1e3) / 1e9, 1),
-        "roofline": {"bound": "hbm", "kernel": "k_eval", "achieved": round(achieved, 1),
+        "roofline": {"bound": "hbm", "kernel": "k_level" if args.path == "fused" else "k_eval",
+                     "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": None, "launches_per_step": st["eval_launches"],
+                     "traffic": traffic, "launches_per_step": launches,
+                     "alg_bytes_per_launch": round(eval_bytes / max(1, launches)),
+                     "ms_per_launch": round(eval_ms / max(1, launches), 3),
                      "alg_bytes_per_step": eval_bytes, "ms_per_step": round(eval_ms, 3)},
         "saif_durations_consistent": ok,
         "cpu_baseline": cpu,
