@@ -62,7 +62,7 @@ def measured_traffic(args, launches):
     (counters scaled by a plain copy of known size).  None for any other
     workload or path: counters cannot be read from inside this process."""
     if (args.path != "fused" or args.cells != 200000 or args.t_end != 20000000
-            or args.seg_events or args.level_blocks):
+            or args.seg_events or args.level_blocks or args.tiles_per_ticket or args.no_prefetch):
         return None
     p = os.path.join(ROOT, "profiles", "k_level_200k_cells_hbm_counters.txt")
     try:
@@ -84,6 +84,8 @@ def main():
     ap.add_argument("--seg-events", type=int, default=0)
     ap.add_argument("--path", choices=["fused", "two-pass"], default="fused")
     ap.add_argument("--level-blocks", type=int, default=0)
+    ap.add_argument("--tiles-per-ticket", type=int, default=0)
+    ap.add_argument("--no-prefetch", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-events", type=int, default=700000)
     ap.add_argument("--copy-probe-mib", type=int, default=0,
@@ -120,7 +122,8 @@ def main():
     ctx = gls.Context(0)
     path = gls.PATH_FUSED if args.path == "fused" else gls.PATH_TWO_PASS
     run = ctx.prepare(d, s, 0, args.t_end + 1, seg_events=args.seg_events, path=path,
-                      level_blocks=args.level_blocks)
+                      level_blocks=args.level_blocks, tiles_per_ticket=args.tiles_per_ticket,
+                      prefetch=not args.no_prefetch)
     log(f"bench: prepared (program + stimulus resident in HBM), "
         f"{run.stats()['level_blocks']} workgroups per level launch")
     for i in range(args.warmup):

@@ -298,6 +298,8 @@ int gls_prepare(gls_ctx* c, const gls_design* d, const gls_stim* s, uint64_t dum
     if (opts->path > GLS_PATH_TWO_PASS) throw GlsError(GLS_E_ARG, "unknown path");
     r->job->mode = (int)opts->path;
     r->job->level_blocks_opt = opts->level_blocks;
+    r->job->tiles_per_ticket = opts->tiles_per_ticket;
+    r->job->prefetch = opts->no_prefetch ? 0u : 1u;
   }
   r->job->prepare(r->pg, s->st, r->stim_nets, dumpon, dumpoff, opts ? opts->pool_bytes / 4 : 0);
   *out = r.release();

@@ -1,5 +1,7 @@
 // Records shared between the host packer and the HIP kernels.
 #pragma once
+#include <hip/hip_runtime.h>
+
 #include <cstdint>
 
 namespace gls {
@@ -23,24 +25,28 @@ struct alignas(16) GateDev {
 };
 static_assert(sizeof(GateDev) == 80, "GateDev layout");
 
-// Gate record of the fused level kernel, 160 bytes.  Delays are folded into
-// two lookup tables per input j, indexed by the output edge class s (1..4)
-// and by whether input j fell at this time:
-//   tmin[j][s-1][fall]: contribution of j to the minimum taken when several
-//                       inputs have an entry (0xFFFF: no related IOPATH),
-//   tone[j][s-1][fall]: the delay when j is the only input with an entry.
+// Gate record of the fused level kernel, 32 bytes.  The delay of an output
+// event is one lookup in the gate's delay table (u16 entries at dtab +
+// dt_off) indexed by dt_index(): the output edge class s (1..4), the set of
+// inputs with an event at this time, and the subset of those that fell.
 struct alignas(16) GateRec {
   uint32_t in_net[4];
   uint32_t out_net;
+  uint32_t dt_off;
   uint16_t lut;
   uint8_t k;
   uint8_t pad0;
   uint16_t dmax;
-  uint16_t pad1[3];
-  uint16_t tmin[4][4][2];
-  uint16_t tone[4][4][2];
+  uint16_t pad1;
 };
-static_assert(sizeof(GateRec) == 160, "GateRec layout");
+static_assert(sizeof(GateRec) == 32, "GateRec layout");
+
+// Delay-table entries of a gate with k inputs, and the index of one entry.
+inline constexpr uint32_t dt_size(uint32_t k) { return 4u << (2 * k); }
+__host__ __device__ inline uint32_t dt_index(uint32_t k, uint32_t s, uint32_t mask,
+                                             uint32_t fell) {
+  return ((s - 1) << (2 * k)) | (mask << k) | fell;
+}
 
 // Per-gate segmentation of one level: segment c covers the leader input's
 // events [c*stride, (c+1)*stride).
@@ -48,14 +54,20 @@ struct GatePlan {
   uint32_t nseg, stride, lead, pad;
 };
 
-// Per-tile input ranges: L[j] = first event of input j at or after the
-// tile's start time, E[j] = end of the range including the look-ahead halo.
-struct TileDesc {
-  uint32_t g, ti;
+// Everything one tile of the fused level kernel needs, 128 bytes, written by
+// k_tile_desc.  Input j's range covers events [B, E) of its waveform with
+// B = L - 1 (L > 0) or 0: L = first event at or after the tile's start time,
+// E = end of the range including the look-ahead halo; in_off = pool word of
+// event B.
+struct alignas(16) TileDesc {
+  uint64_t in_off[4];
   uint32_t L[4], E[4];
-  uint32_t pad[2];
+  uint32_t lutw[4];                // packed truth table words (fan-in <= 3)
+  uint32_t g, ti, nseg, stride;    // gate (index in the level), tile of the gate, segments
+  uint32_t lead, k, dt_off, lut;   // leader input, fan-in, delay table, truth table id
+  uint32_t dmax, out_net, pad[2];
 };
-static_assert(sizeof(TileDesc) == 48, "TileDesc layout");
+static_assert(sizeof(TileDesc) == 128, "TileDesc layout");
 
 // Arguments of the fused level kernel.
 struct LevelArgs {
@@ -63,6 +75,7 @@ struct LevelArgs {
   const GatePlan* plan;       // per gate
   const TileDesc* tiles;      // per tile
   const uint32_t* lutp;       // truth tables packed 2 bits per entry, 16 words each
+  const uint16_t* dtab;       // delay tables (GateRec::dt_off)
   const uint64_t* out_base;   // per gate: output region (pool word offset)
   uint64_t ntiles;
   uint64_t* woff;
@@ -79,12 +92,17 @@ struct LevelArgs {
   unsigned int* err;
   unsigned long long start, end;
   uint32_t cap;                // LDS input capacity per tile (words)
+  uint32_t blk;                // consecutive tiles handed out per ticket
+  uint32_t prefetch;           // 1: stage the next tile before storing this one
   uint32_t pad;
 };
 
 // LDS bytes of one fused-level workgroup for an input capacity of `cap`
-// words: inputs (cap + 64), stacks (cap + 64), truth table, delay tables.
-inline size_t level_lds_bytes(uint32_t cap) { return (2ull * (cap + 64)) * 4 + 256 + 128; }
+// words: inputs (cap + 128: one sentinel per input range, and the tile's
+// output staging), stacks (cap + 64), truth table (256 B), delay table (2 KiB).
+inline size_t level_lds_bytes(uint32_t cap) {
+  return (2ull * cap + 192) * 4 + 256 + 2 * dt_size(4);
+}
 
 // Per time-segment summary written by the evaluation kernel.
 struct SegSum {

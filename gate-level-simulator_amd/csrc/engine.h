@@ -48,6 +48,8 @@ class Job {
   uint32_t seg_target = 0;  // input events per time segment (0: mode default)
   int mode = 0;             // 0: fused LDS-tiled level kernel, 1: two-pass kernels
   uint32_t level_blocks_opt = 0;  // workgroups per fused launch (0: occupancy limit)
+  uint32_t tiles_per_ticket = 0;  // fused tiles handed out per ticket (0: one)
+  uint32_t prefetch = 1;          // stage the next tile before storing the current one
 
  private:
   void ensure(void** p, size_t& cap, size_t need);
@@ -85,12 +87,13 @@ class Job {
   GatePlan* d_plan_ = nullptr;
   TileDesc* d_tiles_ = nullptr;
   uint32_t* d_lutp_ = nullptr;
+  uint16_t* d_dtab_ = nullptr;
   unsigned long long* d_tstate_ = nullptr;
   uint32_t* d_ovf_ = nullptr;
   uint64_t ovf_words_ = 0;
   int level_blocks_ = 0;
   uint32_t cap_ = 0;                          // LDS input words per fused tile
-  static constexpr uint32_t kDefaultSeg = 64;  // input events per time segment
+  static constexpr uint32_t kDefaultSeg = 32;  // input events per time segment
   std::vector<uint64_t> h_woff_init_;
   std::vector<uint32_t> h_wlen_init_;
   JobStats stats_;

@@ -40,8 +40,8 @@ hipError_t launch_plan2(const GateRec* gates, uint32_t ng, const uint32_t* wlen,
                         uint32_t seg_target, GatePlan* plan, uint64_t* ntile, uint64_t* bound,
                         unsigned int* err, hipStream_t st);
 hipError_t launch_tile_desc(const GateRec* gates, uint32_t ng, const GatePlan* plan,
-                            const uint64_t* tile_base, const uint64_t* ntile, const uint64_t* woff,
-                            const uint32_t* wlen, const uint32_t* pool, TileDesc* tiles,
+                            const uint64_t* tile_base, const uint64_t* woff, const uint32_t* wlen,
+                            const uint32_t* pool, const uint32_t* lutp, TileDesc* tiles,
                             uint64_t ntiles, hipStream_t st);
 hipError_t launch_fill_u64(unsigned long long* p, uint64_t n, unsigned long long v, hipStream_t st);
 hipError_t launch_level(const LevelArgs& a, unsigned nblocks, hipStream_t st);
@@ -73,7 +73,7 @@ Job::~Job() {
   void* ptrs[] = {d_gates_, d_grec_, d_luts_, d_woff_, d_wlen_, d_pool_, d_saif_, d_saif_nets_,
                   d_scal_, d_nseg_, d_scr_, d_segbase_, d_scrbase_, d_outlen_, d_outoff_, d_part_,
                   d_gnseg_, d_segmap_, d_sums_, d_carry_, d_scratch_, d_ntile_, d_bound_,
-                  d_tilebase_, d_plan_, d_tiles_, d_lutp_, d_tstate_, d_ovf_};
+                  d_tilebase_, d_plan_, d_tiles_, d_lutp_, d_dtab_, d_tstate_, d_ovf_};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
 }
@@ -92,10 +92,18 @@ static uint32_t sel_delay(uint32_t r, uint32_t f, int s) {
   return s == 1 ? r : (s == 2 ? f : (s == 3 ? std::max(r, f) : std::min(r, f)));
 }
 
-// Fold AddDelay (src/simulate.cpp:1012-1176) into per-input tables indexed
-// by the output edge class and by whether the input fell at this time.
-static void fill_delay_tables(const GateDelays& D, int k, GateRec& r) {
+// AddDelay (src/simulate.cpp:1012-1176, asserts compiled out) folded into
+// one table per gate, indexed by dt_index(k, s, mask, fell): s = output edge
+// class (1..4), mask = inputs with an event at this time, fell = the subset
+// of those whose own edge class is "fall" (CheckDelay == 2).
+//   * one input with an event: with a single IOPATH record for that input the
+//     record is used whatever its edge; with several, the first record whose
+//     edge matches the input's edge (fall -> negedge, else posedge), else 1;
+//   * several inputs: the minimum over every edge-independent record and
+//     every record whose edge matches its input's edge, else 1.
+static void fill_delay_table(const GateDelays& D, int k, uint16_t* tab) {
   const uint16_t NO = kNoDelay;
+  uint32_t tmin[4][4][2], tone[4][4][2];
   for (int j = 0; j < 4; ++j)
     for (int s = 1; s <= 4; ++s)
       for (int fall = 0; fall < 2; ++fall) {
@@ -112,8 +120,24 @@ static void fill_delay_tables(const GateDelays& D, int k, GateRec& r) {
             one = kind[0] != NO ? sel_delay(kind[0], kind[1], s) : 1;
           }
         }
-        r.tmin[j][s - 1][fall] = (uint16_t)m;
-        r.tone[j][s - 1][fall] = (uint16_t)one;
+        tmin[j][s - 1][fall] = m;
+        tone[j][s - 1][fall] = one;
+      }
+  const uint32_t full = (1u << k) - 1;
+  for (uint32_t s = 1; s <= 4; ++s)
+    for (uint32_t mask = 0; mask <= full; ++mask)
+      for (uint32_t fell = 0; fell <= full; ++fell) {
+        uint32_t d = 1;
+        if (mask != 0 && (mask & (mask - 1)) == 0) {
+          const int j = __builtin_ctz(mask);
+          d = tone[j][s - 1][(fell >> j) & 1u];
+        } else if (mask != 0) {
+          uint32_t m = 0xFFFF;
+          for (int j = 0; j < k; ++j)
+            if ((mask >> j) & 1u) m = std::min(m, tmin[j][s - 1][(fell >> j) & 1u]);
+          d = m == 0xFFFF ? 1u : m;
+        }
+        tab[dt_index((uint32_t)k, s, mask, fell)] = (uint16_t)d;
       }
 }
 
@@ -131,6 +155,7 @@ void Job::prepare(const Program& pg, const Stimulus& st, const std::vector<int>&
   // gate records for both paths
   std::vector<GateDev> gd(n_gates_);
   std::vector<GateRec> gr(n_gates_);
+  std::vector<uint16_t> dtab;
   gate_in_nets_.assign(n_gates_, {});
   for (uint32_t i = 0; i < n_gates_; ++i) {
     const CompiledGate& g = pg.gates[i];
@@ -154,7 +179,12 @@ void Job::prepare(const Program& pg, const Stimulus& st, const std::vector<int>&
     r.lut = q.lut = (uint16_t)g.lut;
     r.k = q.k = (uint8_t)g.k;
     r.dmax = q.dmax = D.dmax;
-    fill_delay_tables(D, g.k, q);
+    if (g.k > 4) throw GlsError(GLS_E_UNSUPPORTED, "gate with more than 4 distinct inputs");
+    if (dtab.size() + dt_size(g.k) >= (1ull << 32))
+      throw GlsError(GLS_E_RANGE, "delay tables exceed 2^32 entries");
+    q.dt_off = (uint32_t)dtab.size();
+    dtab.resize(dtab.size() + dt_size(g.k));
+    fill_delay_table(D, g.k, dtab.data() + q.dt_off);
   }
   std::vector<uint8_t> luts(pg.luts.size() * 256, 1);
   for (size_t i = 0; i < pg.luts.size(); ++i)
@@ -212,6 +242,10 @@ void Job::prepare(const Program& pg, const Stimulus& st, const std::vector<int>&
       lutp[i * 16 + e / 16] |= (uint32_t)(pg.luts[i][e] & 3u) << (2 * (e % 16));
   HC(hipMalloc((void**)&d_lutp_, lutp.size() * 4));
   HC(hipMemcpyAsync(d_lutp_, lutp.data(), lutp.size() * 4, hipMemcpyHostToDevice, stream_));
+  HC(hipMalloc((void**)&d_dtab_, std::max<size_t>(1, dtab.size()) * 2));
+  if (!dtab.empty())
+    HC(hipMemcpyAsync(d_dtab_, dtab.data(), dtab.size() * 2, hipMemcpyHostToDevice, stream_));
+  HC(hipStreamSynchronize(stream_));
 
   size_t free_b = 0, total_b = 0;
   HC(hipMemGetInfo(&free_b, &total_b));
@@ -263,7 +297,7 @@ void Job::run_fused(std::vector<hipEvent_t>& ev_eval) {
     if (ntiles >= 0xFFFFFFF0ull) throw GlsError(GLS_E_RANGE, "too many tiles in one level");
     ensure((void**)&d_tiles_, cap_tiles_, ntiles * sizeof(TileDesc));
     ensure((void**)&d_tstate_, cap_tstate_, ntiles * 16);
-    HC(launch_tile_desc(gates, ng, d_plan_, d_tilebase_, d_ntile_, d_woff_, d_wlen_, d_pool_,
+    HC(launch_tile_desc(gates, ng, d_plan_, d_tilebase_, d_woff_, d_wlen_, d_pool_, d_lutp_,
                         d_tiles_, ntiles, stream_));
     HC(launch_fill_u64(d_tstate_, 2 * ntiles, 0ull, stream_));
     HC(launch_fill_u64((unsigned long long*)(d_scal_ + S_TICKET), 2, 0ull, stream_));  // ticket, ovf
@@ -272,6 +306,7 @@ void Job::run_fused(std::vector<hipEvent_t>& ev_eval) {
     a.plan = d_plan_;
     a.tiles = d_tiles_;
     a.lutp = d_lutp_;
+    a.dtab = d_dtab_;
     a.out_base = d_outoff_;
     a.ntiles = ntiles;
     a.woff = d_woff_;
@@ -289,8 +324,12 @@ void Job::run_fused(std::vector<hipEvent_t>& ev_eval) {
     a.start = start_;
     a.end = end_;
     a.cap = cap_;
-    a.pad = 0;
     unsigned nb = (unsigned)std::min<uint64_t>(ntiles, (uint64_t)std::max(1, level_blocks_));
+    // one tile per ticket by default: a wavefront holding several tiles of a
+    // gate makes the gate's later tiles wait for all of them in turn
+    a.blk = tiles_per_ticket ? tiles_per_ticket : 1u;
+    a.prefetch = prefetch;
+    a.pad = 0;
     hipEvent_t e_a, e_b;
     HC(hipEventCreate(&e_a));
     HC(hipEventCreate(&e_b));

@@ -8,12 +8,13 @@
 //      k_tile_desc: one event before the tile for the starting values and a
 //      D_max look-ahead halo after it) are copied into LDS with batched,
 //      coalesced loads;
-//   2. every lane merges its segment's input events in time order, evaluates
-//      the 4-state truth table (packed in registers for fan-in <= 3),
-//      classifies the output edge, picks the SDF delay (AddDelay folded into
-//      per-gate tables, the common rise/fall cases in registers) and applies
-//      the preemption rule as a strict suffix-min filter (stack in LDS);
-//      events in the halo only pre-empt;
+//   2. every lane merges its segment's input events in time order (branch
+//      free: each LDS range ends with an INF sentinel), evaluates the 4-state
+//      truth table (packed in registers for fan-in <= 3), classifies the
+//      output edge, picks the SDF delay with one lookup in the gate's delay
+//      table (AddDelay folded on the host, copied to LDS per tile) and applies
+//      the preemption rule on a stack in LDS, counting value changes as it
+//      goes; events in the halo only pre-empt;
 //   3. a wavefront scan combines the lanes' kept lists (dedup of equal
 //      adjacent values across lanes), a decoupled look-back over the gate's
 //      earlier tiles supplies the carry (last value, output count), and the
@@ -44,11 +45,6 @@ __device__ __forceinline__ uint32_t edge_cls(uint32_t pre, uint32_t cur) {
   // CheckDelay (src/simulate.cpp:1178-1201); codes 0=0 1=X 2=Z 3=1
   const uint64_t T = 0x0222ull << 48 | 0x1042ull << 32 | 0x1302ull << 16 | 0x1110ull;
   return (uint32_t)(T >> ((pre * 4 + cur) * 4)) & 0xF;
-}
-
-// input j fell at this time (CheckDelay(last, cur) == 2)
-__device__ __forceinline__ uint32_t fell(uint32_t pre, uint32_t cur) {
-  return edge_cls(pre, cur) == 2 ? 1u : 0u;
 }
 
 // Aggregate of a run of kept candidates (for dedup + SAIF carry):
@@ -170,7 +166,6 @@ struct TileCtx {
   uint32_t B[K], E[K], L[K];
   uint32_t nseg, stride, lead, dmax;
   uint32_t lut[4];          // packed truth table, fan-in <= 3
-  uint64_t tmin2[K], tone2[K];  // delay tables for rise/fall output edges
 };
 
 template <int K>
@@ -187,50 +182,88 @@ __device__ __forceinline__ uint32_t lut_eval(const TileCtx<K>& X, const uint8_t*
   return s_lut[idx];
 }
 
-// AddDelay (src/simulate.cpp:1012-1176) on the folded tables.
-template <int K>
-__device__ __forceinline__ uint32_t delay_of(const TileCtx<K>& X, const uint16_t* s_tmin,
-                                             const uint16_t* s_tone, uint32_t mask, uint32_t sc,
-                                             const uint32_t (&pv)[K], const uint32_t (&cur)[K]) {
-  const bool fast = sc <= 2;  // rise / fall: tables in registers
-  if (__popc(mask) == 1) {
-    uint32_t d = 1;
-#pragma unroll
-    for (int j = 0; j < K; ++j) {
-      if (mask == (1u << j)) {
-        const uint32_t f = fell(pv[j], cur[j]);
-        const uint32_t q = (sc - 1) * 2 + f;
-        d = fast ? (uint32_t)(X.tone2[j] >> (16 * q)) & 0xFFFFu : s_tone[j * 8 + q];
-      }
-    }
-    return d;
-  }
-  uint32_t m = 0xFFFFu;
-#pragma unroll
-  for (int j = 0; j < K; ++j) {
-    if ((mask >> j) & 1u) {
-      const uint32_t f = fell(pv[j], cur[j]);
-      const uint32_t q = (sc - 1) * 2 + f;
-      const uint32_t v = fast ? (uint32_t)(X.tmin2[j] >> (16 * q)) & 0xFFFFu : s_tmin[j * 8 + q];
-      m = min(m, v);
-    }
-  }
-  return m == 0xFFFFu ? 1u : m;
+// Inputs that fell (CheckDelay == 2: to 0 from X/Z/1, or from 1 to X/Z)
+// between the packed input values p and c (2 bits per input), as a bit mask.
+__device__ __forceinline__ uint32_t fell_bits(uint32_t p, uint32_t c) {
+  const uint32_t M = 0x55u;
+  const uint32_t c0 = ~(c | (c >> 1)) & M;   // now 0
+  const uint32_t pn0 = (p | (p >> 1)) & M;   // was not 0
+  const uint32_t p1 = p & (p >> 1) & M;      // was 1
+  const uint32_t cxz = (c ^ (c >> 1)) & M;   // now X or Z
+  uint32_t x = (c0 & pn0) | (p1 & cxz);      // bit 2j per input j
+  x = (x | (x >> 1)) & 0x33u;
+  return (x | (x >> 2)) & 0x0Fu;
 }
 
+// Remove the top entry of a lane's stack, keeping the count of value changes.
+__device__ __forceinline__ void stack_pop(const uint32_t* st, uint32_t& sp, uint32_t& top,
+                                          uint32_t& cnt) {
+  --sp;
+  const uint32_t nt = sp ? st[sp - 1] : 0u;
+  cnt -= (sp == 0 || ((nt ^ top) & 3u)) ? 1u : 0u;
+  top = nt;
+}
+
+// What one lane's walk over its segment leaves behind: its stack of kept
+// candidates (st[0..sp)), the top entry, the number of value changes in the
+// stack, and the number of gate events it owned.
+struct Lane {
+  uint32_t* st;
+  uint32_t sp, top, cnt, nown;
+};
+
+typedef __attribute__((address_space(3))) void lds_void;
+
+// One 4-byte direct-to-LDS load per lane: lane i's word lands at l + i.
+__device__ __forceinline__ void dma_word(const uint32_t* g, uint32_t* l) {
+  __builtin_amdgcn_global_load_lds(g, (lds_void*)l, 4, 0, 0);
+}
+
+// Wait until this wavefront has no vector-memory access in flight (direct-to-
+// LDS loads included): s_waitcnt vmcnt(0).
+__device__ __forceinline__ void wait_vm() { __builtin_amdgcn_s_waitcnt(0x0F70); }
+
+__device__ __forceinline__ uint32_t range_start(uint32_t L) { return L > 0 ? L - 1 : 0u; }
+
+__device__ __forceinline__ uint32_t tile_words(const TileDesc& D) {
+  uint32_t total = 0;
+  for (uint32_t j = 0; j < D.k; ++j) total += D.E[j] - range_start(D.L[j]);
+  return total;
+}
+
+// Issue the loads of tile D into LDS: its input ranges (range j at
+// sum_{i<j}(n_i + 1), leaving one word for the sentinel) when they fit, its
+// delay table and (fan-in 4) its truth table.  Completion: wait_vm().
+__device__ __forceinline__ void stage_tile(const LevelArgs& A, const TileDesc& D, bool fits,
+                                           uint32_t* s_in, uint32_t* s_lut, uint32_t* s_dt) {
+  const uint32_t lane = threadIdx.x;
+  if (fits) {
+    uint32_t pos = 0;
+    for (uint32_t j = 0; j < D.k; ++j) {
+      const uint32_t n = D.E[j] - range_start(D.L[j]);
+      const uint32_t* src = A.pool + D.in_off[j];
+      for (uint32_t i = 0; i < n; i += kLanes)
+        if (i + lane < n) dma_word(src + i + lane, s_in + pos + i);
+      pos += n + 1;
+    }
+  }
+  const uint32_t nw = dt_size(D.k) / 2;
+  const uint32_t* dsrc = (const uint32_t*)(A.dtab + D.dt_off);
+  for (uint32_t i = 0; i < nw; i += kLanes)
+    if (i + lane < nw) dma_word(dsrc + i + lane, s_dt + i);
+  if (D.k == 4) dma_word((const uint32_t*)(A.luts + D.lut * 256u) + lane, s_lut);
+}
+
+// One lane's walk over its time segment: merge the inputs in time order,
+// evaluate, pick the delay, keep the preemption stack.
 template <int K, bool LDS>
-__device__ __forceinline__ void run_tile(const LevelArgs& A, const GateRec& G, uint32_t g,
-                                         uint32_t ti, uint64_t tk, const TileCtx<K>& X,
-                                         uint32_t* s_in, const uint8_t* s_lut,
-                                         const uint16_t* s_tmin, const uint16_t* s_tone,
-                                         uint32_t* stk_base, unsigned long long& wave_ev) {
+__device__ __forceinline__ Lane lane_walk(const TileCtx<K>& X, uint32_t ti, const uint8_t* s_lut,
+                                          const uint16_t* s_dt, uint32_t* stk_base) {
   const int lane = threadIdx.x;
   const uint32_t c = ti * kLanes + lane;
-  const bool active = c < X.nseg;
   const uint32_t kX = 1u;
   const uint32_t INF = 0xFFFFFFFFu;
-
-  uint32_t r[K], cur[K], h[K], bound[K];
+  uint32_t r[K], h[K], bound[K];
   const uint32_t* srcL = X.src[0];
   uint32_t BL = X.B[0];
 #pragma unroll
@@ -241,9 +274,11 @@ __device__ __forceinline__ void run_tile(const LevelArgs& A, const GateRec& G, u
       BL = X.B[j];
     }
   }
-  uint32_t sp = 0, top = 0, nown = 0, hmin = INF;
-  uint32_t* st = stk_base;
-  if (active) {
+  Lane ln;
+  ln.st = stk_base;
+  // sp: stack depth, top: top entry, cnt: value changes in the stack
+  uint32_t sp = 0, top = 0, cnt = 0, nown = 0, hmin = INF;
+  if (c < X.nseg) {
     // segment bounds from the leader
     uint32_t tau_c = 0, tau_n = 0x3FFFFFFFu, halo_end = 0x3FFFFFFFu;
     const uint32_t pl = c * X.stride;
@@ -252,7 +287,7 @@ __device__ __forceinline__ void run_tile(const LevelArgs& A, const GateRec& G, u
       tau_n = srcL[pl + X.stride - BL] >> 2;
       halo_end = tau_n + X.dmax;
     }
-    uint32_t soff = lane;
+    uint32_t soff = lane, idx = 0;
 #pragma unroll
     for (int j = 0; j < K; ++j) {
       uint32_t pj;
@@ -261,79 +296,139 @@ __device__ __forceinline__ void run_tile(const LevelArgs& A, const GateRec& G, u
       else pj = X.B[j] + lb_words(X.src[j], bound[j], tau_c << 2);
       soff += pj - X.L[j];
       r[j] = pj - X.B[j];
-      cur[j] = pj > 0 ? (X.src[j][r[j] - 1] & 3u) : kX;
-      h[j] = r[j] < bound[j] ? X.src[j][r[j]] : INF;
+      idx |= (pj > 0 ? (X.src[j][r[j] - 1] & 3u) : kX) << (2 * j);
+      // LDS ranges end with an INF sentinel
+      h[j] = (LDS || r[j] < bound[j]) ? X.src[j][r[j]] : INF;
     }
-    st = stk_base + soff;
-    uint32_t li = 0;
-#pragma unroll
-    for (int j = 0; j < K; ++j) li |= cur[j] << (2 * j);
-    uint32_t fprev = c == 0 ? kX : lut_eval<K>(X, s_lut, li);
+    uint32_t* st = stk_base + soff;
+    ln.st = st;
+    uint32_t fprev = c == 0 ? kX : lut_eval<K>(X, s_lut, idx);
     if (c == 0) {  // every output starts with (0, X) (src/simulate.cpp:882-887)
       top = kX;
-      st[sp++] = top;
+      st[0] = top;
+      sp = 1;
+      cnt = 1;
     }
     for (;;) {
       uint32_t w = h[0];
 #pragma unroll
       for (int j = 1; j < K; ++j) w = min(w, h[j]);
       const uint32_t t = w >> 2;
-      if (w == INF || t >= halo_end) break;
+      if (t >= halo_end) break;  // also the end of every input (INF)
+      // every input with an event at time t
       const uint32_t lim = w | 3u;
-      uint32_t pv[K], mask = 0, idx = 0;
+      uint32_t mask = 0, nidx = idx;
 #pragma unroll
       for (int j = 0; j < K; ++j) {
-        pv[j] = cur[j];
-        if (h[j] <= lim) {
-          cur[j] = h[j] & 3u;
-          mask |= 1u << j;
-          ++r[j];
-          h[j] = r[j] < bound[j] ? X.src[j][r[j]] : INF;
-        }
-        idx |= cur[j] << (2 * j);
+        const bool take = h[j] <= lim;
+        mask |= take ? (1u << j) : 0u;
+        nidx = take ? ((nidx & ~(3u << (2 * j))) | ((h[j] & 3u) << (2 * j))) : nidx;
+        r[j] += take ? 1u : 0u;
       }
-      const uint32_t f = lut_eval<K>(X, s_lut, idx);
+#pragma unroll
+      for (int j = 0; j < K; ++j) h[j] = (LDS || r[j] < bound[j]) ? X.src[j][r[j]] : INF;
+      const uint32_t f = lut_eval<K>(X, s_lut, nidx);
       const uint32_t sc = edge_cls(fprev, f);
+      const uint32_t fl = fell_bits(idx, nidx);
+      idx = nidx;
       fprev = f;
       const bool own = t < tau_n;
       nown += own ? 1u : 0u;
-      if (sc == 0) continue;
-      const uint32_t o = t + delay_of<K>(X, s_tmin, s_tone, mask, sc, pv, cur);
-      if (own) {
-        while (sp > 0 && (top >> 2) >= o) {
-          --sp;
-          top = sp ? st[sp - 1] : 0u;
+      if (sc != 0) {
+        const uint32_t o = t + s_dt[dt_index(K, sc, mask, fl)];
+        if (own) {
+          // preemption: drop queued entries at or after o (src/simulate.cpp:991-1007)
+          while (sp > 0 && (top >> 2) >= o) stack_pop(st, sp, top, cnt);
+          const uint32_t e = (o << 2) | f;
+          cnt += (sp == 0 || ((e ^ top) & 3u)) ? 1u : 0u;
+          st[sp++] = e;
+          top = e;
+        } else {
+          hmin = min(hmin, o);
         }
-        top = (o << 2) | f;
-        st[sp++] = top;
-      } else {
-        hmin = min(hmin, o);
       }
     }
-    while (sp > 0 && (top >> 2) >= hmin) {
-      --sp;
-      top = sp ? st[sp - 1] : 0u;
-    }
+    // the next segment's candidates pre-empt this segment's tail
+    while (sp > 0 && (top >> 2) >= hmin) stack_pop(st, sp, top, cnt);
   }
+  ln.sp = sp;
+  ln.top = top;
+  ln.cnt = cnt;
+  ln.nown = nown;
+  return ln;
+}
 
-  // lane aggregate over its kept candidates
-  Agg mine;
-  mine.cnt = 0;
-  mine.has = sp > 0 ? 1u : 0u;
-  mine.fv = mine.lv = mine.lt = 0;
-  {
-    uint32_t pvv = 4;
-    for (uint32_t i = 0; i < sp; ++i) {
-      uint32_t e = st[i];
-      if ((e & 3u) != pvv) mine.cnt++;
-      pvv = e & 3u;
-    }
-    if (sp) {
-      mine.fv = st[0] & 3u;
-      mine.lv = top & 3u;
-      mine.lt = top >> 2;
+// Tile context and walk for fan-in K.  Tiles that do not fit LDS read their
+// inputs from HBM and keep their stacks in the overflow buffer.
+template <int K>
+__device__ __forceinline__ Lane walk_k(const LevelArgs& A, const TileDesc& D, uint64_t tk,
+                                       bool fits, const uint32_t* s_in, uint32_t* s_st,
+                                       const uint8_t* s_lut, const uint16_t* s_dt) {
+  const int lane = threadIdx.x;
+  TileCtx<K> X;
+  X.nseg = D.nseg;
+  X.stride = D.stride;
+  X.lead = D.lead;
+  X.dmax = D.dmax;
+#pragma unroll
+  for (int q = 0; q < 4; ++q) X.lut[q] = D.lutw[q];
+  uint32_t pos = 0, total = 0;
+#pragma unroll
+  for (int j = 0; j < K; ++j) {
+    X.L[j] = D.L[j];
+    X.E[j] = D.E[j];
+    X.B[j] = range_start(D.L[j]);
+    const uint32_t n = X.E[j] - X.B[j];
+    X.src[j] = fits ? s_in + pos : A.pool + D.in_off[j];
+    pos += n + 1;
+    total += n;
+  }
+  if (fits) return lane_walk<K, true>(X, D.ti, s_lut, s_dt, s_st);
+  unsigned long long base = 0;
+  const unsigned long long need = (unsigned long long)total + kLanes;
+  if (lane == 0) base = atomicAdd(A.ovf_used, need);
+  base = __shfl(base, 0, 64);
+  if (base + need > A.ovf_cap) {
+    if (lane == 0) atomicOr(A.err, ERR_OVERFLOW);
+    base = 0;  // results are invalid (the error is reported); keep every wavefront moving
+    if (need > A.ovf_cap) {
+      Lane ln;
+      ln.st = A.ovf;
+      ln.sp = ln.top = ln.cnt = ln.nown = 0;
+      return ln;
     }
   }
+  return lane_walk<K, false>(X, D.ti, s_lut, s_dt, A.ovf + base);
+}
+
+// Combine the lanes' stacks into the tile's output: wavefront scan of the
+// lane aggregates, decoupled look-back over the gate's earlier tiles, dedup
+// against the carry, stores, SAIF durations (summed per lane in 32 bits:
+// times < 2^30) and event count.
+// Wavefront scan of the lane aggregates and the gate prefix before the tile.
+// chain: this wavefront finished the gate's previous tile just before, and
+// prev_incl is its inclusive prefix (no look-back needed).
+struct TileSum {
+  Agg carry;          // everything before this lane's first entry (gate prefix + earlier lanes)
+  Agg incl;           // inclusive prefix of the gate through this tile
+  uint32_t tile_out0; // output index of the tile's first entry
+  uint32_t e_first;   // the lane's first stack entry
+};
+
+__device__ __forceinline__ TileSum combine_tile(const LevelArgs& A, const TileDesc& D,
+                                                uint64_t tk, const Lane& ln, bool chain,
+                                                const Agg& prev_incl) {
+  const int lane = threadIdx.x;
+  const uint32_t ti = D.ti;
+  const uint32_t sp = ln.sp;
+  const uint32_t* st = ln.st;
+  const uint32_t e_first = sp ? st[0] : 0u;
+  Agg mine;
+  mine.cnt = ln.cnt;
+  mine.has = sp > 0 ? 1u : 0u;
+  mine.fv = e_first & 3u;
+  mine.lv = ln.top & 3u;
+  mine.lt = ln.top >> 2;
   // wavefront inclusive scan of lane aggregates
   Agg inc = mine;
 #pragma unroll
@@ -347,7 +442,9 @@ __device__ __forceinline__ void run_tile(const LevelArgs& A, const GateRec& G, u
 
   // decoupled look-back over the gate's earlier tiles, 64 predecessors per
   // round: lane i reads tile tk-1-i (tiles before the gate's first one act
-  // as an empty inclusive prefix)
+  // as an empty inclusive prefix).  A look-back word carries its whole
+  // payload, so relaxed device-scope atomics suffice; release/acquire would
+  // write back and invalidate the XCD's L2 on every tile.
   unsigned long long* ts = A.tstate;
   Agg excl;
   excl.has = 0;
@@ -355,11 +452,16 @@ __device__ __forceinline__ void run_tile(const LevelArgs& A, const GateRec& G, u
   excl.fv = excl.lv = excl.lt = 0;
   if (ti == 0) {
     if (lane == 0)
-      __hip_atomic_store(&ts[2 * tk + 1], pack_inc(tile_agg), __ATOMIC_RELEASE,
+      __hip_atomic_store(&ts[2 * tk + 1], pack_inc(tile_agg), __ATOMIC_RELAXED,
+                         __HIP_MEMORY_SCOPE_AGENT);
+  } else if (chain) {
+    excl = prev_incl;
+    if (lane == 0)
+      __hip_atomic_store(&ts[2 * tk + 1], pack_inc(agg_combine(excl, tile_agg)), __ATOMIC_RELAXED,
                          __HIP_MEMORY_SCOPE_AGENT);
   } else {
     if (lane == 0)
-      __hip_atomic_store(&ts[2 * tk], pack_agg(tile_agg), __ATOMIC_RELEASE,
+      __hip_atomic_store(&ts[2 * tk], pack_agg(tile_agg), __ATOMIC_RELAXED,
                          __HIP_MEMORY_SCOPE_AGENT);
     const uint64_t first = tk - ti;  // the gate's first tile
     Agg acc;                         // combination of the tiles after the window
@@ -373,9 +475,9 @@ __device__ __forceinline__ void run_tile(const LevelArgs& A, const GateRec& G, u
       const uint64_t p = top_tile - lane;
       unsigned long long wi = 0, wa = 0;
       if (in_gate) {
-        wi = __hip_atomic_load(&ts[2 * p + 1], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+        wi = __hip_atomic_load(&ts[2 * p + 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         if (!(wi >> 63))
-          wa = __hip_atomic_load(&ts[2 * p], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+          wa = __hip_atomic_load(&ts[2 * p], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       }
       // status: 2 = inclusive prefix (or before the gate), 1 = aggregate, 0 = not ready
       const uint32_t status = !in_gate ? 2u : ((wi >> 63) ? 2u : ((wa >> 63) ? 1u : 0u));
@@ -409,177 +511,175 @@ __device__ __forceinline__ void run_tile(const LevelArgs& A, const GateRec& G, u
     }
     excl = acc;
     if (lane == 0)
-      __hip_atomic_store(&ts[2 * tk + 1], pack_inc(agg_combine(excl, tile_agg)), __ATOMIC_RELEASE,
+      __hip_atomic_store(&ts[2 * tk + 1], pack_inc(agg_combine(excl, tile_agg)), __ATOMIC_RELAXED,
                          __HIP_MEMORY_SCOPE_AGENT);
   }
-  const Agg incl = agg_combine(excl, tile_agg);
+  TileSum r;
+  r.incl = agg_combine(excl, tile_agg);
+  r.carry = agg_combine(excl, lane_excl);
+  r.tile_out0 = excl.cnt;
+  r.e_first = e_first;
+  return r;
+}
 
-  // emit: dedup against the carry, stage in LDS, coalesced store
-  const Agg carry = agg_combine(excl, lane_excl);
-  const uint64_t gbase = A.out_base[g];
-  const uint32_t tile_out0 = excl.cnt;
+// Emit: dedup against the carry; each lane stores its run of the tile's
+// output straight into the gate's output region.  SAIF durations are added
+// to sacc (lane 0), which the caller flushes per gate.
+__device__ __forceinline__ void emit_tile(const LevelArgs& A, const TileDesc& D, const Lane& ln,
+                                          const TileSum& T, unsigned long long (&sacc)[3],
+                                          unsigned long long& wave_ev) {
+  const int lane = threadIdx.x;
+  const uint32_t ti = D.ti;
+  const uint32_t sp = ln.sp;
+  const uint32_t* st = ln.st;
+  const Agg& carry = T.carry;
+  const Agg& incl = T.incl;
+  const uint32_t e_first = T.e_first;
+  const uint64_t gbase = A.out_base[D.g];
+  const uint32_t tile_out0 = T.tile_out0;
+  const uint32_t s32 = (uint32_t)min(A.start, 0x7FFFFFFFull);
+  const uint32_t e32 = (uint32_t)min(A.end, 0x7FFFFFFFull);
   uint32_t pvv = carry.has ? carry.lv : 4u;
-  uint32_t pt = carry.lt;
-  unsigned long long acc[3] = {0, 0, 0};
-  __syncthreads();
+  uint32_t pct = max(min(carry.has ? carry.lt : (e_first >> 2), e32), s32);
+  const uint32_t ct0 = pct;
+  uint32_t a0 = 0, a1 = 0;
   {
     uint32_t w = carry.has ? carry.cnt - tile_out0 : 0u;  // lane's first slot in the tile output
     for (uint32_t i = 0; i < sp; ++i) {
       const uint32_t e = st[i];
-      const uint32_t v = e & 3u, t = e >> 2;
-      if (pvv != 4u) saif_acc(acc, pvv, pt, t, A.start, A.end);
+      const uint32_t v = e & 3u;
+      const uint32_t ct = max(min(e >> 2, e32), s32);
+      const uint32_t len = ct - pct;
+      a0 += pvv == 0u ? len : 0u;
+      a1 += pvv == 3u ? len : 0u;
       if (v != pvv) {
-        if (LDS) s_in[w] = e;
-        else A.pool[gbase + tile_out0 + w] = e;
+        A.pool[gbase + tile_out0 + w] = e;
         ++w;
       }
       pvv = v;
-      pt = t;
+      pct = ct;
     }
   }
-  __syncthreads();
-  const uint32_t tile_cnt = incl.cnt - tile_out0;
-  if (LDS) {
-    uint32_t* dst = A.pool + gbase + tile_out0;
-    for (uint32_t q = lane; q < tile_cnt; q += kLanes) dst[q] = s_in[q];
-  }
-  // SAIF and event count: one atomic per counter per tile
+  // SAIF durations and event count of the tile, reduced over the wavefront
+  unsigned long long acc[3] = {a0, a1, (unsigned long long)(pct - ct0 - a0 - a1)};
 #pragma unroll
   for (int k = 0; k < 3; ++k)
     for (int o = 32; o > 0; o >>= 1) acc[k] += __shfl_down(acc[k], o, 64);
-  unsigned long long ev = nown;
+  unsigned long long ev = ln.nown;
   for (int o = 32; o > 0; o >>= 1) ev += __shfl_down(ev, o, 64);
   if (lane == 0) {
-    unsigned long long* sa = A.saif + 3ull * G.out_net;
-    const uint32_t ntile_g = (X.nseg + kLanes - 1) / kLanes;
+    const uint32_t ntile_g = (D.nseg + kLanes - 1) / kLanes;
     if (ti + 1 == ntile_g) {  // last tile: waveform length and the tail duration
-      A.wlen[G.out_net] = incl.cnt;
+      A.wlen[D.out_net] = incl.cnt;
       if (incl.has)
         saif_acc(acc, incl.lv, incl.lt, A.end > incl.lt ? A.end : incl.lt, A.start, A.end);
     }
-    if (ti == 0) A.woff[G.out_net] = gbase;
-    for (int k = 0; k < 3; ++k)
-      if (acc[k]) atomicAdd(sa + k, acc[k]);
+    if (ti == 0) A.woff[D.out_net] = gbase;
+    for (int k = 0; k < 3; ++k) sacc[k] += acc[k];
     wave_ev += ev;
   }
 }
 
-template <int K>
-__device__ __forceinline__ void tile_k(const LevelArgs& A, const GateRec& G, const TileDesc& D,
-                                       uint64_t tk, uint32_t* s_in, uint32_t* s_st,
-                                       const uint8_t* s_lut, const uint16_t* s_tmin,
-                                       const uint16_t* s_tone, unsigned long long& wave_ev) {
-  const int lane = threadIdx.x;
-  const uint32_t g = D.g, ti = D.ti;
-  const GatePlan P = A.plan[g];
-  TileCtx<K> X;
-  X.nseg = P.nseg;
-  X.stride = P.stride;
-  X.lead = P.lead;
-  X.dmax = G.dmax;
-  const uint32_t* lp = A.lutp + (uint32_t)G.lut * 16u;
-#pragma unroll
-  for (int q = 0; q < 4; ++q) X.lut[q] = lp[q];
-  const uint32_t* gin[K];
-  uint32_t off[K];
-  uint32_t total = 0;
-#pragma unroll
-  for (int j = 0; j < K; ++j) {
-    X.tmin2[j] = *(const uint64_t*)&G.tmin[j][0][0];
-    X.tone2[j] = *(const uint64_t*)&G.tone[j][0][0];
-    X.L[j] = D.L[j];
-    X.E[j] = D.E[j];
-    X.B[j] = D.L[j] > 0 ? D.L[j] - 1 : 0u;
-    gin[j] = A.pool + A.woff[G.in_net[j]] + X.B[j];
-    off[j] = total;
-    total += X.E[j] - X.B[j];
+// Add a wavefront's SAIF durations of one net to the counters (lane 0).
+__device__ __forceinline__ void flush_saif(const LevelArgs& A, uint32_t net,
+                                           unsigned long long (&sacc)[3]) {
+  if (threadIdx.x == 0) {
+    unsigned long long* sa = A.saif + 3ull * net;
+    for (int k = 0; k < 3; ++k)
+      if (sacc[k]) atomicAdd(sa + k, sacc[k]);
   }
-  if (total <= A.cap) {
-    // batched copy of the concatenated ranges: 8 loads in flight per lane
-    for (uint32_t x0 = 0; x0 < total; x0 += 8 * kLanes) {
-      uint32_t v[8];
-#pragma unroll
-      for (int u = 0; u < 8; ++u) {
-        const uint32_t x = x0 + u * kLanes + lane;
-        v[u] = 0;
-        if (x < total) {
-          const uint32_t* s = gin[0] + x;
-#pragma unroll
-          for (int j = 1; j < K; ++j)
-            if (x >= off[j]) s = gin[j] + (x - off[j]);
-          v[u] = *s;
-        }
-      }
-#pragma unroll
-      for (int u = 0; u < 8; ++u) {
-        const uint32_t x = x0 + u * kLanes + lane;
-        if (x < total) s_in[x] = v[u];
-      }
-    }
-#pragma unroll
-    for (int j = 0; j < K; ++j) X.src[j] = s_in + off[j];
-    __syncthreads();
-    run_tile<K, true>(A, G, g, ti, tk, X, s_in, s_lut, s_tmin, s_tone, s_st, wave_ev);
-  } else {
-    // does not fit LDS: read inputs from HBM, stacks in the overflow buffer
-    unsigned long long base = 0;
-    const unsigned long long need = (unsigned long long)total + kLanes;
-    if (lane == 0) base = atomicAdd(A.ovf_used, need);
-    base = __shfl(base, 0, 64);
-#pragma unroll
-    for (int j = 0; j < K; ++j) X.src[j] = gin[j];
-    if (base + need > A.ovf_cap) {
-      if (lane == 0) atomicOr(A.err, ERR_OVERFLOW);
-      base = 0;  // results are invalid; keep every wavefront moving
-      if (need > A.ovf_cap) {
-        // cannot even alias the buffer: publish an empty tile and stop
-        if (lane == 0) {
-          A.tstate[2 * tk] = 1ull << 63;
-          __hip_atomic_store(&A.tstate[2 * tk + 1], 1ull << 63, __ATOMIC_RELEASE,
-                             __HIP_MEMORY_SCOPE_AGENT);
-        }
-        return;
-      }
-    }
-    run_tile<K, false>(A, G, g, ti, tk, X, s_in, s_lut, s_tmin, s_tone, A.ovf + base, wave_ev);
-  }
+  sacc[0] = sacc[1] = sacc[2] = 0;
 }
 
+// First tile of the next block of A.blk consecutive tiles.  Blocks are
+// handed out in increasing order, so a tile only ever waits for tiles that
+// running wavefronts hold; one device-scope atomic per block instead of per
+// tile keeps the counter from serialising the grid.
+__device__ __forceinline__ uint64_t take_block(const LevelArgs& A) {
+  unsigned int t = 0;
+  if (threadIdx.x == 0) t = atomicAdd(A.ticket, 1u);
+  return (uint64_t)__builtin_amdgcn_readfirstlane(__shfl(t, 0, 64)) * A.blk;
+}
+
+// Persistent, software-pipelined level kernel: while tile n is combined and
+// stored, the inputs of tile n+1 are already on their way into LDS.  Every
+// wavefront exits when the ticket passes the last tile.
 __global__ __launch_bounds__(kLanes) void k_level(LevelArgs A) {
-  // dynamic LDS: inputs (cap + 64 words), stacks (cap + 64), truth table, delay tables
+  // dynamic LDS: inputs (cap + 128 words), stacks (cap + 64), truth table
+  // (64 words), delay table (512 words)
   extern __shared__ uint32_t s_dyn[];
   uint32_t* s_in = s_dyn;
-  uint32_t* s_st = s_dyn + A.cap + kLanes;
-  uint8_t* s_lut = (uint8_t*)(s_st + A.cap + kLanes);
-  uint16_t* s_tab = (uint16_t*)(s_lut + 256);
+  uint32_t* s_st = s_dyn + A.cap + 2 * kLanes;
+  uint32_t* s_lut = s_st + A.cap + kLanes;
+  uint32_t* s_dt = s_lut + 64;
+  const uint8_t* lut8 = (const uint8_t*)s_lut;
+  const uint16_t* dt16 = (const uint16_t*)s_dt;
   const int lane = threadIdx.x;
   unsigned long long wave_ev = 0;  // gate events of this wavefront (lane 0)
-  for (;;) {
-    unsigned int t = 0;
-    if (lane == 0) t = atomicAdd(A.ticket, 1u);
-    const uint64_t tk = (uint64_t)__builtin_amdgcn_readfirstlane(__shfl(t, 0, 64));
-    if (tk >= A.ntiles) break;
+  unsigned long long sacc[3] = {0, 0, 0};
+  uint64_t tk = take_block(A), blk_end = tk + A.blk;
+  bool fits = false;
+  if (tk < A.ntiles) {
+    fits = tile_words(A.tiles[tk]) <= A.cap;
+    stage_tile(A, A.tiles[tk], fits, s_in, s_lut, s_dt);
+  }
+  bool have_prev = false;
+  uint64_t prev_tk = 0;
+  Agg prev_incl = Agg{0, 0, 0, 0, 0};
+  while (tk < A.ntiles) {
     const TileDesc& D = A.tiles[tk];
-    const uint32_t g = __builtin_amdgcn_readfirstlane(D.g);
-    const GateRec& G = A.gates[g];
-    const uint32_t k = G.k;
-    // full truth table (fan-in 4) and delay tables (X/Z edges) into LDS
-    if (k == 4) {
-      const uint32_t* lsrc = (const uint32_t*)(A.luts + (uint32_t)G.lut * 256u);
-      ((uint32_t*)s_lut)[lane] = lsrc[lane];
-    }
-    if (lane < 32) {
-      const uint32_t* tb = (const uint32_t*)&G.tmin[0][0][0];
-      ((uint32_t*)s_tab)[lane] = tb[lane];
+    wait_vm();
+    if (fits) {  // INF sentinel after every input range
+      uint32_t pos = 0;
+      for (uint32_t j = 0; j < D.k; ++j) {
+        pos += D.E[j] - range_start(D.L[j]);
+        if ((uint32_t)lane == j) s_in[pos] = 0xFFFFFFFFu;
+        ++pos;
+      }
     }
     __syncthreads();
-    switch (k) {
-      case 1: tile_k<1>(A, G, D, tk, s_in, s_st, s_lut, s_tab, s_tab + 32, wave_ev); break;
-      case 2: tile_k<2>(A, G, D, tk, s_in, s_st, s_lut, s_tab, s_tab + 32, wave_ev); break;
-      case 3: tile_k<3>(A, G, D, tk, s_in, s_st, s_lut, s_tab, s_tab + 32, wave_ev); break;
-      default: tile_k<4>(A, G, D, tk, s_in, s_st, s_lut, s_tab, s_tab + 32, wave_ev); break;
+    Lane ln;
+    switch (D.k) {
+      case 1: ln = walk_k<1>(A, D, tk, fits, s_in, s_st, lut8, dt16); break;
+      case 2: ln = walk_k<2>(A, D, tk, fits, s_in, s_st, lut8, dt16); break;
+      case 3: ln = walk_k<3>(A, D, tk, fits, s_in, s_st, lut8, dt16); break;
+      default: ln = walk_k<4>(A, D, tk, fits, s_in, s_st, lut8, dt16); break;
     }
+    __syncthreads();  // inputs, truth and delay tables no longer read
+    const bool chain = have_prev && prev_tk + 1 == tk && D.ti > 0;
+    // the look-back may wait for other wavefronts: it runs before this
+    // wavefront takes its next tile, so no tile is ever held by a waiting one
+    const TileSum T = combine_tile(A, D, tk, ln, chain, prev_incl);
+    uint64_t tn = tk + 1;
+    bool fits_n = false;
+    if (A.prefetch) {  // stage the next tile while this one is stored
+      if (tn >= blk_end) {
+        tn = take_block(A);
+        blk_end = tn + A.blk;
+      }
+      if (tn < A.ntiles) {
+        fits_n = tile_words(A.tiles[tn]) <= A.cap;
+        stage_tile(A, A.tiles[tn], fits_n, s_in, s_lut, s_dt);
+      }
+    }
+    emit_tile(A, D, ln, T, sacc, wave_ev);
+    prev_incl = T.incl;
+    if (!A.prefetch) {
+      if (tn >= blk_end) {
+        tn = take_block(A);
+        blk_end = tn + A.blk;
+      }
+      if (tn < A.ntiles) {
+        fits_n = tile_words(A.tiles[tn]) <= A.cap;
+        stage_tile(A, A.tiles[tn], fits_n, s_in, s_lut, s_dt);
+      }
+    }
+    if (tn >= A.ntiles || A.tiles[tn].out_net != D.out_net) flush_saif(A, D.out_net, sacc);
+    have_prev = true;
+    prev_tk = tk;
     __syncthreads();
+    tk = tn;
+    fits = fits_n;
   }
   if (lane == 0 && wave_ev) atomicAdd(A.ev_count, wave_ev);
 }
@@ -618,12 +718,13 @@ __global__ void k_plan2(const GateRec* __restrict__ gates, uint32_t ng,
   bound[g] = N + 1;
 }
 
-// per (tile, input): gate, tile index within the gate, input range
+// per (tile, input): the tile's descriptor (gate, tile index within the
+// gate, segmentation, tables) and input j's range
 __global__ void k_tile_desc(const GateRec* __restrict__ gates, uint32_t ng,
                             const GatePlan* __restrict__ plan,
                             const uint64_t* __restrict__ tile_base,
-                            const uint64_t* __restrict__ ntile, const uint64_t* __restrict__ woff,
-                            const uint32_t* __restrict__ wlen, const uint32_t* __restrict__ pool,
+                            const uint64_t* __restrict__ woff, const uint32_t* __restrict__ wlen,
+                            const uint32_t* __restrict__ pool, const uint32_t* __restrict__ lutp,
                             TileDesc* __restrict__ tiles, uint64_t ntiles) {
   const uint64_t x = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const uint64_t t = x >> 2;
@@ -641,14 +742,24 @@ __global__ void k_tile_desc(const GateRec* __restrict__ gates, uint32_t ng,
   const GatePlan P = plan[g];
   const uint32_t ti = (uint32_t)(t - tile_base[g]);
   TileDesc& D = tiles[t];
+  D.lutw[j] = lutp[(uint32_t)G.lut * 16u + j];
   if (j == 0) {
     D.g = g;
     D.ti = ti;
+    D.nseg = P.nseg;
+    D.stride = P.stride;
+    D.lead = P.lead;
+    D.k = G.k;
+    D.dt_off = G.dt_off;
+    D.lut = G.lut;
+    D.dmax = G.dmax;
+    D.out_net = G.out_net;
     D.pad[0] = D.pad[1] = 0;
   }
   if (j >= G.k) {
     D.L[j] = 0;
     D.E[j] = 0;
+    D.in_off[j] = 0;
     return;
   }
   const uint32_t* lw = pool + woff[G.in_net[P.lead]];
@@ -667,7 +778,7 @@ __global__ void k_tile_desc(const GateRec* __restrict__ gates, uint32_t ng,
   }
   D.L[j] = L;
   D.E[j] = E;
-  (void)ntile;
+  D.in_off[j] = woff[G.in_net[j]] + (L > 0 ? L - 1 : 0u);
 }
 
 __global__ void k_fill_u64(unsigned long long* p, uint64_t n, unsigned long long v) {
@@ -683,13 +794,13 @@ hipError_t launch_plan2(const GateRec* gates, uint32_t ng, const uint32_t* wlen,
 }
 
 hipError_t launch_tile_desc(const GateRec* gates, uint32_t ng, const GatePlan* plan,
-                            const uint64_t* tile_base, const uint64_t* ntile, const uint64_t* woff,
-                            const uint32_t* wlen, const uint32_t* pool, TileDesc* tiles,
+                            const uint64_t* tile_base, const uint64_t* woff, const uint32_t* wlen,
+                            const uint32_t* pool, const uint32_t* lutp, TileDesc* tiles,
                             uint64_t ntiles, hipStream_t st) {
   if (ntiles == 0) return hipSuccess;
   const uint64_t threads = ntiles * 4;
-  k_tile_desc<<<(unsigned)((threads + 255) / 256), 256, 0, st>>>(gates, ng, plan, tile_base, ntile,
-                                                                  woff, wlen, pool, tiles, ntiles);
+  k_tile_desc<<<(unsigned)((threads + 255) / 256), 256, 0, st>>>(gates, ng, plan, tile_base, woff,
+                                                                  wlen, pool, lutp, tiles, ntiles);
   return hipGetLastError();
 }
 

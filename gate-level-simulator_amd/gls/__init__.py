@@ -34,7 +34,8 @@ class DesignInfo(ctypes.Structure):
 class SimOpts(ctypes.Structure):
     _fields_ = [("pool_bytes", ctypes.c_uint64), ("seg_events", ctypes.c_uint32),
                 ("keep_waves", ctypes.c_uint32), ("path", ctypes.c_uint32),
-                ("level_blocks", ctypes.c_uint32)]
+                ("level_blocks", ctypes.c_uint32), ("tiles_per_ticket", ctypes.c_uint32),
+                ("no_prefetch", ctypes.c_uint32)]
 
 
 PATH_FUSED = 0
@@ -259,21 +260,25 @@ class Context(_Handle):
         super().__init__(h.value)
 
     @staticmethod
-    def _opts(pool_bytes, seg_events, keep_waves, path, level_blocks=0):
-        return SimOpts(pool_bytes, seg_events, 1 if keep_waves else 0, path, level_blocks)
+    def _opts(pool_bytes, seg_events, keep_waves, path, level_blocks=0, tiles_per_ticket=0,
+              prefetch=True):
+        return SimOpts(pool_bytes, seg_events, 1 if keep_waves else 0, path, level_blocks,
+                       tiles_per_ticket, 0 if prefetch else 1)
 
     def prepare(self, design, stim, dumpon, dumpoff, pool_bytes=0, seg_events=0, keep_waves=False,
-                path=PATH_FUSED, level_blocks=0):
+                path=PATH_FUSED, level_blocks=0, tiles_per_ticket=0, prefetch=True):
         h = ctypes.c_void_p()
-        o = self._opts(pool_bytes, seg_events, keep_waves, path, level_blocks)
+        o = self._opts(pool_bytes, seg_events, keep_waves, path, level_blocks, tiles_per_ticket,
+                       prefetch)
         _check(load().gls_prepare(self.ptr, design.ptr, stim.ptr, dumpon, dumpoff, ctypes.byref(o),
                                   ctypes.byref(h)))
         return Run(h.value)
 
     def simulate(self, design, stim, dumpon, dumpoff, pool_bytes=0, seg_events=0, keep_waves=False,
-                 path=PATH_FUSED, level_blocks=0):
+                 path=PATH_FUSED, level_blocks=0, tiles_per_ticket=0, prefetch=True):
         h = ctypes.c_void_p()
-        o = self._opts(pool_bytes, seg_events, keep_waves, path, level_blocks)
+        o = self._opts(pool_bytes, seg_events, keep_waves, path, level_blocks, tiles_per_ticket,
+                       prefetch)
         _check(load().gls_simulate(self.ptr, design.ptr, stim.ptr, dumpon, dumpoff, ctypes.byref(o),
                                    ctypes.byref(h)))
         return Run(h.value)

@@ -104,6 +104,21 @@ def test_segment_size_invariance(ctx):
     assert "%016x" % ref[0] == GOLDEN["c1_small"]["waves_hash"]
 
 
+def test_tiles_per_ticket_invariance(ctx):
+    # a workgroup takes 1..64 consecutive tiles at once: the prefix carry moves
+    # from the look-back to registers, and SAIF flushes once per gate run
+    d, s = golden_cases.make("c1_small")
+    start, end = golden_cases.CASES["c1_small"]["window"]
+    g = GOLDEN["c1_small"]
+    for seg, tpt, blocks, pf in ((8, 1, 0, True), (8, 3, 0, True), (8, 64, 0, False),
+                                 (32, 16, 7, True), (2, 5, 1, False), (64, 1, 0, False)):
+        say("tiles_per_ticket", seg, tpt, blocks, pf)
+        r = ctx.simulate(d, s, start, end, seg_events=seg, keep_waves=True, level_blocks=blocks,
+                         tiles_per_ticket=tpt, prefetch=pf)
+        assert "%016x" % r.waves_hash() == g["waves_hash"], (seg, tpt, blocks, pf)
+        assert oracle_bind.sha256(r.saif_text()) == g["saif_sha256"], (seg, tpt, blocks, pf)
+
+
 def test_cli_dropin(tmp_path):
     say("cli")
     paths = golden_cases.write_inputs("c0_small", str(tmp_path))
