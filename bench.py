@@ -74,6 +74,137 @@ def measured_traffic(args, launches):
     return None
 
 
+def pool_bytes_for(d, s):
+    """Event-pool size for one instance: stimulus plus each gate's output
+    reservation (its inputs' events; internal nets toggle a few times more
+    often than the inputs), with headroom."""
+    gates = d.info()["n_gates"]
+    sig, ev = s.info()["n_signals"], s.info()["n_events"]
+    return 4 * (ev + 12 * gates * (ev // max(1, sig) + 1) + (1 << 20))
+
+
+def batched_vs_plain(ctx, n_sets, scale):
+    """Batched form against plain runs on a scaled-down batching workload:
+    one netlist from the 200k-cell generator at `scale` of its size driven by
+    `n_sets` seeded stimulus sets (0-2 Mps windows, mean gap U[100, 5000] ps),
+    plus n_sets/4 netlists with log-uniform cell counts and depths U{10..60}.
+    Plain: every instance prepared and executed alone, one after the other.
+    Batched: all instances in one prepared run (level l of every instance in
+    one launch), autotuned, then executed.  Every instance's SAIF is compared
+    with its plain run."""
+    import random
+    import gls
+    rng = random.Random(0x5EED0003)
+    cells = max(1000, int(200000 * scale))
+    main_d = gls.Design.generate(1, 0x5EED0003, cells, 40, max(16, int(1024 * scale)))
+    insts = []
+    for i in range(n_sets):
+        s = gls.Stimulus.generate(main_d, 0x5EED0003 + 1 + i, 2000000, rng.uniform(100, 5000), 0.01)
+        insts.append((main_d, s))
+    for i in range(max(1, n_sets // 4)):
+        c = int(round(10 ** rng.uniform(3.7, 5.7) * scale))
+        c = max(500, c)
+        d = gls.Design.generate(1, 0x5EED1003 + i, c, rng.randint(10, 60), max(16, int(1024 * c / 200000)))
+        s = gls.Stimulus.generate(d, 0x5EED2003 + i, 2000000, rng.uniform(100, 5000), 0.01)
+        insts.append((d, s))
+    # plain: one instance at a time (the first run of each pays nothing extra:
+    # its netlist image is uploaded in prepare, outside the timed execute)
+    log(f"bench: batch workload generated: {len(insts)} instances")
+    plain_s, plain_ev, saifs = 0.0, 0, []
+    for k, (d, s) in enumerate(insts):
+        r = ctx.prepare(d, s, 0, 2000001, pool_bytes=pool_bytes_for(d, s))
+        r.execute()
+        t0 = time.perf_counter()
+        r.execute()
+        plain_s += time.perf_counter() - t0
+        plain_ev += r.stats()["n_events"]
+        saifs.append(r.saif_text())
+        r.close()
+        if k % 8 == 7:
+            log(f"bench: plain instance {k + 1}/{len(insts)} done")
+    b = ctx.prepare_batch([i[0] for i in insts], [i[1] for i in insts], [0] * len(insts),
+                          [2000001] * len(insts),
+                          pool_bytes=sum(pool_bytes_for(d, s) for d, s in insts))
+    log("bench: batch prepared, autotuning")
+    tune = b.autotune()
+    log(f"bench: autotune picked {tune['seg_events']} events/segment, "
+        f"{tune['level_blocks']} workgroups, prefetch {tune['prefetch']}")
+    t0 = time.perf_counter()
+    b.execute()
+    batch_s = time.perf_counter() - t0
+    st = b.stats()
+    same = all(b.instance_saif_text(k) == saifs[k] for k in range(len(insts)))
+    b.close()
+    return {"workload": f"{n_sets} stimulus sets on a {cells}-cell netlist + {len(insts) - n_sets} "
+                        f"netlists (log-uniform cells, depth 10-60), 0-2 Mps windows, gap U[100,5000] ps",
+            "instances": len(insts), "gate_events": st["n_events"],
+            "plain_events_per_s": round(plain_ev / plain_s, 1), "plain_s": round(plain_s, 4),
+            "batched_events_per_s": round(st["n_events"] / batch_s, 1), "batched_s": round(batch_s, 4),
+            "speedup": round(plain_s / batch_s, 3), "saif_bit_exact_vs_plain": same,
+            "autotune": {k: tune[k] for k in ("seg_events", "level_blocks", "prefetch", "best_ms")}}
+
+
+def serving_mix(ctx, n_jobs, scale):
+    """Serving loop on a seeded Poisson job stream: netlists log-uniform in
+    [10k, 2M] x `scale` cells, windows U[1, 20] Mps x `scale`, 30% of jobs
+    reuse a netlist already submitted; the arrival rate is set from the mean
+    time of a few jobs run alone, for about 75% device utilisation.  Reports
+    events/s and p50/p99 job latency; every job's SAIF is checked against the
+    same job run alone."""
+    import random
+    import gls
+    rng = random.Random(0x5EED0004)
+    designs, jobs = [], []
+    for i in range(n_jobs):
+        if designs and rng.random() < 0.3:
+            d = rng.choice(designs)
+        else:
+            c = max(500, int(10 ** rng.uniform(4, 6.3) * scale))
+            d = gls.Design.generate(1, 0x5EED4000 + i, c, rng.randint(10, 60), max(16, int(1024 * c / 200000)))
+            designs.append(d)
+        t_end = int(rng.uniform(1e6, 2e7) * scale)
+        s = gls.Stimulus.generate(d, 0x5EED5000 + i, t_end, 500.0, 0.01)
+        jobs.append((d, s, 0, t_end + 1))
+    log(f"bench: serving workload generated: {n_jobs} jobs, {len(designs)} netlists")
+    alone, t_alone = [], []
+    for k, (d, s, on, off) in enumerate(jobs):
+        # the second run of each job is timed: its netlist is resident, as
+        # for a served job
+        for rep in range(2):
+            t0 = time.perf_counter()
+            r = ctx.simulate(d, s, on, off, pool_bytes=pool_bytes_for(d, s))
+            dt = time.perf_counter() - t0
+            if rep == 0:
+                alone.append(r.saif_text())
+            r.close()
+        t_alone.append(dt)
+        if k % 10 == 9:
+            log(f"bench: alone job {k + 1}/{n_jobs} done")
+    mean_s = sum(t_alone) / len(t_alone)
+    rate = 0.75 / mean_s  # jobs per second
+    gaps = [rng.expovariate(rate) for _ in jobs]
+    srv = gls.Server(ctx, max_running=4)
+    ids = []
+    for (d, s, on, off), g in zip(jobs, gaps):
+        time.sleep(g)
+        ids.append(srv.submit(d, s, on, off))
+    log(f"bench: {n_jobs} jobs submitted")
+    same = True
+    for k, j in enumerate(ids):
+        if srv.wait(j) != 0 or srv.saif_text(j) != alone[k]:
+            same = False
+    st = srv.stats()
+    srv.close()
+    return {"workload": f"{n_jobs} Poisson jobs at {rate:.2f}/s, cells log-uniform in "
+                        f"[{int(1e4 * scale)}, {int(2e6 * scale)}], windows U[{int(1e6 * scale)}, "
+                        f"{int(2e7 * scale)}] ps, gap 500 ps, 30% netlist reuse",
+            "jobs": int(st["done"]), "failed": int(st["failed"]),
+            "events_per_s": round(st["events_per_s"], 1), "p50_ms": round(st["p50_ms"], 3),
+            "p99_ms": round(st["p99_ms"], 3), "mean_alone_ms": round(mean_s * 1e3, 3),
+            "peak_running": int(st["peak_running"]), "resident_programs": int(st["resident_programs"]),
+            "saif_same_alone_and_in_mix": same}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -88,6 +219,12 @@ def main():
     ap.add_argument("--no-prefetch", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-events", type=int, default=700000)
+    ap.add_argument("--batch-sets", type=int, default=32,
+                    help="stimulus sets of the batched-vs-plain measurement (0: skip)")
+    ap.add_argument("--batch-scale", type=float, default=0.1)
+    ap.add_argument("--serve-jobs", type=int, default=40,
+                    help="jobs of the serving measurement (0: skip)")
+    ap.add_argument("--serve-scale", type=float, default=0.05)
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="only run a plain device copy of this many MiB (3 times): the "
                          "reference point that scales FETCH_SIZE/WRITE_SIZE counters to bytes")
@@ -186,6 +323,16 @@ def main():
         "saif_durations_consistent": ok,
         "cpu_baseline": cpu,
     }
+    run.close()
+    if args.batch_sets:
+        t0 = time.time()
+        out["batched"] = batched_vs_plain(ctx, args.batch_sets, args.batch_scale)
+        log(f"bench: batched vs plain {out['batched']['speedup']}x ({time.time() - t0:.1f}s)")
+    if args.serve_jobs:
+        t0 = time.time()
+        out["serving"] = serving_mix(ctx, args.serve_jobs, args.serve_scale)
+        log(f"bench: serving p50 {out['serving']['p50_ms']} ms p99 {out['serving']['p99_ms']} ms "
+            f"({time.time() - t0:.1f}s)")
     print(json.dumps(out), flush=True)
 
 

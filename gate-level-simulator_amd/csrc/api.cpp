@@ -4,8 +4,11 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cstring>
+#include <map>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <unordered_map>
 
@@ -13,6 +16,7 @@
 #include "engine.h"
 #include "formats.h"
 #include "gen.h"
+#include "server.h"
 
 using namespace gls;
 
@@ -22,6 +26,10 @@ struct gls_design {
   Library lib;
   Netlist nl;
   std::string vlib_text, gv_text, sdf_text;  // empty when generated
+  // compiled programs by stimulus signal list (the VCD decides which nets
+  // are driven from outside)
+  std::mutex mu;
+  std::map<std::vector<std::string>, std::shared_ptr<const Program>> progs;
 };
 struct gls_stim {
   Stimulus st;
@@ -29,23 +37,72 @@ struct gls_stim {
 struct gls_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
+  ProgramCache cache;  // resident program images of this device
 };
 struct gls_run {
   std::unique_ptr<Job> job;
-  Program pg;
-  std::vector<int> stim_nets;
-  std::vector<std::string> names;  // all net names incl. stimulus-only signals
-  std::vector<uint8_t> in_saif;
-  std::vector<uint8_t> is_gate;
-  uint64_t start = 0, end = 0;
-  int timescale_ps = 1;
+  std::vector<RunInst> inst;
   bool keep_waves = false;
   bool executed = false;
-  std::vector<uint64_t> saif;  // 3 per net
+  std::vector<uint64_t> saif;  // 3 per net of the union net space
   std::vector<uint64_t> woff;
   std::vector<uint32_t> wlen, pool;
-  std::unordered_map<std::string, int> by_name;
 };
+
+namespace gls {
+std::shared_ptr<const Program> design_program(gls_design* d, const std::vector<std::string>& names) {
+  std::lock_guard<std::mutex> lk(d->mu);
+  auto it = d->progs.find(names);
+  if (it != d->progs.end()) return it->second;
+  auto pg = std::make_shared<const Program>(compile_program(d->nl, d->lib, names));
+  d->progs.emplace(names, pg);
+  return pg;
+}
+
+RunInst make_inst(gls_design* d, const gls_stim* s, uint64_t dumpon, uint64_t dumpoff) {
+  RunInst r;
+  r.pg = design_program(d, s->st.names);
+  r.names = r.pg->net_names;
+  r.in_saif = r.pg->in_saif;
+  r.is_gate.assign(r.names.size(), 0);
+  for (auto& g : r.pg->gates) r.is_gate[g.out_net] = 1;
+  for (size_t i = 0; i < r.names.size(); ++i) r.by_name.emplace(r.names[i], (int)i);
+  for (auto& nm : s->st.names) {
+    int id = d->nl.net(nm);
+    if (id < 0) {  // a VCD signal the netlist does not use: SAIF record only
+      auto it = r.by_name.find(nm);
+      if (it != r.by_name.end()) id = it->second;
+      else {
+        id = (int)r.names.size();
+        r.names.push_back(nm);
+        r.in_saif.push_back(1);
+        r.is_gate.push_back(0);
+        r.by_name.emplace(nm, id);
+      }
+    }
+    r.stim_nets.push_back(id);
+  }
+  r.st = &s->st;
+  r.start = dumpon;
+  r.end = dumpoff;
+  r.timescale_ps = s->st.timescale_ps;
+  return r;
+}
+
+std::string inst_saif(const RunInst& in, const std::vector<uint64_t>& saif, uint32_t base) {
+  std::vector<SaifRecord> recs;
+  for (size_t n = 0; n < in.names.size(); ++n) {
+    if (!in.in_saif[n]) continue;
+    SaifRecord s;
+    s.name = in.names[n];
+    s.t0 = saif[3 * (base + n) + 0];
+    s.t1 = saif[3 * (base + n) + 1];
+    s.tx = saif[3 * (base + n) + 2];
+    recs.push_back(std::move(s));
+  }
+  return saif_text(std::move(recs), in.start, in.end, in.timescale_ps);
+}
+}  // namespace gls
 
 #define API_BEGIN try {
 #define API_END                                  \
@@ -262,48 +319,50 @@ void gls_ctx_destroy(gls_ctx* c) {
   delete c;
 }
 
-int gls_prepare(gls_ctx* c, const gls_design* d, const gls_stim* s, uint64_t dumpon,
-                uint64_t dumpoff, const gls_sim_opts* opts, gls_run** out) {
+static void apply_opts(Job& job, const gls_sim_opts* opts) {
+  if (!opts) return;
+  if (opts->path > GLS_PATH_TWO_PASS) throw GlsError(GLS_E_ARG, "unknown path");
+  if (opts->seg_events) job.seg_target = opts->seg_events;
+  job.mode = (int)opts->path;
+  job.level_blocks_opt = opts->level_blocks;
+  job.tiles_per_ticket = opts->tiles_per_ticket;
+  job.prefetch = opts->no_prefetch ? 0u : 1u;
+}
+
+int gls_prepare_batch(gls_ctx* c, size_t n, const gls_design* const* designs,
+                      const gls_stim* const* stims, const uint64_t* dumpon, const uint64_t* dumpoff,
+                      const gls_sim_opts* opts, gls_run** out) {
   API_BEGIN
-  if (!c || !d || !s || !out) throw GlsError(GLS_E_ARG, "null argument");
+  if (!c || !designs || !stims || !dumpon || !dumpoff || !out || n == 0)
+    throw GlsError(GLS_E_ARG, "null argument");
   auto r = std::make_unique<gls_run>();
-  r->pg = compile_program(d->nl, d->lib, s->st.names);
-  r->names = r->pg.net_names;
-  r->in_saif = r->pg.in_saif;
-  r->is_gate.assign(r->names.size(), 0);
-  for (auto& g : r->pg.gates) r->is_gate[g.out_net] = 1;
-  for (size_t i = 0; i < r->names.size(); ++i) r->by_name.emplace(r->names[i], (int)i);
-  for (auto& nm : s->st.names) {
-    int id = d->nl.net(nm);
-    if (id < 0) {  // a VCD signal the netlist does not use: SAIF record only
-      auto it = r->by_name.find(nm);
-      if (it != r->by_name.end()) id = it->second;
-      else {
-        id = (int)r->names.size();
-        r->names.push_back(nm);
-        r->in_saif.push_back(1);
-        r->is_gate.push_back(0);
-        r->by_name.emplace(nm, id);
-      }
-    }
-    r->stim_nets.push_back(id);
+  std::vector<SimInstance> inst;
+  for (size_t i = 0; i < n; ++i) {
+    if (!designs[i] || !stims[i]) throw GlsError(GLS_E_ARG, "null design or stimulus");
+    r->inst.push_back(make_inst(const_cast<gls_design*>(designs[i]), stims[i], dumpon[i], dumpoff[i]));
   }
-  r->start = dumpon;
-  r->end = dumpoff;
-  r->timescale_ps = s->st.timescale_ps;
+  for (auto& in : r->inst) {
+    SimInstance I;
+    I.pg = in.pg.get();
+    // one instance: the context's resident image of the program
+    if (n == 1) I.image = c->cache.get(in.pg, (uint32_t)in.timescale_ps);
+    I.st = in.st;
+    I.stim_nets = in.stim_nets;
+    I.start = in.start;
+    I.end = in.end;
+    inst.push_back(std::move(I));
+  }
   r->keep_waves = opts && opts->keep_waves;
   r->job = std::make_unique<Job>(c->device, c->stream);
-  if (opts && opts->seg_events) r->job->seg_target = opts->seg_events;
-  if (opts) {
-    if (opts->path > GLS_PATH_TWO_PASS) throw GlsError(GLS_E_ARG, "unknown path");
-    r->job->mode = (int)opts->path;
-    r->job->level_blocks_opt = opts->level_blocks;
-    r->job->tiles_per_ticket = opts->tiles_per_ticket;
-    r->job->prefetch = opts->no_prefetch ? 0u : 1u;
-  }
-  r->job->prepare(r->pg, s->st, r->stim_nets, dumpon, dumpoff, opts ? opts->pool_bytes / 4 : 0);
+  apply_opts(*r->job, opts);
+  r->job->prepare(inst, opts ? opts->pool_bytes / 4 : 0);
   *out = r.release();
   API_END
+}
+
+int gls_prepare(gls_ctx* c, const gls_design* d, const gls_stim* s, uint64_t dumpon,
+                uint64_t dumpoff, const gls_sim_opts* opts, gls_run** out) {
+  return gls_prepare_batch(c, 1, &d, &s, &dumpon, &dumpoff, opts, out);
 }
 
 int gls_run_execute(gls_run* r) {
@@ -349,37 +408,46 @@ int gls_run_stats_get(const gls_run* r, gls_run_stats* st) {
   API_END
 }
 
-static std::string make_saif(const gls_run* r) {
+static const RunInst& inst_of(const gls_run* r, size_t i) {
+  if (i >= r->inst.size()) throw GlsError(GLS_E_ARG, "instance index out of range");
+  return r->inst[i];
+}
+
+static std::string make_saif(const gls_run* r, size_t i) {
   if (!r->executed) throw GlsError(GLS_E_STATE, "run not executed");
-  std::vector<SaifRecord> recs;
-  for (size_t n = 0; n < r->names.size(); ++n) {
-    if (!r->in_saif[n]) continue;
-    SaifRecord s;
-    s.name = r->names[n];
-    s.t0 = r->saif[3 * n + 0];
-    s.t1 = r->saif[3 * n + 1];
-    s.tx = r->saif[3 * n + 2];
-    recs.push_back(std::move(s));
-  }
-  return saif_text(std::move(recs), r->start, r->end, r->timescale_ps);
+  return inst_saif(inst_of(r, i), r->saif, r->job->net_base(i));
+}
+
+int gls_run_instances(const gls_run* r, size_t* n) {
+  API_BEGIN
+  if (!r || !n) throw GlsError(GLS_E_ARG, "null argument");
+  *n = r->inst.size();
+  API_END
 }
 
 int gls_run_write_saif(const gls_run* r, const char* path) {
   API_BEGIN
   if (!r || !path) throw GlsError(GLS_E_ARG, "null argument");
-  write_file(path, make_saif(r));
+  write_file(path, make_saif(r, 0));
   API_END
 }
 
-int gls_run_saif_text(const gls_run* r, char* buf, size_t cap, size_t* len) {
-  API_BEGIN
-  if (!r || !len) throw GlsError(GLS_E_ARG, "null argument");
-  std::string s = make_saif(r);
+static void copy_text(const std::string& s, char* buf, size_t cap, size_t* len) {
   *len = s.size();
   if (buf) {
     if (cap < s.size()) throw GlsError(GLS_E_ARG, "buffer too small");
     std::memcpy(buf, s.data(), s.size());
   }
+}
+
+int gls_run_saif_text(const gls_run* r, char* buf, size_t cap, size_t* len) {
+  return gls_run_instance_saif_text(r, 0, buf, cap, len);
+}
+
+int gls_run_instance_saif_text(const gls_run* r, size_t i, char* buf, size_t cap, size_t* len) {
+  API_BEGIN
+  if (!r || !len) throw GlsError(GLS_E_ARG, "null argument");
+  copy_text(make_saif(r, i), buf, cap, len);
   API_END
 }
 
@@ -387,8 +455,9 @@ int gls_run_net_saif(const gls_run* r, const char* net, uint64_t v[3]) {
   API_BEGIN
   if (!r || !net || !v) throw GlsError(GLS_E_ARG, "null argument");
   if (!r->executed) throw GlsError(GLS_E_STATE, "run not executed");
-  auto it = r->by_name.find(net);
-  if (it == r->by_name.end()) throw GlsError(GLS_E_ARG, std::string("unknown net ") + net);
+  const RunInst& in = inst_of(r, 0);
+  auto it = in.by_name.find(net);
+  if (it == in.by_name.end()) throw GlsError(GLS_E_ARG, std::string("unknown net ") + net);
   for (int i = 0; i < 3; ++i) v[i] = r->saif[3 * it->second + i];
   API_END
 }
@@ -397,8 +466,9 @@ int gls_run_waveform(const gls_run* r, const char* net, uint32_t* words, size_t 
   API_BEGIN
   if (!r || !net || !n) throw GlsError(GLS_E_ARG, "null argument");
   if (!r->executed || !r->keep_waves) throw GlsError(GLS_E_STATE, "waveforms not kept");
-  auto it = r->by_name.find(net);
-  if (it == r->by_name.end()) throw GlsError(GLS_E_ARG, std::string("unknown net ") + net);
+  const RunInst& in = inst_of(r, 0);
+  auto it = in.by_name.find(net);
+  if (it == in.by_name.end()) throw GlsError(GLS_E_ARG, std::string("unknown net ") + net);
   int id = it->second;
   *n = r->wlen[id];
   if (words) {
@@ -410,18 +480,20 @@ int gls_run_waveform(const gls_run* r, const char* net, uint32_t* words, size_t 
 
 // Per-net FNV-1a over the name bytes then the little-endian event words,
 // finalised with SplitMix64 and summed over gate-output nets.
-int gls_run_waves_hash(const gls_run* r, uint64_t* hash) {
+int gls_run_instance_waves_hash(const gls_run* r, size_t i, uint64_t* hash) {
   API_BEGIN
   if (!r || !hash) throw GlsError(GLS_E_ARG, "null argument");
   if (!r->executed || !r->keep_waves) throw GlsError(GLS_E_STATE, "waveforms not kept");
+  const RunInst& in = inst_of(r, i);
+  const uint32_t base = r->job->net_base(i);
   uint64_t total = 0;
-  for (size_t n = 0; n < r->names.size(); ++n) {
-    if (!r->is_gate[n]) continue;
+  for (size_t n = 0; n < in.names.size(); ++n) {
+    if (!in.is_gate[n]) continue;
     uint64_t h = 1469598103934665603ull;
-    for (unsigned char ch : r->names[n]) h = (h ^ ch) * 1099511628211ull;
-    const uint32_t* w = r->pool.data() + r->woff[n];
-    for (uint32_t i = 0; i < r->wlen[n]; ++i)
-      for (int b = 0; b < 4; ++b) h = (h ^ ((w[i] >> (8 * b)) & 0xFF)) * 1099511628211ull;
+    for (unsigned char ch : in.names[n]) h = (h ^ ch) * 1099511628211ull;
+    const uint32_t* w = r->pool.data() + r->woff[base + n];
+    for (uint32_t q = 0; q < r->wlen[base + n]; ++q)
+      for (int b = 0; b < 4; ++b) h = (h ^ ((w[q] >> (8 * b)) & 0xFF)) * 1099511628211ull;
     uint64_t z = h + 0x9E3779B97F4A7C15ull;
     z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
     z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
@@ -431,6 +503,167 @@ int gls_run_waves_hash(const gls_run* r, uint64_t* hash) {
   API_END
 }
 
+int gls_run_waves_hash(const gls_run* r, uint64_t* hash) {
+  return gls_run_instance_waves_hash(r, 0, hash);
+}
+
+// Autotuner: runs the prepared simulation once per candidate setting and
+// keeps the fastest.  Every candidate computes identical results, so the
+// run's results stay valid.
+int gls_run_autotune(gls_run* r, gls_tune_report* rep) {
+  API_BEGIN
+  if (!r) throw GlsError(GLS_E_ARG, "null argument");
+  if (r->job->mode != 0) throw GlsError(GLS_E_UNSUPPORTED, "the autotuner tunes the fused path");
+  Job& j = *r->job;
+  gls_tune_report R;
+  std::memset(&R, 0, sizeof R);
+  auto trial = [&](uint32_t seg, uint32_t blocks, uint32_t pf) {
+    j.seg_target = seg;
+    j.level_blocks_opt = blocks;
+    j.prefetch = pf;
+    j.run();
+    const double ms = j.stats().kernel_ms;
+    if (R.n_tried < GLS_TUNE_MAX) {
+      R.tried_seg[R.n_tried] = seg;
+      R.tried_blocks[R.n_tried] = (uint32_t)j.stats().level_blocks;
+      R.tried_prefetch[R.n_tried] = pf;
+      R.tried_ms[R.n_tried] = ms;
+      ++R.n_tried;
+    }
+    return ms;
+  };
+  // LDS tile size (events per segment), then launch width, then staging
+  uint32_t best_seg = 32;
+  double best = 1e300;
+  for (uint32_t seg : {24u, 32u, 48u, 64u}) {
+    double ms = trial(seg, 0, 1);
+    if (ms < best) best = ms, best_seg = seg;
+  }
+  j.seg_target = best_seg;
+  j.level_blocks_opt = 0;
+  j.run();
+  const uint32_t occ = (uint32_t)j.stats().level_blocks;
+  uint32_t best_blocks = 0;
+  for (uint32_t b : {occ * 3 / 4, occ / 2}) {
+    if (b == 0) continue;
+    double ms = trial(best_seg, b, 1);
+    if (ms < best) best = ms, best_blocks = b;
+  }
+  uint32_t best_pf = 1;
+  {
+    double ms = trial(best_seg, best_blocks, 0);
+    if (ms < best) best = ms, best_pf = 0;
+  }
+  j.seg_target = best_seg;
+  j.level_blocks_opt = best_blocks;
+  j.prefetch = best_pf;
+  j.run();
+  r->job->fetch_saif(r->saif);
+  if (r->keep_waves) r->job->fetch_waves(r->woff, r->wlen, r->pool);
+  r->executed = true;
+  R.seg_events = best_seg;
+  R.level_blocks = (uint32_t)j.stats().level_blocks;
+  R.prefetch = best_pf;
+  R.best_ms = j.stats().kernel_ms;
+  if (rep) *rep = R;
+  API_END
+}
+
 void gls_run_free(gls_run* r) { delete r; }
+
+// ------------------------------------------------------------- serving ---
+
+struct gls_server {
+  std::unique_ptr<Server> srv;
+  std::mutex mu;
+  std::vector<std::shared_ptr<const Stimulus>> stims;
+};
+
+int gls_server_create(gls_ctx* c, const gls_server_opts* o, gls_server** out) {
+  API_BEGIN
+  if (!c || !out) throw GlsError(GLS_E_ARG, "null argument");
+  ServerOptions so;
+  if (o) {
+    so.device_budget = o->device_budget;
+    so.max_running = o->max_running ? o->max_running : 4;
+    so.seg_events = o->seg_events;
+    so.pool_factor = o->pool_factor > 0 ? o->pool_factor : 1.0;
+  }
+  auto s = std::make_unique<gls_server>();
+  s->srv = std::make_unique<Server>(c->device, so, &c->cache);
+  *out = s.release();
+  API_END
+}
+
+int gls_server_submit(gls_server* s, const gls_design* d, const gls_stim* st, uint64_t dumpon,
+                      uint64_t dumpoff, uint64_t* job) {
+  API_BEGIN
+  if (!s || !d || !st || !job) throw GlsError(GLS_E_ARG, "null argument");
+  if (dumpoff <= dumpon) throw GlsError(GLS_E_ARG, "dumpoff time must be above dumpon time");
+  // the server keeps its own copy of the stimulus: the caller may free theirs
+  auto keep = std::make_shared<const Stimulus>(st->st);
+  RunInst in = make_inst(const_cast<gls_design*>(d), st, dumpon, dumpoff);
+  in.st = keep.get();
+  *job = s->srv->submit(std::move(in), keep);
+  API_END
+}
+
+int gls_server_wait(gls_server* s, uint64_t job, double timeout_s, int* status) {
+  API_BEGIN
+  if (!s || !status) throw GlsError(GLS_E_ARG, "null argument");
+  *status = s->srv->wait(job, timeout_s);
+  API_END
+}
+
+int gls_server_job_info(gls_server* s, uint64_t job, gls_job_info* info) {
+  API_BEGIN
+  if (!s || !info) throw GlsError(GLS_E_ARG, "null argument");
+  ServerJobInfo j = s->srv->info(job);
+  info->status = j.status;
+  info->queue_ms = j.queue_ms;
+  info->run_ms = j.run_ms;
+  info->latency_ms = j.latency_ms;
+  info->n_events = j.n_events;
+  info->device_bytes = j.device_bytes;
+  info->attempts = j.attempts;
+  info->resident = j.resident;
+  info->overlap = j.overlap;
+  API_END
+}
+
+int gls_server_job_saif_text(gls_server* s, uint64_t job, char* buf, size_t cap, size_t* len) {
+  API_BEGIN
+  if (!s || !len) throw GlsError(GLS_E_ARG, "null argument");
+  copy_text(s->srv->saif(job), buf, cap, len);
+  API_END
+}
+
+int gls_server_drain(gls_server* s) {
+  API_BEGIN
+  if (!s) throw GlsError(GLS_E_ARG, "null argument");
+  s->srv->drain();
+  API_END
+}
+
+int gls_server_stats_get(gls_server* s, gls_server_stats* st) {
+  API_BEGIN
+  if (!s || !st) throw GlsError(GLS_E_ARG, "null argument");
+  ServerStats x = s->srv->stats();
+  st->submitted = x.submitted;
+  st->done = x.done;
+  st->failed = x.failed;
+  st->events = x.events;
+  st->p50_ms = x.p50_ms;
+  st->p99_ms = x.p99_ms;
+  st->mean_ms = x.mean_ms;
+  st->wall_s = x.wall_s;
+  st->events_per_s = x.events_per_s;
+  st->peak_running = x.peak_running;
+  st->resident_programs = x.resident_programs;
+  st->peak_bytes = x.peak_bytes;
+  API_END
+}
+
+void gls_server_destroy(gls_server* s) { delete s; }
 
 }  // extern "C"

@@ -37,7 +37,7 @@ struct alignas(16) GateRec {
   uint8_t k;
   uint8_t pad0;
   uint16_t dmax;
-  uint16_t pad1;
+  uint16_t job;  // instance of a batch (SAIF window)
 };
 static_assert(sizeof(GateRec) == 32, "GateRec layout");
 
@@ -65,7 +65,7 @@ struct alignas(16) TileDesc {
   uint32_t lutw[4];                // packed truth table words (fan-in <= 3)
   uint32_t g, ti, nseg, stride;    // gate (index in the level), tile of the gate, segments
   uint32_t lead, k, dt_off, lut;   // leader input, fan-in, delay table, truth table id
-  uint32_t dmax, out_net, pad[2];
+  uint32_t dmax, out_net, job, pad;
 };
 static_assert(sizeof(TileDesc) == 128, "TileDesc layout");
 
@@ -90,7 +90,7 @@ struct LevelArgs {
   unsigned long long* saif;
   unsigned long long* ev_count;
   unsigned int* err;
-  unsigned long long start, end;
+  const unsigned long long* win;  // SAIF window (start, end) per instance
   uint32_t cap;                // LDS input capacity per tile (words)
   uint32_t blk;                // consecutive tiles handed out per ticket
   uint32_t prefetch;           // 1: stage the next tile before storing this one

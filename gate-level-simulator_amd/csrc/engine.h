@@ -1,10 +1,14 @@
-// Device engine: uploads a compiled program + stimulus into HBM and runs the
-// level kernels.  All device memory is owned here.
+// Device engine: uploads compiled programs and stimulus into HBM and runs
+// the level kernels.  All device memory is owned here.
 #pragma once
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <map>
+#include <memory>
+#include <mutex>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "csr.h"
@@ -21,13 +25,69 @@ struct JobStats {
   uint64_t alg_bytes = 0;       // algorithmic bytes of the level kernels
   double kernel_ms = 0;         // device time of the level loop (hipEvent)
   double wall_ms = 0;           // host wall time of the level loop
-  double eval_ms = 0;           // summed device time of the k_eval launches
+  double eval_ms = 0;           // summed device time of the gate-evaluation launches
   uint64_t eval_launches = 0;
   uint64_t level_blocks = 0;    // workgroups per fused level launch
   uint64_t pool_words_used = 0;
 };
 
-// A prepared simulation: program, delays, stimulus resident on the device.
+// Host and device image of one compiled program for one delay time scale:
+// gate records (local net ids), delay tables, truth tables.  Shared by every
+// job that simulates the program, so a resident netlist is uploaded once.
+struct DevProgram {
+  int device = -1;
+  uint32_t n_gates = 0, n_nets = 0, n_pins = 0, timescale_ps = 1;
+  std::vector<uint32_t> level_start;
+  std::vector<GateRec> h_grec;
+  std::vector<GateDev> h_gdev;
+  std::vector<uint16_t> h_dtab;
+  std::vector<uint8_t> h_luts;   // 256 bytes per table
+  std::vector<uint32_t> h_lutp;  // 16 packed words per table
+  GateRec* d_grec = nullptr;
+  GateDev* d_gdev = nullptr;
+  uint16_t* d_dtab = nullptr;
+  uint8_t* d_luts = nullptr;
+  uint32_t* d_lutp = nullptr;
+  std::mutex upload_mu;
+  DevProgram() = default;
+  DevProgram(const DevProgram&) = delete;
+  DevProgram& operator=(const DevProgram&) = delete;
+  ~DevProgram();
+  size_t device_bytes() const;
+  void upload(int device, hipStream_t st);  // no-op when already resident
+};
+
+std::shared_ptr<DevProgram> build_program_image(const Program& pg, uint32_t timescale_ps);
+
+// One simulation instance: a program, its stimulus (signal i drives net
+// stim_nets[i]) and its SAIF window.
+struct SimInstance {
+  const Program* pg = nullptr;
+  std::shared_ptr<DevProgram> image;  // optional: resident image of pg
+  const Stimulus* st = nullptr;
+  std::vector<int> stim_nets;
+  uint64_t start = 0, end = 0;
+};
+
+// Resident program images of one device, by (program, time scale).
+class ProgramCache {
+ public:
+  // The image of pg (built and cached on first use; the cache keeps pg alive).
+  std::shared_ptr<DevProgram> get(const std::shared_ptr<const Program>& pg, uint32_t timescale_ps);
+  bool resident(const Program* pg, uint32_t timescale_ps) const;
+  size_t device_bytes() const;
+  size_t size() const;
+  void clear();
+
+ private:
+  mutable std::mutex mu_;
+  std::map<std::pair<const Program*, uint32_t>,
+           std::pair<std::shared_ptr<const Program>, std::shared_ptr<DevProgram>>>
+      map_;
+};
+
+// A prepared simulation of one or more instances, run level by level: level
+// l of every instance is evaluated by the same launches.
 class Job {
  public:
   Job(int device, hipStream_t stream);
@@ -35,43 +95,49 @@ class Job {
   Job(const Job&) = delete;
   Job& operator=(const Job&) = delete;
 
-  // Compile-time inputs; stim_nets[i] is the net id of stimulus signal i.
-  void prepare(const Program& pg, const Stimulus& st, const std::vector<int>& stim_nets,
-               uint64_t start, uint64_t end, uint64_t pool_words);
-  // Run all levels (plain per-level launches). Returns GLS status.
+  void prepare(const std::vector<SimInstance>& inst, uint64_t pool_words);
+  // Run all levels.  Throws GlsError.
   void run();
-  // Results
+  // Results (union net space; instance i's nets start at net_base(i))
   void fetch_saif(std::vector<uint64_t>& t0t1tx) const;
   void fetch_waves(std::vector<uint64_t>& off, std::vector<uint32_t>& len,
                    std::vector<uint32_t>& pool_words) const;
   const JobStats& stats() const { return stats_; }
-  uint32_t seg_target = 0;  // input events per time segment (0: mode default)
-  int mode = 0;             // 0: fused LDS-tiled level kernel, 1: two-pass kernels
+  size_t n_instances() const { return net_base_.size(); }
+  uint32_t net_base(size_t i) const { return net_base_[i]; }
+  uint64_t device_bytes() const { return dev_bytes_; }
+
+  uint32_t seg_target = 0;        // input events per time segment (0: mode default)
+  int mode = 0;                   // 0: fused LDS-tiled level kernel, 1: two-pass kernels
   uint32_t level_blocks_opt = 0;  // workgroups per fused launch (0: occupancy limit)
   uint32_t tiles_per_ticket = 0;  // fused tiles handed out per ticket (0: one)
   uint32_t prefetch = 1;          // stage the next tile before storing the current one
+  uint64_t ovf_words_opt = 0;     // overflow stack words (0: 2^24)
+  static constexpr uint32_t kDefaultSeg = 32;  // input events per time segment
 
  private:
   void ensure(void** p, size_t& cap, size_t need);
   void run_fused(std::vector<hipEvent_t>& ev_eval);
   void run_two_pass(std::vector<hipEvent_t>& ev_eval);
+  void* dmalloc(size_t bytes);
   int device_;
   hipStream_t stream_;
-  uint64_t start_ = 0, end_ = 0;
+  std::shared_ptr<DevProgram> prog_;
+  std::vector<uint32_t> net_base_;
+  std::vector<uint64_t> win_;  // start, end per instance
   uint32_t n_nets_ = 0, n_gates_ = 0;
   std::vector<uint32_t> level_start_;
-  std::vector<uint32_t> saif_net_list_;
-  uint64_t pool_words_ = 0, stim_words_ = 0;
-  // device buffers
-  GateDev* d_gates_ = nullptr;
-  GateRec* d_grec_ = nullptr;
-  uint8_t* d_luts_ = nullptr;
+  std::vector<uint32_t> saif_net_list_, saif_net_job_;
+  uint64_t pool_words_ = 0, stim_words_ = 0, dev_bytes_ = 0;
+  // device buffers of the job
   uint64_t* d_woff_ = nullptr;
   uint32_t* d_wlen_ = nullptr;
   uint32_t* d_pool_ = nullptr;
   unsigned long long* d_saif_ = nullptr;
   uint32_t* d_saif_nets_ = nullptr;
-  uint64_t* d_scal_ = nullptr;  // [0]=bump [1]=total_segs [2]=total_scr [3]=total_out [4]=ev_count [5]=overflow
+  uint32_t* d_saif_job_ = nullptr;
+  unsigned long long* d_win_ = nullptr;
+  uint64_t* d_scal_ = nullptr;
   // per-level temporaries
   uint64_t *d_nseg_ = nullptr, *d_scr_ = nullptr, *d_segbase_ = nullptr, *d_scrbase_ = nullptr;
   uint64_t *d_outlen_ = nullptr, *d_outoff_ = nullptr, *d_part_ = nullptr;
@@ -86,18 +152,15 @@ class Job {
   size_t cap_tiles_ = 0, cap_tstate_ = 0;
   GatePlan* d_plan_ = nullptr;
   TileDesc* d_tiles_ = nullptr;
-  uint32_t* d_lutp_ = nullptr;
-  uint16_t* d_dtab_ = nullptr;
   unsigned long long* d_tstate_ = nullptr;
   uint32_t* d_ovf_ = nullptr;
   uint64_t ovf_words_ = 0;
   int level_blocks_ = 0;
-  uint32_t cap_ = 0;                          // LDS input words per fused tile
-  static constexpr uint32_t kDefaultSeg = 32;  // input events per time segment
+  uint32_t cap_ = 0;  // LDS input words per fused tile
+  std::vector<void*> owned_;
   std::vector<uint64_t> h_woff_init_;
   std::vector<uint32_t> h_wlen_init_;
   JobStats stats_;
-  std::vector<std::vector<uint32_t>> gate_in_nets_;  // for stats
 };
 
 void hip_check(hipError_t e, const char* what);

@@ -31,9 +31,10 @@ hipError_t launch_emit(const GateDev* gates, const uint64_t* seg_base, const uin
                        uint32_t* pool, uint64_t* woff, uint32_t* wlen, unsigned long long* saif,
                        uint64_t start, uint64_t end, hipStream_t st);
 hipError_t launch_bump(uint64_t* bump, const uint64_t* total, hipStream_t st);
-hipError_t launch_saif_nets(const uint32_t* nets, uint32_t count, const uint64_t* woff,
-                            const uint32_t* wlen, const uint32_t* pool, unsigned long long* saif,
-                            uint64_t start, uint64_t end, hipStream_t st);
+hipError_t launch_saif_nets(const uint32_t* nets, const uint32_t* job, uint32_t count,
+                            const uint64_t* woff, const uint32_t* wlen, const uint32_t* pool,
+                            unsigned long long* saif, const unsigned long long* win,
+                            hipStream_t st);
 
 // level.hip: fused path
 hipError_t launch_plan2(const GateRec* gates, uint32_t ng, const uint32_t* wlen,
@@ -66,27 +67,6 @@ void hip_check(hipError_t e, const char* what) {
 // scalar slots in d_scal_
 enum { S_BUMP = 0, S_TOT_A = 1, S_TOT_B = 2, S_TOT_OUT = 3, S_EVC = 4, S_TICKET = 5,
        S_OVF_USED = 6, S_ERR = 7 };
-
-Job::Job(int device, hipStream_t stream) : device_(device), stream_(stream) {}
-
-Job::~Job() {
-  void* ptrs[] = {d_gates_, d_grec_, d_luts_, d_woff_, d_wlen_, d_pool_, d_saif_, d_saif_nets_,
-                  d_scal_, d_nseg_, d_scr_, d_segbase_, d_scrbase_, d_outlen_, d_outoff_, d_part_,
-                  d_gnseg_, d_segmap_, d_sums_, d_carry_, d_scratch_, d_ntile_, d_bound_,
-                  d_tilebase_, d_plan_, d_tiles_, d_lutp_, d_dtab_, d_tstate_, d_ovf_};
-  for (void* p : ptrs)
-    if (p) (void)hipFree(p);
-}
-
-void Job::ensure(void** p, size_t& cap, size_t need) {
-  if (need <= cap && *p) return;
-  size_t n = std::max<size_t>(need, cap + cap / 2);
-  n = std::max<size_t>(n, 4096);
-  if (*p) HC(hipFree(*p));
-  *p = nullptr;
-  HC(hipMalloc(p, n));
-  cap = n;
-}
 
 static uint32_t sel_delay(uint32_t r, uint32_t f, int s) {
   return s == 1 ? r : (s == 2 ? f : (s == 3 ? std::max(r, f) : std::min(r, f)));
@@ -141,29 +121,55 @@ static void fill_delay_table(const GateDelays& D, int k, uint16_t* tab) {
       }
 }
 
-void Job::prepare(const Program& pg, const Stimulus& st, const std::vector<int>& stim_nets,
-                  uint64_t start, uint64_t end, uint64_t pool_words) {
-  HC(hipSetDevice(device_));
-  if (end <= start) throw GlsError(GLS_E_ARG, "dumpoff time must be above dumpon time");
-  start_ = start;
-  end_ = end;
-  n_gates_ = (uint32_t)pg.gates.size();
-  level_start_ = pg.level_start;
-  n_nets_ = (uint32_t)pg.net_names.size();
-  for (int sn : stim_nets) n_nets_ = std::max<uint32_t>(n_nets_, (uint32_t)sn + 1);
+// ------------------------------------------------------------- programs ---
 
-  // gate records for both paths
-  std::vector<GateDev> gd(n_gates_);
-  std::vector<GateRec> gr(n_gates_);
-  std::vector<uint16_t> dtab;
-  gate_in_nets_.assign(n_gates_, {});
-  for (uint32_t i = 0; i < n_gates_; ++i) {
+DevProgram::~DevProgram() {
+  void* ptrs[] = {d_grec, d_gdev, d_dtab, d_luts, d_lutp};
+  for (void* p : ptrs)
+    if (p) (void)hipFree(p);
+}
+
+size_t DevProgram::device_bytes() const {
+  return h_grec.size() * sizeof(GateRec) + h_gdev.size() * sizeof(GateDev) +
+         h_dtab.size() * 2 + h_luts.size() + h_lutp.size() * 4;
+}
+
+void DevProgram::upload(int dev, hipStream_t st) {
+  std::lock_guard<std::mutex> lk(upload_mu);
+  if (d_grec) {
+    if (dev != device) throw GlsError(GLS_E_ARG, "program image belongs to another device");
+    return;
+  }
+  device = dev;
+  auto put = [&](void** d, const void* h, size_t bytes) {
+    HC(hipMalloc(d, std::max<size_t>(bytes, 256)));
+    if (bytes) HC(hipMemcpyAsync(*d, h, bytes, hipMemcpyHostToDevice, st));
+  };
+  put((void**)&d_grec, h_grec.data(), h_grec.size() * sizeof(GateRec));
+  put((void**)&d_gdev, h_gdev.data(), h_gdev.size() * sizeof(GateDev));
+  put((void**)&d_dtab, h_dtab.data(), h_dtab.size() * 2);
+  put((void**)&d_luts, h_luts.data(), h_luts.size());
+  put((void**)&d_lutp, h_lutp.data(), h_lutp.size() * 4);
+  HC(hipStreamSynchronize(st));
+}
+
+std::shared_ptr<DevProgram> build_program_image(const Program& pg, uint32_t timescale_ps) {
+  auto P = std::make_shared<DevProgram>();
+  P->timescale_ps = timescale_ps;
+  P->n_gates = (uint32_t)pg.gates.size();
+  P->n_nets = (uint32_t)pg.net_names.size();
+  P->n_pins = (uint32_t)pg.n_pins;
+  P->level_start = pg.level_start;
+  P->h_grec.resize(P->n_gates);
+  P->h_gdev.resize(P->n_gates);
+  for (uint32_t i = 0; i < P->n_gates; ++i) {
     const CompiledGate& g = pg.gates[i];
     if (g.k < 1) throw GlsError(GLS_E_UNSUPPORTED, "gate without inputs");
-    GateDelays D = gate_delays(g, st.timescale_ps);
-    GateDev& r = gd[i];
+    if (g.k > 4) throw GlsError(GLS_E_UNSUPPORTED, "gate with more than 4 distinct inputs");
+    GateDelays D = gate_delays(g, timescale_ps);
+    GateDev& r = P->h_gdev[i];
     std::memset(&r, 0, sizeof r);
-    GateRec& q = gr[i];
+    GateRec& q = P->h_grec[i];
     std::memset(&q, 0, sizeof q);
     for (int j = 0; j < 4; ++j) {
       r.in_net[j] = j < g.k ? (uint32_t)g.in_net[j] : 0u;
@@ -173,98 +179,249 @@ void Job::prepare(const Program& pg, const Stimulus& st, const std::vector<int>&
         r.dly[j][kd][0] = D.v[j][kd][0];
         r.dly[j][kd][1] = D.v[j][kd][1];
       }
-      if (j < g.k) gate_in_nets_[i].push_back((uint32_t)g.in_net[j]);
     }
     r.out_net = q.out_net = (uint32_t)g.out_net;
     r.lut = q.lut = (uint16_t)g.lut;
     r.k = q.k = (uint8_t)g.k;
     r.dmax = q.dmax = D.dmax;
-    if (g.k > 4) throw GlsError(GLS_E_UNSUPPORTED, "gate with more than 4 distinct inputs");
-    if (dtab.size() + dt_size(g.k) >= (1ull << 32))
+    if (P->h_dtab.size() + dt_size(g.k) >= (1ull << 32))
       throw GlsError(GLS_E_RANGE, "delay tables exceed 2^32 entries");
-    q.dt_off = (uint32_t)dtab.size();
-    dtab.resize(dtab.size() + dt_size(g.k));
-    fill_delay_table(D, g.k, dtab.data() + q.dt_off);
+    q.dt_off = (uint32_t)P->h_dtab.size();
+    P->h_dtab.resize(P->h_dtab.size() + dt_size(g.k));
+    fill_delay_table(D, g.k, P->h_dtab.data() + q.dt_off);
   }
-  std::vector<uint8_t> luts(pg.luts.size() * 256, 1);
-  for (size_t i = 0; i < pg.luts.size(); ++i)
-    std::memcpy(&luts[i * 256], pg.luts[i].data(), pg.luts[i].size());
+  const size_t nl = std::max<size_t>(1, pg.luts.size());
+  P->h_luts.assign(nl * 256, 1);
+  P->h_lutp.assign(nl * 16, 0);
+  for (size_t i = 0; i < pg.luts.size(); ++i) {
+    std::memcpy(&P->h_luts[i * 256], pg.luts[i].data(), std::min<size_t>(256, pg.luts[i].size()));
+    // packed 2 bits per entry for register lookups (fan-in <= 3)
+    for (size_t e = 0; e < pg.luts[i].size() && e < 256; ++e)
+      P->h_lutp[i * 16 + e / 16] |= (uint32_t)(pg.luts[i][e] & 3u) << (2 * (e % 16));
+  }
+  return P;
+}
 
-  // initial waveforms: stimulus signals, then constants
+std::shared_ptr<DevProgram> ProgramCache::get(const std::shared_ptr<const Program>& pg,
+                                             uint32_t timescale_ps) {
+  std::lock_guard<std::mutex> lk(mu_);
+  auto key = std::make_pair(pg.get(), timescale_ps);
+  auto it = map_.find(key);
+  if (it != map_.end()) return it->second.second;
+  auto img = build_program_image(*pg, timescale_ps);
+  map_[key] = {pg, img};
+  return img;
+}
+
+bool ProgramCache::resident(const Program* pg, uint32_t timescale_ps) const {
+  std::lock_guard<std::mutex> lk(mu_);
+  auto it = map_.find(std::make_pair(pg, timescale_ps));
+  return it != map_.end() && it->second.second->d_grec != nullptr;
+}
+
+size_t ProgramCache::device_bytes() const {
+  std::lock_guard<std::mutex> lk(mu_);
+  size_t b = 0;
+  for (auto& kv : map_)
+    if (kv.second.second->d_grec) b += kv.second.second->device_bytes();
+  return b;
+}
+
+size_t ProgramCache::size() const {
+  std::lock_guard<std::mutex> lk(mu_);
+  return map_.size();
+}
+
+void ProgramCache::clear() {
+  std::lock_guard<std::mutex> lk(mu_);
+  map_.clear();
+}
+
+// Union of several instances' images: nets of instance i are shifted by
+// net_base[i], gates are merged level by level, each gate records its
+// instance; truth and delay tables are shared per distinct image.
+static std::shared_ptr<DevProgram> union_image(const std::vector<std::shared_ptr<DevProgram>>& parts,
+                                               const std::vector<uint32_t>& net_base) {
+  auto U = std::make_shared<DevProgram>();
+  std::map<const DevProgram*, std::pair<uint32_t, uint32_t>> bases;  // dtab, lut
+  for (auto& p : parts) {
+    if (bases.count(p.get())) continue;
+    bases[p.get()] = {(uint32_t)U->h_dtab.size(), (uint32_t)(U->h_luts.size() / 256)};
+    if (U->h_dtab.size() + p->h_dtab.size() >= (1ull << 32))
+      throw GlsError(GLS_E_RANGE, "delay tables exceed 2^32 entries");
+    U->h_dtab.insert(U->h_dtab.end(), p->h_dtab.begin(), p->h_dtab.end());
+    U->h_luts.insert(U->h_luts.end(), p->h_luts.begin(), p->h_luts.end());
+    U->h_lutp.insert(U->h_lutp.end(), p->h_lutp.begin(), p->h_lutp.end());
+  }
+  if (U->h_luts.size() / 256 > 65535) throw GlsError(GLS_E_RANGE, "too many truth tables in one batch");
+  size_t nlev = 0;
+  for (auto& p : parts) nlev = std::max(nlev, p->level_start.size() - 1);
+  U->level_start.assign(1, 0);
+  for (size_t l = 0; l < nlev; ++l) {
+    for (size_t i = 0; i < parts.size(); ++i) {
+      const DevProgram& p = *parts[i];
+      if (l + 1 >= p.level_start.size()) continue;
+      const auto b = bases[&p];
+      for (uint32_t g = p.level_start[l]; g < p.level_start[l + 1]; ++g) {
+        GateRec q = p.h_grec[g];
+        GateDev r = p.h_gdev[g];
+        for (int j = 0; j < 4; ++j) {
+          if (j < q.k) q.in_net[j] += net_base[i];
+          r.in_net[j] = q.in_net[j];
+        }
+        q.out_net += net_base[i];
+        r.out_net = q.out_net;
+        q.dt_off += b.first;
+        q.lut = r.lut = (uint16_t)(q.lut + b.second);
+        q.job = (uint16_t)i;
+        U->h_grec.push_back(q);
+        U->h_gdev.push_back(r);
+      }
+    }
+    U->level_start.push_back((uint32_t)U->h_grec.size());
+  }
+  U->n_gates = (uint32_t)U->h_grec.size();
+  for (size_t i = 0; i < parts.size(); ++i) {
+    U->n_nets = std::max(U->n_nets, net_base[i] + parts[i]->n_nets);
+    U->n_pins += parts[i]->n_pins;
+  }
+  return U;
+}
+
+// ------------------------------------------------------------------ job ---
+
+Job::Job(int device, hipStream_t stream) : device_(device), stream_(stream) {}
+
+Job::~Job() {
+  for (void* p : owned_)
+    if (p) (void)hipFree(p);
+  void* grown[] = {d_segmap_, d_sums_, d_carry_, d_scratch_, d_tiles_, d_tstate_};
+  for (void* p : grown)
+    if (p) (void)hipFree(p);
+}
+
+void* Job::dmalloc(size_t bytes) {
+  void* p = nullptr;
+  bytes = std::max<size_t>(bytes, 256);
+  HC(hipMalloc(&p, bytes));
+  owned_.push_back(p);
+  dev_bytes_ += bytes;
+  return p;
+}
+
+void Job::ensure(void** p, size_t& cap, size_t need) {
+  if (need <= cap && *p) return;
+  size_t n = std::max<size_t>(need, cap + cap / 2);
+  n = std::max<size_t>(n, 4096);
+  if (*p) HC(hipFree(*p));
+  *p = nullptr;
+  HC(hipMalloc(p, n));
+  dev_bytes_ += n - cap;
+  cap = n;
+}
+
+void Job::prepare(const std::vector<SimInstance>& inst, uint64_t pool_words) {
+  HC(hipSetDevice(device_));
+  if (inst.empty()) throw GlsError(GLS_E_ARG, "no instance to simulate");
+  if (inst.size() > 65535) throw GlsError(GLS_E_ARG, "at most 65535 instances per batch");
+  if (mode == 1 && inst.size() > 1)
+    throw GlsError(GLS_E_UNSUPPORTED, "the two-pass path simulates one instance at a time");
+  // program images and net spaces
+  std::vector<std::shared_ptr<DevProgram>> parts;
+  net_base_.clear();
+  win_.clear();
+  uint64_t nets = 0;
+  for (const SimInstance& in : inst) {
+    if (in.end <= in.start) throw GlsError(GLS_E_ARG, "dumpoff time must be above dumpon time");
+    auto img = in.image ? in.image : build_program_image(*in.pg, (uint32_t)in.st->timescale_ps);
+    if (img->timescale_ps != (uint32_t)in.st->timescale_ps)
+      throw GlsError(GLS_E_ARG, "program image built for another time scale");
+    parts.push_back(img);
+    net_base_.push_back((uint32_t)nets);
+    uint64_t n = img->n_nets;
+    for (int sn : in.stim_nets) n = std::max<uint64_t>(n, (uint64_t)sn + 1);
+    nets += n;
+    if (nets >= (1ull << 32) - 1) throw GlsError(GLS_E_RANGE, "batch exceeds 2^32 nets");
+    win_.push_back(in.start);
+    win_.push_back(in.end);
+  }
+  prog_ = parts.size() == 1 ? parts[0] : union_image(parts, net_base_);
+  prog_->upload(device_, stream_);
+  n_gates_ = prog_->n_gates;
+  n_nets_ = (uint32_t)nets;
+  level_start_ = prog_->level_start;
+
+  // initial waveforms: stimulus signals, then constants, per instance
   std::vector<uint32_t> words;
   h_woff_init_.assign(n_nets_, 0);
   h_wlen_init_.assign(n_nets_, 0);
   saif_net_list_.clear();
-  for (size_t i = 0; i < st.waves.size(); ++i) {
-    int net = stim_nets[i];
-    const Wave& w = st.waves[i];
-    h_woff_init_[net] = words.size();
-    h_wlen_init_[net] = (uint32_t)w.size();
-    for (size_t q = 0; q < w.size(); ++q) {
-      if (w.t[q] >= (1u << 30) - (1u << 17))
-        throw GlsError(GLS_E_RANGE, "stimulus time beyond the 30-bit event word");
-      words.push_back((w.t[q] << 2) | (w.v[q] & 3u));
+  saif_net_job_.clear();
+  for (size_t i = 0; i < inst.size(); ++i) {
+    const SimInstance& in = inst[i];
+    const uint32_t nb = net_base_[i];
+    for (size_t s = 0; s < in.st->waves.size(); ++s) {
+      const uint32_t net = nb + (uint32_t)in.stim_nets[s];
+      const Wave& w = in.st->waves[s];
+      h_woff_init_[net] = words.size();
+      h_wlen_init_[net] = (uint32_t)w.size();
+      for (size_t q = 0; q < w.size(); ++q) {
+        if (w.t[q] >= (1u << 30) - (1u << 17))
+          throw GlsError(GLS_E_RANGE, "stimulus time beyond the 30-bit event word");
+        words.push_back((w.t[q] << 2) | (w.v[q] & 3u));
+      }
+      saif_net_list_.push_back(net);
+      saif_net_job_.push_back((uint32_t)i);
     }
-    saif_net_list_.push_back((uint32_t)net);
-  }
-  for (uint32_t n = 0; n < (uint32_t)pg.net_names.size(); ++n) {
-    if (pg.role[n] != NR_CONST) continue;
-    h_woff_init_[n] = words.size();
-    h_wlen_init_[n] = 1;
-    words.push_back((0u << 2) | (uint32_t)(pg.const_val[n] & 3));
-    saif_net_list_.push_back(n);
+    const Program& pg = *in.pg;
+    for (uint32_t n = 0; n < (uint32_t)pg.net_names.size(); ++n) {
+      if (pg.role[n] != NR_CONST) continue;
+      h_woff_init_[nb + n] = words.size();
+      h_wlen_init_[nb + n] = 1;
+      words.push_back((0u << 2) | (uint32_t)(pg.const_val[n] & 3));
+      saif_net_list_.push_back(nb + n);
+      saif_net_job_.push_back((uint32_t)i);
+    }
   }
   stim_words_ = words.size();
 
-  HC(hipMalloc((void**)&d_gates_, std::max<size_t>(1, gd.size()) * sizeof(GateDev)));
-  HC(hipMalloc((void**)&d_grec_, std::max<size_t>(1, gr.size()) * sizeof(GateRec)));
-  HC(hipMalloc((void**)&d_luts_, std::max<size_t>(256, luts.size())));
-  HC(hipMalloc((void**)&d_woff_, n_nets_ * sizeof(uint64_t)));
-  HC(hipMalloc((void**)&d_wlen_, n_nets_ * sizeof(uint32_t)));
-  HC(hipMalloc((void**)&d_saif_, (size_t)n_nets_ * 3 * sizeof(unsigned long long)));
-  HC(hipMalloc((void**)&d_saif_nets_, std::max<size_t>(1, saif_net_list_.size()) * 4));
-  HC(hipMalloc((void**)&d_scal_, 16 * sizeof(uint64_t)));
+  d_woff_ = (uint64_t*)dmalloc(n_nets_ * sizeof(uint64_t));
+  d_wlen_ = (uint32_t*)dmalloc(n_nets_ * sizeof(uint32_t));
+  d_saif_ = (unsigned long long*)dmalloc((size_t)n_nets_ * 3 * sizeof(unsigned long long));
+  d_saif_nets_ = (uint32_t*)dmalloc(saif_net_list_.size() * 4);
+  d_saif_job_ = (uint32_t*)dmalloc(saif_net_job_.size() * 4);
+  d_win_ = (unsigned long long*)dmalloc(win_.size() * 8);
+  d_scal_ = (uint64_t*)dmalloc(16 * sizeof(uint64_t));
   uint32_t maxw = 1;
   for (size_t l = 0; l + 1 < level_start_.size(); ++l)
     maxw = std::max(maxw, level_start_[l + 1] - level_start_[l]);
-  for (uint64_t** p : {&d_nseg_, &d_scr_, &d_segbase_, &d_scrbase_, &d_outlen_, &d_outoff_,
-                       &d_ntile_, &d_bound_, &d_tilebase_})
-    HC(hipMalloc((void**)p, maxw * 8ull));
-  HC(hipMalloc((void**)&d_part_, scan_part_size(maxw) * 8ull));
-  HC(hipMalloc((void**)&d_gnseg_, maxw * 4ull));
-  HC(hipMalloc((void**)&d_plan_, maxw * sizeof(GatePlan)));
-  ovf_words_ = 1ull << 24;  // stacks of tiles that do not fit LDS (64 MiB)
-  HC(hipMalloc((void**)&d_ovf_, ovf_words_ * 4));
-  // truth tables packed 2 bits per entry (16 words each) for register lookups
-  std::vector<uint32_t> lutp(std::max<size_t>(1, pg.luts.size()) * 16, 0);
-  for (size_t i = 0; i < pg.luts.size(); ++i)
-    for (size_t e = 0; e < pg.luts[i].size() && e < 256; ++e)
-      lutp[i * 16 + e / 16] |= (uint32_t)(pg.luts[i][e] & 3u) << (2 * (e % 16));
-  HC(hipMalloc((void**)&d_lutp_, lutp.size() * 4));
-  HC(hipMemcpyAsync(d_lutp_, lutp.data(), lutp.size() * 4, hipMemcpyHostToDevice, stream_));
-  HC(hipMalloc((void**)&d_dtab_, std::max<size_t>(1, dtab.size()) * 2));
-  if (!dtab.empty())
-    HC(hipMemcpyAsync(d_dtab_, dtab.data(), dtab.size() * 2, hipMemcpyHostToDevice, stream_));
-  HC(hipStreamSynchronize(stream_));
+  for (uint64_t** p : {&d_outoff_, &d_ntile_, &d_bound_, &d_tilebase_})
+    *p = (uint64_t*)dmalloc(maxw * 8ull);
+  if (mode == 1)
+    for (uint64_t** p : {&d_nseg_, &d_scr_, &d_segbase_, &d_scrbase_, &d_outlen_})
+      *p = (uint64_t*)dmalloc(maxw * 8ull);
+  d_part_ = (uint64_t*)dmalloc(scan_part_size(maxw) * 8ull);
+  d_gnseg_ = (uint32_t*)dmalloc(maxw * 4ull);
+  d_plan_ = (GatePlan*)dmalloc(maxw * sizeof(GatePlan));
+  ovf_words_ = ovf_words_opt ? ovf_words_opt : (1ull << 24);  // stacks of tiles that do not fit LDS
+  d_ovf_ = (uint32_t*)dmalloc(ovf_words_ * 4);
 
   size_t free_b = 0, total_b = 0;
   HC(hipMemGetInfo(&free_b, &total_b));
   if (pool_words == 0) pool_words = (uint64_t)(free_b * 0.85) / 4;
   if (pool_words < stim_words_ + 1024) pool_words = stim_words_ + 1024;
   pool_words_ = pool_words;
-  HC(hipMalloc((void**)&d_pool_, pool_words_ * sizeof(uint32_t)));
+  d_pool_ = (uint32_t*)dmalloc(pool_words_ * sizeof(uint32_t));
 
-  if (!gd.empty()) {
-    HC(hipMemcpyAsync(d_gates_, gd.data(), gd.size() * sizeof(GateDev), hipMemcpyHostToDevice, stream_));
-    HC(hipMemcpyAsync(d_grec_, gr.data(), gr.size() * sizeof(GateRec), hipMemcpyHostToDevice, stream_));
-  }
-  if (!luts.empty())
-    HC(hipMemcpyAsync(d_luts_, luts.data(), luts.size(), hipMemcpyHostToDevice, stream_));
   if (!words.empty())
     HC(hipMemcpyAsync(d_pool_, words.data(), words.size() * 4, hipMemcpyHostToDevice, stream_));
-  if (!saif_net_list_.empty())
+  if (!saif_net_list_.empty()) {
     HC(hipMemcpyAsync(d_saif_nets_, saif_net_list_.data(), saif_net_list_.size() * 4,
                       hipMemcpyHostToDevice, stream_));
+    HC(hipMemcpyAsync(d_saif_job_, saif_net_job_.data(), saif_net_job_.size() * 4,
+                      hipMemcpyHostToDevice, stream_));
+  }
+  HC(hipMemcpyAsync(d_win_, win_.data(), win_.size() * 8, hipMemcpyHostToDevice, stream_));
   HC(hipStreamSynchronize(stream_));
   cap_ = tile_cap(seg_target ? seg_target : kDefaultSeg);
   level_blocks_ = level_blocks_opt ? (int)level_blocks_opt : level_max_blocks(cap_);
@@ -272,7 +429,7 @@ void Job::prepare(const Program& pg, const Stimulus& st, const std::vector<int>&
   stats_.level_blocks = (uint64_t)level_blocks_;
   stats_.n_nets = n_nets_;
   stats_.n_gates = n_gates_;
-  stats_.n_pins = pg.n_pins;
+  stats_.n_pins = prog_->n_pins;
 }
 
 void Job::run_fused(std::vector<hipEvent_t>& ev_eval) {
@@ -282,7 +439,7 @@ void Job::run_fused(std::vector<hipEvent_t>& ev_eval) {
   for (size_t l = 0; l + 1 < level_start_.size(); ++l) {
     uint32_t g0 = level_start_[l], ng = level_start_[l + 1] - g0;
     if (ng == 0) continue;
-    const GateRec* gates = d_grec_ + g0;
+    const GateRec* gates = prog_->d_grec + g0;
     HC(launch_plan2(gates, ng, d_wlen_, seg, d_plan_, d_ntile_, d_bound_,
                     (unsigned int*)(d_scal_ + S_ERR), stream_));
     HC(launch_scan(d_ntile_, d_tilebase_, ng, d_part_, d_scal_ + S_TOT_A, nullptr, stream_));
@@ -297,7 +454,7 @@ void Job::run_fused(std::vector<hipEvent_t>& ev_eval) {
     if (ntiles >= 0xFFFFFFF0ull) throw GlsError(GLS_E_RANGE, "too many tiles in one level");
     ensure((void**)&d_tiles_, cap_tiles_, ntiles * sizeof(TileDesc));
     ensure((void**)&d_tstate_, cap_tstate_, ntiles * 16);
-    HC(launch_tile_desc(gates, ng, d_plan_, d_tilebase_, d_woff_, d_wlen_, d_pool_, d_lutp_,
+    HC(launch_tile_desc(gates, ng, d_plan_, d_tilebase_, d_woff_, d_wlen_, d_pool_, prog_->d_lutp,
                         d_tiles_, ntiles, stream_));
     HC(launch_fill_u64(d_tstate_, 2 * ntiles, 0ull, stream_));
     HC(launch_fill_u64((unsigned long long*)(d_scal_ + S_TICKET), 2, 0ull, stream_));  // ticket, ovf
@@ -305,14 +462,14 @@ void Job::run_fused(std::vector<hipEvent_t>& ev_eval) {
     a.gates = gates;
     a.plan = d_plan_;
     a.tiles = d_tiles_;
-    a.lutp = d_lutp_;
-    a.dtab = d_dtab_;
+    a.lutp = prog_->d_lutp;
+    a.dtab = prog_->d_dtab;
     a.out_base = d_outoff_;
     a.ntiles = ntiles;
     a.woff = d_woff_;
     a.wlen = d_wlen_;
     a.pool = d_pool_;
-    a.luts = d_luts_;
+    a.luts = prog_->d_luts;
     a.tstate = d_tstate_;
     a.ticket = (unsigned int*)(d_scal_ + S_TICKET);
     a.ovf = d_ovf_;
@@ -321,8 +478,7 @@ void Job::run_fused(std::vector<hipEvent_t>& ev_eval) {
     a.saif = d_saif_;
     a.ev_count = evc;
     a.err = (unsigned int*)(d_scal_ + S_ERR);
-    a.start = start_;
-    a.end = end_;
+    a.win = d_win_;
     a.cap = cap_;
     unsigned nb = (unsigned)std::min<uint64_t>(ntiles, (uint64_t)std::max(1, level_blocks_));
     // one tile per ticket by default: a wavefront holding several tiles of a
@@ -349,7 +505,7 @@ void Job::run_two_pass(std::vector<hipEvent_t>& ev_eval) {
   for (size_t l = 0; l + 1 < level_start_.size(); ++l) {
     uint32_t g0 = level_start_[l], ng = level_start_[l + 1] - g0;
     if (ng == 0) continue;
-    const GateDev* gates = d_gates_ + g0;
+    const GateDev* gates = prog_->d_gdev + g0;
     HC(launch_plan(gates, ng, d_wlen_, seg, d_nseg_, d_scr_, d_gnseg_, stream_));
     HC(launch_scan(d_nseg_, d_segbase_, ng, d_part_, d_scal_ + S_TOT_A, nullptr, stream_));
     HC(launch_scan(d_scr_, d_scrbase_, ng, d_part_, d_scal_ + S_TOT_B, nullptr, stream_));
@@ -373,19 +529,22 @@ void Job::run_two_pass(std::vector<hipEvent_t>& ev_eval) {
     ev_eval.push_back(e_b);
     HC(hipEventRecord(e_a, stream_));
     HC(launch_eval(gates, d_segbase_, d_scrbase_, d_gnseg_, d_segmap_, nsegs, d_woff_, d_wlen_,
-                   d_pool_, d_luts_, d_scratch_, d_sums_, evc, stream_));
+                   d_pool_, prog_->d_luts, d_scratch_, d_sums_, evc, stream_));
     HC(hipEventRecord(e_b, stream_));
     HC(launch_gate_scan(gates, ng, d_segbase_, d_gnseg_, d_sums_, d_carry_, d_outlen_, d_saif_,
-                        start_, end_, stream_));
+                        win_[0], win_[1], stream_));
     HC(launch_scan(d_outlen_, d_outoff_, ng, d_part_, d_scal_ + S_TOT_OUT, bump, stream_));
     HC(launch_emit(gates, d_segbase_, d_segmap_, nsegs, d_sums_, d_carry_, d_outoff_, d_outlen_,
-                   d_scratch_, d_pool_, d_woff_, d_wlen_, d_saif_, start_, end_, stream_));
+                   d_scratch_, d_pool_, d_woff_, d_wlen_, d_saif_, win_[0], win_[1], stream_));
     HC(launch_bump(bump, d_scal_ + S_TOT_OUT, stream_));
   }
 }
 
 void Job::run() {
   HC(hipSetDevice(device_));
+  cap_ = tile_cap(seg_target ? seg_target : kDefaultSeg);
+  level_blocks_ = level_blocks_opt ? (int)level_blocks_opt : level_max_blocks(cap_);
+  stats_.level_blocks = (uint64_t)level_blocks_;
   auto w0 = std::chrono::steady_clock::now();
   hipEvent_t e0, e1;
   HC(hipEventCreate(&e0));
@@ -399,8 +558,8 @@ void Job::run() {
   HC(hipStreamSynchronize(stream_));
 
   HC(hipEventRecord(e0, stream_));
-  HC(launch_saif_nets(d_saif_nets_, (uint32_t)saif_net_list_.size(), d_woff_, d_wlen_, d_pool_,
-                      d_saif_, start_, end_, stream_));
+  HC(launch_saif_nets(d_saif_nets_, d_saif_job_, (uint32_t)saif_net_list_.size(), d_woff_, d_wlen_,
+                      d_pool_, d_saif_, d_win_, stream_));
   std::vector<hipEvent_t> ev_eval;
   try {
     if (mode == 1) run_two_pass(ev_eval);
@@ -428,7 +587,8 @@ void Job::run() {
   stats_.eval_ms = eval_ms;
   stats_.eval_launches = ev_eval.size() / 2;
   uint64_t h[16];
-  HC(hipMemcpy(h, d_scal_, sizeof h, hipMemcpyDeviceToHost));
+  HC(hipMemcpyAsync(h, d_scal_, sizeof h, hipMemcpyDeviceToHost, stream_));
+  HC(hipStreamSynchronize(stream_));
   if (h[S_ERR] & 1u) throw GlsError(GLS_E_DEVICE, "tile look-back did not complete (spin limit)");
   if (h[S_ERR] & 2u)
     throw GlsError(GLS_E_UNSUPPORTED, "a tile exceeded the overflow stack buffer (event density too skewed)");
@@ -442,13 +602,14 @@ void Job::run() {
   // algorithmic bytes: every input waveform read once per reading gate, every
   // output written once, 3 u64 SAIF counters per net, CSR metadata
   std::vector<uint32_t> wlen(n_nets_);
-  HC(hipMemcpy(wlen.data(), d_wlen_, n_nets_ * 4ull, hipMemcpyDeviceToHost));
+  HC(hipMemcpyAsync(wlen.data(), d_wlen_, n_nets_ * 4ull, hipMemcpyDeviceToHost, stream_));
+  HC(hipStreamSynchronize(stream_));
   uint64_t in_w = 0, out_w = 0;
-  for (uint32_t g = 0; g < n_gates_; ++g)
-    for (uint32_t n : gate_in_nets_[g]) in_w += wlen[n];
-  std::vector<GateDev> gd(n_gates_);
-  if (n_gates_) HC(hipMemcpy(gd.data(), d_gates_, n_gates_ * sizeof(GateDev), hipMemcpyDeviceToHost));
-  for (auto& g : gd) out_w += wlen[g.out_net];
+  for (const GateRec& g : prog_->h_grec) {
+    for (int j = 0; j < g.k; ++j) in_w += wlen[g.in_net[j]];
+    out_w += wlen[g.out_net];
+  }
+  if (prog_->h_grec.size() != n_gates_) in_w = out_w = 0;
   stats_.in_words = in_w;
   stats_.out_words = out_w;
   stats_.alg_bytes = 4 * (in_w + out_w) + 24ull * n_nets_ + 16ull * n_gates_ + 4ull * stats_.n_pins;
@@ -456,19 +617,28 @@ void Job::run() {
 
 void Job::fetch_saif(std::vector<uint64_t>& v) const {
   v.resize((size_t)n_nets_ * 3);
-  hip_check(hipMemcpy(v.data(), d_saif_, v.size() * 8, hipMemcpyDeviceToHost), "saif D2H");
+  hip_check(hipMemcpyAsync(v.data(), d_saif_, v.size() * 8, hipMemcpyDeviceToHost, stream_),
+            "saif D2H");
+  hip_check(hipStreamSynchronize(stream_), "saif D2H");
 }
 
 void Job::fetch_waves(std::vector<uint64_t>& off, std::vector<uint32_t>& len,
                       std::vector<uint32_t>& pool) const {
   off.resize(n_nets_);
   len.resize(n_nets_);
-  hip_check(hipMemcpy(off.data(), d_woff_, n_nets_ * 8ull, hipMemcpyDeviceToHost), "woff D2H");
-  hip_check(hipMemcpy(len.data(), d_wlen_, n_nets_ * 4ull, hipMemcpyDeviceToHost), "wlen D2H");
   uint64_t used = 0;
-  hip_check(hipMemcpy(&used, d_scal_, 8, hipMemcpyDeviceToHost), "bump D2H");
+  hip_check(hipMemcpyAsync(off.data(), d_woff_, n_nets_ * 8ull, hipMemcpyDeviceToHost, stream_),
+            "woff D2H");
+  hip_check(hipMemcpyAsync(len.data(), d_wlen_, n_nets_ * 4ull, hipMemcpyDeviceToHost, stream_),
+            "wlen D2H");
+  hip_check(hipMemcpyAsync(&used, d_scal_, 8, hipMemcpyDeviceToHost, stream_), "bump D2H");
+  hip_check(hipStreamSynchronize(stream_), "waves D2H");
   pool.resize(used);
-  if (used) hip_check(hipMemcpy(pool.data(), d_pool_, used * 4, hipMemcpyDeviceToHost), "pool D2H");
+  if (used) {
+    hip_check(hipMemcpyAsync(pool.data(), d_pool_, used * 4, hipMemcpyDeviceToHost, stream_),
+              "pool D2H");
+    hip_check(hipStreamSynchronize(stream_), "pool D2H");
+  }
 }
 
 }  // namespace gls

@@ -333,28 +333,38 @@ __global__ __launch_bounds__(256) void k_emit(
 }
 
 // Stimulus / constant waveforms: SAIF durations, one thread per net.
-__global__ void k_saif_nets(const uint32_t* __restrict__ nets, uint32_t count,
-                            const uint64_t* __restrict__ woff, const uint32_t* __restrict__ wlen,
-                            const uint32_t* __restrict__ pool, unsigned long long* __restrict__ saif,
-                            uint64_t start, uint64_t end) {
-  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+// SAIF durations of nets that no gate drives (stimulus signals, constants):
+// one wavefront per net, lane i takes entries i, i+64, ... (coalesced); entry
+// q holds its value until entry q+1 (the last one until the window end), and
+// the net is X before its first entry.  Window of net i: win[2*job[i]].
+__global__ void k_saif_nets(const uint32_t* __restrict__ nets, const uint32_t* __restrict__ job,
+                            uint32_t count, const uint64_t* __restrict__ woff,
+                            const uint32_t* __restrict__ wlen, const uint32_t* __restrict__ pool,
+                            unsigned long long* __restrict__ saif,
+                            const unsigned long long* __restrict__ win) {
+  const uint32_t i = blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64;
+  const int lane = threadIdx.x & 63;
   if (i >= count) return;
-  uint32_t net = nets[i];
+  const uint32_t net = nets[i];
+  const uint64_t start = win[2 * job[i]], end = win[2 * job[i] + 1];
   const uint32_t* w = pool + woff[net];
-  uint32_t n = wlen[net];
+  const uint32_t n = wlen[net];
   unsigned long long acc[3] = {0, 0, 0};
-  uint32_t pv = 1, pt = 0;  // X before the first entry
-  for (uint32_t q = 0; q < n; ++q) {
-    uint32_t e = w[q];
-    saif_add(acc, pv, pt, e >> 2, start, end);
-    pv = e & 3u;
-    pt = e >> 2;
+  if (lane == 0) saif_add(acc, 1u, 0, n ? (w[0] >> 2) : (end > 0 ? end : 0), start, end);
+  for (uint32_t q = lane; q < n; q += 64) {
+    const uint32_t e = w[q];
+    const uint64_t t = e >> 2;
+    const uint64_t tn = q + 1 < n ? (uint64_t)(w[q + 1] >> 2) : (end > t ? end : t);
+    saif_add(acc, e & 3u, t, tn, start, end);
   }
-  saif_add(acc, pv, pt, end > pt ? end : pt, start, end);
-  unsigned long long* d = saif + 3ull * net;
-  d[0] = acc[0];
-  d[1] = acc[1];
-  d[2] = acc[2];
+  for (int k = 0; k < 3; ++k)
+    for (int o = 32; o > 0; o >>= 1) acc[k] += __shfl_down(acc[k], o, 64);
+  if (lane == 0) {
+    unsigned long long* d = saif + 3ull * net;
+    d[0] = acc[0];
+    d[1] = acc[1];
+    d[2] = acc[2];
+  }
 }
 
 // ------------------------------------------------------------------ scan ----
@@ -522,12 +532,12 @@ hipError_t launch_bump(uint64_t* bump, const uint64_t* total, hipStream_t st) {
   return hipSuccess;
 }
 
-hipError_t launch_saif_nets(const uint32_t* nets, uint32_t count, const uint64_t* woff,
-                            const uint32_t* wlen, const uint32_t* pool, unsigned long long* saif,
-                            uint64_t start, uint64_t end, hipStream_t st) {
+hipError_t launch_saif_nets(const uint32_t* nets, const uint32_t* job, uint32_t count,
+                            const uint64_t* woff, const uint32_t* wlen, const uint32_t* pool,
+                            unsigned long long* saif, const unsigned long long* win,
+                            hipStream_t st) {
   if (count == 0) return hipSuccess;
-  k_saif_nets<<<blocks_for(count, 256), 256, 0, st>>>(nets, count, woff, wlen, pool, saif, start,
-                                                     end);
+  k_saif_nets<<<blocks_for(count, 4), 256, 0, st>>>(nets, job, count, woff, wlen, pool, saif, win);
   GLS_CHECK_LAUNCH();
   return hipSuccess;
 }

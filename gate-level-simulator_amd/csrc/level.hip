@@ -537,8 +537,9 @@ __device__ __forceinline__ void emit_tile(const LevelArgs& A, const TileDesc& D,
   const uint32_t e_first = T.e_first;
   const uint64_t gbase = A.out_base[D.g];
   const uint32_t tile_out0 = T.tile_out0;
-  const uint32_t s32 = (uint32_t)min(A.start, 0x7FFFFFFFull);
-  const uint32_t e32 = (uint32_t)min(A.end, 0x7FFFFFFFull);
+  const unsigned long long w_start = A.win[2 * D.job], w_end = A.win[2 * D.job + 1];
+  const uint32_t s32 = (uint32_t)min(w_start, 0x7FFFFFFFull);
+  const uint32_t e32 = (uint32_t)min(w_end, 0x7FFFFFFFull);
   uint32_t pvv = carry.has ? carry.lv : 4u;
   uint32_t pct = max(min(carry.has ? carry.lt : (e_first >> 2), e32), s32);
   const uint32_t ct0 = pct;
@@ -572,7 +573,7 @@ __device__ __forceinline__ void emit_tile(const LevelArgs& A, const TileDesc& D,
     if (ti + 1 == ntile_g) {  // last tile: waveform length and the tail duration
       A.wlen[D.out_net] = incl.cnt;
       if (incl.has)
-        saif_acc(acc, incl.lv, incl.lt, A.end > incl.lt ? A.end : incl.lt, A.start, A.end);
+        saif_acc(acc, incl.lv, incl.lt, w_end > incl.lt ? w_end : incl.lt, w_start, w_end);
     }
     if (ti == 0) A.woff[D.out_net] = gbase;
     for (int k = 0; k < 3; ++k) sacc[k] += acc[k];
@@ -754,7 +755,8 @@ __global__ void k_tile_desc(const GateRec* __restrict__ gates, uint32_t ng,
     D.lut = G.lut;
     D.dmax = G.dmax;
     D.out_net = G.out_net;
-    D.pad[0] = D.pad[1] = 0;
+    D.job = G.job;
+    D.pad = 0;
   }
   if (j >= G.k) {
     D.L[j] = 0;

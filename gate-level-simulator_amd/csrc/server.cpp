@@ -1,0 +1,271 @@
+// Serving loop (see server.h).
+#include "server.h"
+
+#include <algorithm>
+#include <cmath>
+
+#include "../../include/gls_status.h"
+
+namespace gls {
+
+using Clock = std::chrono::steady_clock;
+
+static double ms_between(Clock::time_point a, Clock::time_point b) {
+  return std::chrono::duration<double, std::milli>(b - a).count();
+}
+
+Server::Server(int device, const ServerOptions& o, ProgramCache* cache)
+    : device_(device), opt_(o), cache_(cache) {
+  hip_check(hipSetDevice(device_), "hipSetDevice");
+  if (opt_.device_budget == 0) {
+    size_t free_b = 0, total_b = 0;
+    hip_check(hipMemGetInfo(&free_b, &total_b), "hipMemGetInfo");
+    opt_.device_budget = (uint64_t)(free_b * 0.9);
+  }
+  if (opt_.max_running == 0) opt_.max_running = 1;
+  if (!(opt_.pool_factor > 0)) opt_.pool_factor = 1.0;
+  for (uint32_t i = 0; i < opt_.max_running; ++i) workers_.emplace_back(&Server::worker, this, (int)i);
+}
+
+Server::~Server() {
+  drain();
+  {
+    std::lock_guard<std::mutex> lk(mu_);
+    stop_ = true;
+  }
+  cv_work_.notify_all();
+  for (auto& t : workers_) t.join();
+}
+
+// Device bytes a job needs: its event pool (stimulus plus an estimate of the
+// per-level output reservations: each gate reserves its input event count),
+// per-net arrays, level scratch, overflow stacks, and the program image when
+// it is not resident yet.
+uint64_t Server::estimate(const Rec& r, bool* resident) const {
+  const RunInst& in = r.inst;
+  uint64_t stim = 0;
+  for (auto& w : in.st->waves) stim += w.size();
+  const uint64_t n_in = std::max<uint64_t>(1, in.st->waves.size());
+  const uint64_t gates = in.pg->gates.size(), nets = in.names.size();
+  const uint64_t per_input = stim / n_in + 1;
+  // internal nets toggle a few times more often than the inputs (glitches);
+  // a gate reserves about fan-in times its inputs' events
+  uint64_t pool = stim + (uint64_t)(opt_.pool_factor * 10.0 * (double)gates * (double)per_input) +
+                  (1u << 20);
+  const uint64_t ovf = std::min<uint64_t>(1ull << 24, std::max<uint64_t>(1ull << 18, pool / 16));
+  uint64_t bytes = pool * 4 + ovf * 4 + nets * 48 + gates * 64 + (64ull << 20);
+  *resident = cache_->resident(in.pg.get(), (uint32_t)in.timescale_ps);
+  if (!*resident) bytes += gates * (sizeof(GateRec) + sizeof(GateDev) + 512);
+  return bytes;
+}
+
+uint64_t Server::submit(RunInst inst, std::shared_ptr<const Stimulus> keep) {
+  auto r = std::make_unique<Rec>();
+  r->inst = std::move(inst);
+  r->keep = std::move(keep);
+  r->t_submit = Clock::now();
+  uint64_t id;
+  {
+    std::lock_guard<std::mutex> lk(mu_);
+    if (stop_) throw GlsError(GLS_E_STATE, "server is stopping");
+    if (!started_) {
+      t0_ = r->t_submit;
+      started_ = true;
+    }
+    id = next_id_++;
+    r->id = id;
+    jobs_[id] = std::move(r);
+    queue_.push_back(id);
+  }
+  cv_work_.notify_all();
+  return id;
+}
+
+void Server::worker(int slot) {
+  (void)slot;
+  hip_check(hipSetDevice(device_), "hipSetDevice");
+  hipStream_t stream = nullptr;
+  hip_check(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking), "hipStreamCreate");
+  for (;;) {
+    Rec* r = nullptr;
+    {
+      std::unique_lock<std::mutex> lk(mu_);
+      for (;;) {
+        if (stop_ && queue_.empty()) break;
+        // admission: the oldest queued job whose footprint fits what is free;
+        // a job may pass the head of the queue only while the head does not fit
+        for (auto it = queue_.begin(); it != queue_.end(); ++it) {
+          Rec* c = jobs_[*it].get();
+          bool resident = false;
+          const uint64_t need = c->reserve ? c->reserve : estimate(*c, &resident);
+          if (!c->reserve) {
+            c->reserve = need;
+            c->info.resident = resident ? 1u : 0u;
+          }
+          if (need > opt_.device_budget) {  // can never run: fail it
+            c->info.status = GLS_E_OOM;
+            c->t_end = Clock::now();
+            queue_.erase(it);
+            cv_done_.notify_all();
+            break;
+          }
+          if (in_use_ + need <= opt_.device_budget) {
+            r = c;
+            queue_.erase(it);
+            break;
+          }
+        }
+        if (r) break;
+        cv_work_.wait(lk);
+      }
+      if (!r) break;
+      in_use_ += r->reserve;
+      peak_bytes_ = std::max(peak_bytes_, in_use_);
+      ++running_;
+      peak_running_ = std::max(peak_running_, running_);
+      r->info.overlap = running_ - 1;
+      r->info.status = 2;
+      r->t_start = Clock::now();
+    }
+    int status = 0;
+    uint64_t events = 0, bytes = 0;
+    uint32_t attempts = 0;
+    std::vector<uint64_t> saif;
+    try {
+      const RunInst& in = r->inst;
+      SimInstance I;
+      I.pg = in.pg.get();
+      I.image = cache_->get(in.pg, (uint32_t)in.timescale_ps);
+      I.st = in.st;
+      I.stim_nets = in.stim_nets;
+      I.start = in.start;
+      I.end = in.end;
+      uint64_t stim = 0;
+      for (auto& w : in.st->waves) stim += w.size();
+      const uint64_t n_in = std::max<uint64_t>(1, in.st->waves.size());
+      uint64_t pool = stim + (uint64_t)(opt_.pool_factor * 10.0 * (double)in.pg->gates.size() *
+                                        (double)(stim / n_in + 1)) + (1u << 20);
+      for (uint32_t attempt = 0; attempt < 2; ++attempt) {
+        ++attempts;
+        try {
+          Job job(device_, stream);
+          job.seg_target = opt_.seg_events;
+          job.ovf_words_opt = std::min<uint64_t>(1ull << 24, std::max<uint64_t>(1ull << 18, pool / 16));
+          job.prepare({I}, pool);
+          job.run();
+          job.fetch_saif(saif);
+          events = job.stats().n_events;
+          bytes = job.device_bytes();
+          status = 0;
+          break;
+        } catch (const GlsError& e) {
+          status = e.code;
+          if (e.code != GLS_E_OOM || attempt == 1) break;
+          pool *= 2;  // the estimate was short: one retry with a larger pool
+          std::lock_guard<std::mutex> lk(mu_);
+          const uint64_t more = pool * 4 / 2;
+          if (in_use_ + more > opt_.device_budget) break;
+          in_use_ += more;
+          r->reserve += more;
+          peak_bytes_ = std::max(peak_bytes_, in_use_);
+        }
+      }
+    } catch (const GlsError& e) {
+      status = e.code;
+    } catch (const std::exception&) {
+      status = GLS_E_DEVICE;
+    }
+    {
+      std::lock_guard<std::mutex> lk(mu_);
+      r->t_end = Clock::now();
+      r->info.status = status;
+      r->info.n_events = events;
+      r->info.device_bytes = bytes;
+      r->info.attempts = attempts;
+      r->info.queue_ms = ms_between(r->t_submit, r->t_start);
+      r->info.run_ms = ms_between(r->t_start, r->t_end);
+      r->info.latency_ms = ms_between(r->t_submit, r->t_end);
+      r->saif = std::move(saif);
+      in_use_ -= r->reserve;
+      --running_;
+    }
+    cv_done_.notify_all();
+    cv_work_.notify_all();
+  }
+  (void)hipStreamDestroy(stream);
+}
+
+int Server::wait(uint64_t id, double timeout_s) {
+  std::unique_lock<std::mutex> lk(mu_);
+  auto it = jobs_.find(id);
+  if (it == jobs_.end()) throw GlsError(GLS_E_ARG, "unknown job");
+  Rec* r = it->second.get();
+  auto done = [&] { return r->info.status <= 0; };
+  if (timeout_s < 0) cv_done_.wait(lk, done);
+  else cv_done_.wait_for(lk, std::chrono::duration<double>(timeout_s), done);
+  return r->info.status;
+}
+
+ServerJobInfo Server::info(uint64_t id) {
+  std::lock_guard<std::mutex> lk(mu_);
+  auto it = jobs_.find(id);
+  if (it == jobs_.end()) throw GlsError(GLS_E_ARG, "unknown job");
+  return it->second->info;
+}
+
+std::string Server::saif(uint64_t id) {
+  std::lock_guard<std::mutex> lk(mu_);
+  auto it = jobs_.find(id);
+  if (it == jobs_.end()) throw GlsError(GLS_E_ARG, "unknown job");
+  Rec* r = it->second.get();
+  if (r->info.status != 0) throw GlsError(GLS_E_STATE, "job not finished successfully");
+  return inst_saif(r->inst, r->saif, 0);
+}
+
+void Server::drain() {
+  std::unique_lock<std::mutex> lk(mu_);
+  cv_done_.wait(lk, [&] {
+    if (!queue_.empty() || running_) return false;
+    return true;
+  });
+}
+
+ServerStats Server::stats() {
+  std::lock_guard<std::mutex> lk(mu_);
+  ServerStats s;
+  std::vector<double> lat;
+  Clock::time_point last = t0_;
+  for (auto& kv : jobs_) {
+    const Rec& r = *kv.second;
+    ++s.submitted;
+    if (r.info.status == 0) {
+      ++s.done;
+      s.events += r.info.n_events;
+      lat.push_back(r.info.latency_ms);
+      last = std::max(last, r.t_end);
+    } else if (r.info.status < 0) {
+      ++s.failed;
+    }
+  }
+  if (!lat.empty()) {
+    std::sort(lat.begin(), lat.end());
+    auto pct = [&](double p) {
+      // nearest rank
+      size_t k = (size_t)std::ceil(p / 100.0 * (double)lat.size());
+      return lat[std::min(lat.size() - 1, k ? k - 1 : 0)];
+    };
+    s.p50_ms = pct(50);
+    s.p99_ms = pct(99);
+    double sum = 0;
+    for (double x : lat) sum += x;
+    s.mean_ms = sum / (double)lat.size();
+  }
+  s.wall_s = started_ ? std::chrono::duration<double>(last - t0_).count() : 0;
+  s.events_per_s = s.wall_s > 0 ? (double)s.events / s.wall_s : 0;
+  s.peak_running = peak_running_;
+  s.peak_bytes = peak_bytes_;
+  s.resident_programs = (uint32_t)cache_->size();
+  return s;
+}
+
+}  // namespace gls

@@ -1,0 +1,103 @@
+// Serving loop: a request queue in front of one device.  Jobs (netlist +
+// stimulus + SAIF window) are admitted by their device-memory footprint as
+// memory and slots free up, each admitted job runs on its own HIP stream so
+// the kernels of concurrent jobs overlap on the device, and compiled
+// netlists stay resident for later jobs.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <atomic>
+#include <condition_variable>
+#include <cstdint>
+#include <deque>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+#include "engine.h"
+
+namespace gls {
+
+// One simulation as the C-ABI sees it: program, names for the SAIF writer,
+// stimulus, window.
+struct RunInst {
+  std::shared_ptr<const Program> pg;
+  std::vector<int> stim_nets;
+  std::vector<std::string> names;  // all net names incl. stimulus-only signals
+  std::vector<uint8_t> in_saif;
+  std::vector<uint8_t> is_gate;
+  std::unordered_map<std::string, int> by_name;
+  const Stimulus* st = nullptr;
+  uint64_t start = 0, end = 0;
+  int timescale_ps = 1;
+};
+
+std::string inst_saif(const RunInst& in, const std::vector<uint64_t>& saif, uint32_t base);
+
+struct ServerOptions {
+  uint64_t device_budget = 0;  // bytes of HBM the server may use (0: 90% of free)
+  uint32_t max_running = 4;    // jobs on the device at once
+  uint32_t seg_events = 0;     // fused-kernel segment size (0: default)
+  double pool_factor = 1.0;    // scales the event-pool estimate of a job
+};
+
+struct ServerJobInfo {
+  int status = 1;              // 1 queued, 2 running, 0 done, < 0 GLS_E_* failure
+  double queue_ms = 0, run_ms = 0, latency_ms = 0;
+  uint64_t n_events = 0, device_bytes = 0;
+  uint32_t attempts = 0, resident = 0, overlap = 0;
+};
+
+struct ServerStats {
+  uint64_t submitted = 0, done = 0, failed = 0, events = 0;
+  double p50_ms = 0, p99_ms = 0, mean_ms = 0, wall_s = 0, events_per_s = 0;
+  uint32_t peak_running = 0, resident_programs = 0;
+  uint64_t peak_bytes = 0;
+};
+
+class Server {
+ public:
+  Server(int device, const ServerOptions& o, ProgramCache* cache);
+  ~Server();  // finishes every submitted job first
+  uint64_t submit(RunInst inst, std::shared_ptr<const Stimulus> keep);
+  // Wait for a job (timeout_s < 0: forever).  Returns its status (1/2 when
+  // still queued/running at the timeout).
+  int wait(uint64_t id, double timeout_s);
+  ServerJobInfo info(uint64_t id);
+  std::string saif(uint64_t id);
+  ServerStats stats();
+  void drain();
+
+ private:
+  struct Rec {
+    uint64_t id = 0;
+    RunInst inst;
+    std::shared_ptr<const Stimulus> keep;
+    std::shared_ptr<DevProgram> image;
+    ServerJobInfo info;
+    uint64_t pool_words = 0, reserve = 0;
+    std::vector<uint64_t> saif;
+    std::chrono::steady_clock::time_point t_submit, t_start, t_end;
+  };
+  void worker(int slot);
+  uint64_t estimate(const Rec& r, bool* resident) const;
+  int device_;
+  ServerOptions opt_;
+  ProgramCache* cache_;
+  std::mutex mu_;
+  std::condition_variable cv_work_, cv_done_;
+  std::deque<uint64_t> queue_;
+  std::map<uint64_t, std::unique_ptr<Rec>> jobs_;
+  uint64_t next_id_ = 1, in_use_ = 0, peak_bytes_ = 0;
+  uint32_t running_ = 0, peak_running_ = 0;
+  bool stop_ = false;
+  std::vector<std::thread> workers_;
+  std::chrono::steady_clock::time_point t0_;
+  bool started_ = false;
+};
+
+}  // namespace gls

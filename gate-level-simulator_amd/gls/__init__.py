@@ -51,6 +51,41 @@ class RunStats(ctypes.Structure):
                 ("eval_launches", ctypes.c_uint64), ("level_blocks", ctypes.c_uint64)]
 
 
+TUNE_MAX = 16
+
+
+class TuneReport(ctypes.Structure):
+    _fields_ = [("seg_events", ctypes.c_uint32), ("level_blocks", ctypes.c_uint32),
+                ("prefetch", ctypes.c_uint32), ("n_tried", ctypes.c_uint32),
+                ("best_ms", ctypes.c_double),
+                ("tried_seg", ctypes.c_uint32 * TUNE_MAX),
+                ("tried_blocks", ctypes.c_uint32 * TUNE_MAX),
+                ("tried_prefetch", ctypes.c_uint32 * TUNE_MAX),
+                ("tried_ms", ctypes.c_double * TUNE_MAX)]
+
+
+class ServerOpts(ctypes.Structure):
+    _fields_ = [("device_budget", ctypes.c_uint64), ("max_running", ctypes.c_uint32),
+                ("seg_events", ctypes.c_uint32), ("pool_factor", ctypes.c_double)]
+
+
+class JobInfo(ctypes.Structure):
+    _fields_ = [("status", ctypes.c_int), ("queue_ms", ctypes.c_double),
+                ("run_ms", ctypes.c_double), ("latency_ms", ctypes.c_double),
+                ("n_events", ctypes.c_uint64), ("device_bytes", ctypes.c_uint64),
+                ("attempts", ctypes.c_uint32), ("resident", ctypes.c_uint32),
+                ("overlap", ctypes.c_uint32)]
+
+
+class ServerStats(ctypes.Structure):
+    _fields_ = [("submitted", ctypes.c_uint64), ("done", ctypes.c_uint64),
+                ("failed", ctypes.c_uint64), ("events", ctypes.c_uint64),
+                ("p50_ms", ctypes.c_double), ("p99_ms", ctypes.c_double),
+                ("mean_ms", ctypes.c_double), ("wall_s", ctypes.c_double),
+                ("events_per_s", ctypes.c_double), ("peak_running", ctypes.c_uint32),
+                ("resident_programs", ctypes.c_uint32), ("peak_bytes", ctypes.c_uint64)]
+
+
 # Every symbol include/gls.h declares: (name, restype, argtypes)
 _P = ctypes.c_void_p
 _PP = ctypes.POINTER(ctypes.c_void_p)
@@ -86,6 +121,22 @@ SIGNATURES = [
                               ctypes.POINTER(ctypes.c_size_t)]),
     ("gls_run_waves_hash", _I, [_P, ctypes.POINTER(_U64)]),
     ("gls_run_free", None, [_P]),
+    ("gls_prepare_batch", _I, [_P, ctypes.c_size_t, _PP, _PP, ctypes.POINTER(_U64),
+                               ctypes.POINTER(_U64), ctypes.POINTER(SimOpts), _PP]),
+    ("gls_run_instances", _I, [_P, ctypes.POINTER(ctypes.c_size_t)]),
+    ("gls_run_instance_saif_text", _I, [_P, ctypes.c_size_t, ctypes.c_char_p, ctypes.c_size_t,
+                                        ctypes.POINTER(ctypes.c_size_t)]),
+    ("gls_run_instance_waves_hash", _I, [_P, ctypes.c_size_t, ctypes.POINTER(_U64)]),
+    ("gls_run_autotune", _I, [_P, ctypes.POINTER(TuneReport)]),
+    ("gls_server_create", _I, [_P, ctypes.POINTER(ServerOpts), _PP]),
+    ("gls_server_submit", _I, [_P, _P, _P, _U64, _U64, ctypes.POINTER(_U64)]),
+    ("gls_server_wait", _I, [_P, _U64, ctypes.c_double, ctypes.POINTER(ctypes.c_int)]),
+    ("gls_server_job_info", _I, [_P, _U64, ctypes.POINTER(JobInfo)]),
+    ("gls_server_job_saif_text", _I, [_P, _U64, ctypes.c_char_p, ctypes.c_size_t,
+                                      ctypes.POINTER(ctypes.c_size_t)]),
+    ("gls_server_drain", _I, [_P]),
+    ("gls_server_stats_get", _I, [_P, ctypes.POINTER(ServerStats)]),
+    ("gls_server_destroy", None, [_P]),
 ]
 
 _lib = None
@@ -250,6 +301,75 @@ class Run(_Handle):
         _check(load().gls_run_waves_hash(self.ptr, ctypes.byref(h)))
         return h.value
 
+    def instances(self):
+        n = ctypes.c_size_t()
+        _check(load().gls_run_instances(self.ptr, ctypes.byref(n)))
+        return n.value
+
+    def instance_saif_text(self, i):
+        n = ctypes.c_size_t()
+        _check(load().gls_run_instance_saif_text(self.ptr, i, None, 0, ctypes.byref(n)))
+        buf = ctypes.create_string_buffer(n.value)
+        _check(load().gls_run_instance_saif_text(self.ptr, i, buf, n.value, ctypes.byref(n)))
+        return buf.raw[:n.value].decode()
+
+    def instance_waves_hash(self, i):
+        h = _U64()
+        _check(load().gls_run_instance_waves_hash(self.ptr, i, ctypes.byref(h)))
+        return h.value
+
+    def autotune(self):
+        r = TuneReport()
+        _check(load().gls_run_autotune(self.ptr, ctypes.byref(r)))
+        tried = [dict(seg_events=r.tried_seg[i], level_blocks=r.tried_blocks[i],
+                      prefetch=r.tried_prefetch[i], ms=r.tried_ms[i]) for i in range(r.n_tried)]
+        return dict(seg_events=r.seg_events, level_blocks=r.level_blocks, prefetch=r.prefetch,
+                    best_ms=r.best_ms, tried=tried)
+
+
+class Server(_Handle):
+    """Serving loop over one device (see gls_server_* in include/gls.h)."""
+    _free = "gls_server_destroy"
+
+    def __init__(self, ctx, device_budget=0, max_running=4, seg_events=0, pool_factor=1.0):
+        h = ctypes.c_void_p()
+        o = ServerOpts(device_budget, max_running, seg_events, pool_factor)
+        _check(load().gls_server_create(ctx.ptr, ctypes.byref(o), ctypes.byref(h)))
+        self._keep = [ctx]
+        super().__init__(h.value)
+
+    def submit(self, design, stim, dumpon, dumpoff):
+        j = _U64()
+        _check(load().gls_server_submit(self.ptr, design.ptr, stim.ptr, dumpon, dumpoff,
+                                        ctypes.byref(j)))
+        self._keep.append(design)
+        return j.value
+
+    def wait(self, job, timeout_s=-1.0):
+        st = ctypes.c_int()
+        _check(load().gls_server_wait(self.ptr, job, timeout_s, ctypes.byref(st)))
+        return st.value
+
+    def info(self, job):
+        i = JobInfo()
+        _check(load().gls_server_job_info(self.ptr, job, ctypes.byref(i)))
+        return {n: getattr(i, n) for n, _ in JobInfo._fields_}
+
+    def saif_text(self, job):
+        n = ctypes.c_size_t()
+        _check(load().gls_server_job_saif_text(self.ptr, job, None, 0, ctypes.byref(n)))
+        buf = ctypes.create_string_buffer(n.value)
+        _check(load().gls_server_job_saif_text(self.ptr, job, buf, n.value, ctypes.byref(n)))
+        return buf.raw[:n.value].decode()
+
+    def drain(self):
+        _check(load().gls_server_drain(self.ptr))
+
+    def stats(self):
+        s = ServerStats()
+        _check(load().gls_server_stats_get(self.ptr, ctypes.byref(s)))
+        return {n: getattr(s, n) for n, _ in ServerStats._fields_}
+
 
 class Context(_Handle):
     _free = "gls_ctx_destroy"
@@ -272,6 +392,21 @@ class Context(_Handle):
                        prefetch)
         _check(load().gls_prepare(self.ptr, design.ptr, stim.ptr, dumpon, dumpoff, ctypes.byref(o),
                                   ctypes.byref(h)))
+        return Run(h.value)
+
+    def prepare_batch(self, designs, stims, dumpons, dumpoffs, pool_bytes=0, seg_events=0,
+                      keep_waves=False, level_blocks=0, tiles_per_ticket=0, prefetch=True):
+        n = len(designs)
+        assert len(stims) == n and len(dumpons) == n and len(dumpoffs) == n
+        D = (ctypes.c_void_p * n)(*[d.ptr for d in designs])
+        S = (ctypes.c_void_p * n)(*[s.ptr for s in stims])
+        on = (_U64 * n)(*dumpons)
+        off = (_U64 * n)(*dumpoffs)
+        h = ctypes.c_void_p()
+        o = self._opts(pool_bytes, seg_events, keep_waves, PATH_FUSED, level_blocks,
+                       tiles_per_ticket, prefetch)
+        _check(load().gls_prepare_batch(self.ptr, n, D, S, on, off, ctypes.byref(o),
+                                        ctypes.byref(h)))
         return Run(h.value)
 
     def simulate(self, design, stim, dumpon, dumpoff, pool_bytes=0, seg_events=0, keep_waves=False,

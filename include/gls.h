@@ -132,6 +132,75 @@ int gls_run_waveform(const gls_run* r, const char* net, uint32_t* words, size_t 
 int gls_run_waves_hash(const gls_run* r, uint64_t* hash);
 void gls_run_free(gls_run* r);
 
+/* ------------------------------------------------------------ batches --- */
+/* Batched form (no counterpart in the reference, which simulates one
+ * stimulus per process): n instances — netlists of any size and depth, each
+ * with its own stimulus and SAIF window — prepared together and evaluated
+ * level by level by the same kernel launches (level l of every instance in
+ * one launch).  Results per instance are bit-identical to a plain run of that
+ * instance.  Fused path only when n > 1.  The single-run accessors above
+ * address instance 0. */
+int gls_prepare_batch(gls_ctx* c, size_t n, const gls_design* const* designs,
+                      const gls_stim* const* stims, const uint64_t* dumpon,
+                      const uint64_t* dumpoff, const gls_sim_opts* opts, gls_run** out);
+int gls_run_instances(const gls_run* r, size_t* n);
+int gls_run_instance_saif_text(const gls_run* r, size_t i, char* buf, size_t cap, size_t* len);
+int gls_run_instance_waves_hash(const gls_run* r, size_t i, uint64_t* hash);
+
+/* Autotuner: executes the prepared run once per candidate of the fused
+ * kernel's events per segment (LDS tile size), workgroups per launch and
+ * tile staging, keeps the fastest for later executions, and leaves the run
+ * executed.  The report lists every trial. */
+#define GLS_TUNE_MAX 16
+typedef struct {
+  uint32_t seg_events, level_blocks, prefetch, n_tried;
+  double best_ms;
+  uint32_t tried_seg[GLS_TUNE_MAX], tried_blocks[GLS_TUNE_MAX], tried_prefetch[GLS_TUNE_MAX];
+  double tried_ms[GLS_TUNE_MAX];
+} gls_tune_report;
+int gls_run_autotune(gls_run* r, gls_tune_report* rep);
+
+/* ------------------------------------------------------------ serving --- */
+/* Serving loop (no counterpart in the reference): a request queue in front
+ * of one device.  Jobs are admitted by their estimated device-memory
+ * footprint as memory and slots free up (continuous batching: a job that
+ * arrives while others run starts as soon as it fits), each running job has
+ * its own stream so concurrent jobs overlap on the device, and compiled
+ * netlists stay resident in HBM for later jobs.  Submit and wait are
+ * thread-safe. */
+typedef struct gls_server gls_server;
+typedef struct {
+  uint64_t device_budget; /* HBM bytes the server may use (0: 90% of free) */
+  uint32_t max_running;   /* jobs on the device at once (0: 4) */
+  uint32_t seg_events;    /* fused-kernel events per segment (0: default) */
+  double pool_factor;     /* scales a job's event-pool estimate (0: 1) */
+} gls_server_opts;
+typedef struct {
+  int status;             /* 1 queued, 2 running, 0 done, < 0 GLS_E_* failure */
+  double queue_ms, run_ms, latency_ms;
+  uint64_t n_events, device_bytes;
+  uint32_t attempts, resident, overlap; /* resident: netlist was already in HBM;
+                                           overlap: jobs running when it started */
+} gls_job_info;
+typedef struct {
+  uint64_t submitted, done, failed, events;
+  double p50_ms, p99_ms, mean_ms, wall_s, events_per_s;
+  uint32_t peak_running, resident_programs;
+  uint64_t peak_bytes;
+} gls_server_stats;
+int gls_server_create(gls_ctx* c, const gls_server_opts* o, gls_server** out);
+/* The server copies the stimulus; the design must outlive the server. */
+int gls_server_submit(gls_server* s, const gls_design* d, const gls_stim* st, uint64_t dumpon,
+                      uint64_t dumpoff, uint64_t* job);
+/* timeout_s < 0 waits forever; *status as in gls_job_info. */
+int gls_server_wait(gls_server* s, uint64_t job, double timeout_s, int* status);
+int gls_server_job_info(gls_server* s, uint64_t job, gls_job_info* info);
+int gls_server_job_saif_text(gls_server* s, uint64_t job, char* buf, size_t cap, size_t* len);
+int gls_server_drain(gls_server* s);
+int gls_server_stats_get(gls_server* s, gls_server_stats* st);
+/* Finishes every submitted job, then frees the server. */
+void gls_server_destroy(gls_server* s);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,149 @@
+"""Batched form, autotuner and serving loop (through the C ABI).
+
+Bar: every instance of a batch and every served job produces exactly the
+waveforms and SAIF text of its own plain run, and the plain runs are pinned
+to the oracle (golden digests here, live oracle runs in test_gpu_parity.py).
+"""
+import json
+import os
+import random
+import sys
+
+import pytest
+
+import gls
+import golden_cases
+import oracle_bind
+
+pytestmark = pytest.mark.gpu
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+GOLDEN = json.load(open(os.path.join(ROOT, "tests", "golden", "oracle_golden.json")))
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = gls.Context(0)
+    yield c
+    c.close()
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def mixed_instances():
+    """Golden workloads plus netlists of other sizes and depths, stimuli with
+    different gaps and windows; one netlist appears with two stimuli."""
+    insts = []
+    for name in ("c0_small", "c1_small"):
+        d, s = golden_cases.make(name)
+        insts.append((d, s) + tuple(golden_cases.CASES[name]["window"]) + (name,))
+    rng = random.Random(7)
+    for k in range(4):
+        cfg = k % 2
+        d = gls.Design.generate(cfg, 900 + k, rng.randint(80, 900), rng.randint(3, 30),
+                                rng.randint(8, 64))
+        for rep in range(1 + (k == 1)):
+            t_end = rng.randint(5000, 80000)
+            s = gls.Stimulus.generate(d, 40 + 10 * k + rep, t_end, rng.choice([20.0, 150.0, 600.0]),
+                                      0.03)
+            start = rng.randint(0, t_end // 4)
+            insts.append((d, s, start, t_end + rng.randint(1, 300), None))
+    return insts
+
+
+def test_batch_matches_plain_runs(ctx):
+    insts = mixed_instances()
+    say("batch", len(insts), "instances")
+    run = ctx.prepare_batch([i[0] for i in insts], [i[1] for i in insts], [i[2] for i in insts],
+                            [i[3] for i in insts], keep_waves=True).execute()
+    assert run.instances() == len(insts)
+    for k, (d, s, on, off, name) in enumerate(insts):
+        say("instance", k, name)
+        plain = ctx.simulate(d, s, on, off, keep_waves=True)
+        assert run.instance_saif_text(k) == plain.saif_text(), k
+        assert run.instance_waves_hash(k) == plain.waves_hash(), k
+        if name:
+            g = GOLDEN[name]
+            assert "%016x" % run.instance_waves_hash(k) == g["waves_hash"]
+            assert oracle_bind.sha256(run.instance_saif_text(k)) == g["saif_sha256"]
+    # every instance's events are counted once
+    total = sum(ctx.simulate(d, s, on, off).stats()["n_events"] for d, s, on, off, _ in insts)
+    assert run.stats()["n_events"] == total
+
+
+def test_batch_segment_and_ticket_variants(ctx):
+    insts = mixed_instances()[:4]
+    args = ([i[0] for i in insts], [i[1] for i in insts], [i[2] for i in insts],
+            [i[3] for i in insts])
+    ref = ctx.prepare_batch(*args, keep_waves=True).execute()
+    for seg, tpt, pf in ((2, 1, True), (48, 3, False), (1 << 20, 1, True)):
+        say("batch variant", seg, tpt, pf)
+        r = ctx.prepare_batch(*args, keep_waves=True, seg_events=seg, tiles_per_ticket=tpt,
+                              prefetch=pf).execute()
+        for k in range(len(insts)):
+            assert r.instance_waves_hash(k) == ref.instance_waves_hash(k), (seg, k)
+            assert r.instance_saif_text(k) == ref.instance_saif_text(k), (seg, k)
+
+
+def test_autotune_keeps_results(ctx):
+    d, s = golden_cases.make("c1_small")
+    on, off = golden_cases.CASES["c1_small"]["window"]
+    run = ctx.prepare(d, s, on, off, keep_waves=True)
+    rep = run.autotune()
+    say("autotune", rep["seg_events"], rep["level_blocks"], rep["prefetch"], rep["best_ms"])
+    assert rep["tried"] and rep["seg_events"] in {t["seg_events"] for t in rep["tried"]}
+    g = GOLDEN["c1_small"]
+    assert "%016x" % run.waves_hash() == g["waves_hash"]
+    assert oracle_bind.sha256(run.saif_text()) == g["saif_sha256"]
+    run.execute()  # later executions use the tuned setting
+    assert "%016x" % run.waves_hash() == g["waves_hash"]
+
+
+def test_two_pass_rejects_batches(ctx):
+    insts = mixed_instances()[:2]
+    with pytest.raises(gls.GlsError) as e:
+        o = gls.Context._opts(0, 0, False, gls.PATH_TWO_PASS)
+        import ctypes
+        n = 2
+        D = (ctypes.c_void_p * n)(*[i[0].ptr for i in insts])
+        S = (ctypes.c_void_p * n)(*[i[1].ptr for i in insts])
+        on = (ctypes.c_uint64 * n)(*[i[2] for i in insts])
+        off = (ctypes.c_uint64 * n)(*[i[3] for i in insts])
+        h = ctypes.c_void_p()
+        gls._check(gls.load().gls_prepare_batch(ctx.ptr, n, D, S, on, off, ctypes.byref(o),
+                                                ctypes.byref(h)))
+    assert e.value.status == "GLS_E_UNSUPPORTED"
+
+
+def test_server_results_same_alone_and_in_mix(ctx):
+    insts = mixed_instances()
+    # alone: one job at a time through the plain path
+    alone = [ctx.simulate(d, s, on, off).saif_text() for d, s, on, off, _ in insts]
+    srv = gls.Server(ctx, max_running=4)
+    jobs = []
+    for rep in range(2):  # the second round reuses resident netlists
+        for d, s, on, off, _ in insts:
+            jobs.append(srv.submit(d, s, on, off))
+    for k, j in enumerate(jobs):
+        st = srv.wait(j)
+        assert st == 0, (k, st)
+        assert srv.saif_text(j) == alone[k % len(insts)], k
+    stats = srv.stats()
+    say("server", stats)
+    assert stats["done"] == len(jobs) and stats["failed"] == 0
+    assert stats["p99_ms"] >= stats["p50_ms"] > 0
+    assert stats["peak_running"] >= 2  # jobs overlapped on the device
+    infos = [srv.info(j) for j in jobs]
+    assert any(i["resident"] for i in infos[len(insts):])
+    assert any(i["overlap"] for i in infos)
+    srv.close()
+
+
+def test_server_rejects_jobs_over_budget(ctx):
+    d, s = golden_cases.make("c0_small")
+    srv = gls.Server(ctx, device_budget=1 << 20, max_running=1)
+    j = srv.submit(d, s, 0, 200001)
+    assert srv.wait(j) == -3  # GLS_E_OOM: the estimate can never fit
+    srv.close()
