@@ -124,14 +124,16 @@ def batched_vs_plain(ctx, n_sets, scale):
             log(f"bench: plain instance {k + 1}/{len(insts)} done")
     b = ctx.prepare_batch([i[0] for i in insts], [i[1] for i in insts], [0] * len(insts),
                           [2000001] * len(insts),
-                          pool_bytes=sum(pool_bytes_for(d, s) for d, s in insts))
+                          pool_bytes=sum(pool_bytes_for(d, s) for d, s in insts), graph=True)
     log("bench: batch prepared, autotuning")
     tune = b.autotune()
     log(f"bench: autotune picked {tune['seg_events']} events/segment, "
         f"{tune['level_blocks']} workgroups, prefetch {tune['prefetch']}")
+    b.execute()  # eager run with the tuned setting, captured as one graph
     t0 = time.perf_counter()
-    b.execute()
+    b.execute()  # graph replay: every level of every instance, no host round trip
     batch_s = time.perf_counter() - t0
+    replayed = b.stats()["eval_launches"] == 1
     st = b.stats()
     same = all(b.instance_saif_text(k) == saifs[k] for k in range(len(insts)))
     b.close()
@@ -141,6 +143,7 @@ def batched_vs_plain(ctx, n_sets, scale):
             "plain_events_per_s": round(plain_ev / plain_s, 1), "plain_s": round(plain_s, 4),
             "batched_events_per_s": round(st["n_events"] / batch_s, 1), "batched_s": round(batch_s, 4),
             "speedup": round(plain_s / batch_s, 3), "saif_bit_exact_vs_plain": same,
+            "form": "one captured hipGraph of all levels" if replayed else "level-synchronous launches",
             "autotune": {k: tune[k] for k in ("seg_events", "level_blocks", "prefetch", "best_ms")}}
 
 
@@ -219,6 +222,8 @@ def main():
     ap.add_argument("--no-prefetch", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-events", type=int, default=700000)
+    ap.add_argument("--graph-steps", type=int, default=2,
+                    help="also time the default workload as a captured graph (0: skip)")
     ap.add_argument("--batch-sets", type=int, default=32,
                     help="stimulus sets of the batched-vs-plain measurement (0: skip)")
     ap.add_argument("--batch-scale", type=float, default=0.1)
@@ -323,6 +328,25 @@ def main():
         "saif_durations_consistent": ok,
         "cpu_baseline": cpu,
     }
+    # the same workload as one captured graph (no host round trip per level)
+    if args.graph_steps:
+        run.close()
+        g = ctx.prepare(d, s, 0, args.t_end + 1, seg_events=args.seg_events,
+                        level_blocks=args.level_blocks, tiles_per_ticket=args.tiles_per_ticket,
+                        prefetch=not args.no_prefetch, graph=True)
+        g.execute()  # eager run, captured
+        gw = []
+        for i in range(args.graph_steps):
+            w0 = time.perf_counter()
+            g.execute()
+            gw.append(time.perf_counter() - w0)
+        gst = g.stats()
+        same = g.saif_text() == saif
+        out["graph"] = {"ms_per_step": round(sum(gw) / len(gw) * 1e3, 3),
+                        "events_per_s": round(gst["n_events"] / (sum(gw) / len(gw)), 1),
+                        "replayed": gst["eval_launches"] == 1, "saif_same_as_plain": same}
+        log(f"bench: graph replay {out['graph']['ms_per_step']} ms per step")
+        run = g
     run.close()
     if args.batch_sets:
         t0 = time.time()

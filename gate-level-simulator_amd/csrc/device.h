@@ -77,7 +77,7 @@ struct LevelArgs {
   const uint32_t* lutp;       // truth tables packed 2 bits per entry, 16 words each
   const uint16_t* dtab;       // delay tables (GateRec::dt_off)
   const uint64_t* out_base;   // per gate: output region (pool word offset)
-  uint64_t ntiles;
+  const uint64_t* ntiles;     // tiles of the level (device memory)
   uint64_t* woff;
   uint32_t* wlen;
   uint32_t* pool;

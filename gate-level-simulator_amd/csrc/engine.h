@@ -113,11 +113,17 @@ class Job {
   uint32_t tiles_per_ticket = 0;  // fused tiles handed out per ticket (0: one)
   uint32_t prefetch = 1;          // stage the next tile before storing the current one
   uint64_t ovf_words_opt = 0;     // overflow stack words (0: 2^24)
+  // 1: after a plain execution, capture the whole level sequence (no host
+  // round trips) into a hipGraph and replay it on later executions
+  uint32_t graph = 0;
+  bool graph_replayed() const { return last_replay_; }
   static constexpr uint32_t kDefaultSeg = 32;  // input events per time segment
 
  private:
   void ensure(void** p, size_t& cap, size_t need);
-  void run_fused(std::vector<hipEvent_t>& ev_eval);
+  void run_fused(std::vector<hipEvent_t>& ev_eval, bool device_driven = false);
+  void build_graph();
+  void finish_stats(const uint64_t* scal, double ms);
   void run_two_pass(std::vector<hipEvent_t>& ev_eval);
   void* dmalloc(size_t bytes);
   int device_;
@@ -158,6 +164,12 @@ class Job {
   int level_blocks_ = 0;
   uint32_t cap_ = 0;  // LDS input words per fused tile
   std::vector<void*> owned_;
+  uint64_t* d_woff_init_ = nullptr;
+  uint32_t* d_wlen_init_ = nullptr;
+  uint64_t max_tiles_ = 0, tiles_cap_ = 0;
+  hipGraphExec_t gexec_ = nullptr;
+  std::vector<uint64_t> graph_sig_;
+  bool last_replay_ = false;
   std::vector<uint64_t> h_woff_init_;
   std::vector<uint32_t> h_wlen_init_;
   JobStats stats_;

@@ -43,8 +43,12 @@ hipError_t launch_plan2(const GateRec* gates, uint32_t ng, const uint32_t* wlen,
 hipError_t launch_tile_desc(const GateRec* gates, uint32_t ng, const GatePlan* plan,
                             const uint64_t* tile_base, const uint64_t* woff, const uint32_t* wlen,
                             const uint32_t* pool, const uint32_t* lutp, TileDesc* tiles,
-                            uint64_t ntiles, hipStream_t st);
+                            const uint64_t* ntiles_dev, uint64_t ntiles, hipStream_t st);
 hipError_t launch_fill_u64(unsigned long long* p, uint64_t n, unsigned long long v, hipStream_t st);
+hipError_t launch_fill_u64_dev(unsigned long long* p, const uint64_t* n, uint64_t mult, uint64_t cap,
+                               unsigned long long v, hipStream_t st);
+hipError_t launch_guard(uint64_t* scal, uint64_t tiles_cap, uint64_t pool_words, hipStream_t st);
+hipError_t launch_set_scal(uint64_t* scal, uint64_t bump, hipStream_t st);
 hipError_t launch_level(const LevelArgs& a, unsigned nblocks, hipStream_t st);
 int level_max_blocks(uint32_t cap);
 
@@ -293,6 +297,7 @@ static std::shared_ptr<DevProgram> union_image(const std::vector<std::shared_ptr
 Job::Job(int device, hipStream_t stream) : device_(device), stream_(stream) {}
 
 Job::~Job() {
+  if (gexec_) (void)hipGraphExecDestroy(gexec_);
   for (void* p : owned_)
     if (p) (void)hipFree(p);
   void* grown[] = {d_segmap_, d_sums_, d_carry_, d_scratch_, d_tiles_, d_tstate_};
@@ -415,6 +420,11 @@ void Job::prepare(const std::vector<SimInstance>& inst, uint64_t pool_words) {
 
   if (!words.empty())
     HC(hipMemcpyAsync(d_pool_, words.data(), words.size() * 4, hipMemcpyHostToDevice, stream_));
+  // device copies of the initial waveform tables (graph replays start from them)
+  d_woff_init_ = (uint64_t*)dmalloc(n_nets_ * sizeof(uint64_t));
+  d_wlen_init_ = (uint32_t*)dmalloc(n_nets_ * sizeof(uint32_t));
+  HC(hipMemcpyAsync(d_woff_init_, h_woff_init_.data(), n_nets_ * 8ull, hipMemcpyHostToDevice, stream_));
+  HC(hipMemcpyAsync(d_wlen_init_, h_wlen_init_.data(), n_nets_ * 4ull, hipMemcpyHostToDevice, stream_));
   if (!saif_net_list_.empty()) {
     HC(hipMemcpyAsync(d_saif_nets_, saif_net_list_.data(), saif_net_list_.size() * 4,
                       hipMemcpyHostToDevice, stream_));
@@ -432,7 +442,12 @@ void Job::prepare(const std::vector<SimInstance>& inst, uint64_t pool_words) {
   stats_.n_pins = prog_->n_pins;
 }
 
-void Job::run_fused(std::vector<hipEvent_t>& ev_eval) {
+// Fused path.  Plain: one host round trip per level sizes the tile arrays and
+// checks the pool.  Device-driven (graph capture): no host round trip; the
+// tile arrays have the capacity learned by an earlier plain run, every kernel
+// reads the level's tile count from device memory, and k_guard skips a level
+// that would not fit (reported as an error after the run).
+void Job::run_fused(std::vector<hipEvent_t>& ev_eval, bool device_driven) {
   uint64_t* bump = d_scal_ + S_BUMP;
   unsigned long long* evc = (unsigned long long*)(d_scal_ + S_EVC);
   const uint32_t seg = seg_target ? seg_target : kDefaultSeg;
@@ -444,19 +459,26 @@ void Job::run_fused(std::vector<hipEvent_t>& ev_eval) {
                     (unsigned int*)(d_scal_ + S_ERR), stream_));
     HC(launch_scan(d_ntile_, d_tilebase_, ng, d_part_, d_scal_ + S_TOT_A, nullptr, stream_));
     HC(launch_scan(d_bound_, d_outoff_, ng, d_part_, d_scal_ + S_TOT_B, bump, stream_));
-    uint64_t h[3];
-    HC(hipMemcpyAsync(h, d_scal_, 3 * sizeof(uint64_t), hipMemcpyDeviceToHost, stream_));
-    HC(hipStreamSynchronize(stream_));
-    const uint64_t ntiles = h[S_TOT_A], need = h[S_TOT_B], used = h[S_BUMP];
-    if (used + need > pool_words_)
-      throw GlsError(GLS_E_OOM, "event pool exhausted at level " + std::to_string(l) +
-                                    " (raise the pool size or shorten the window)");
-    if (ntiles >= 0xFFFFFFF0ull) throw GlsError(GLS_E_RANGE, "too many tiles in one level");
-    ensure((void**)&d_tiles_, cap_tiles_, ntiles * sizeof(TileDesc));
-    ensure((void**)&d_tstate_, cap_tstate_, ntiles * 16);
+    uint64_t ntiles = tiles_cap_;
+    if (!device_driven) {
+      uint64_t h[3];
+      HC(hipMemcpyAsync(h, d_scal_, 3 * sizeof(uint64_t), hipMemcpyDeviceToHost, stream_));
+      HC(hipStreamSynchronize(stream_));
+      const uint64_t need = h[S_TOT_B], used = h[S_BUMP];
+      ntiles = h[S_TOT_A];
+      if (used + need > pool_words_)
+        throw GlsError(GLS_E_OOM, "event pool exhausted at level " + std::to_string(l) +
+                                      " (raise the pool size or shorten the window)");
+      if (ntiles >= 0xFFFFFFF0ull) throw GlsError(GLS_E_RANGE, "too many tiles in one level");
+      ensure((void**)&d_tiles_, cap_tiles_, ntiles * sizeof(TileDesc));
+      ensure((void**)&d_tstate_, cap_tstate_, ntiles * 16);
+      max_tiles_ = std::max(max_tiles_, ntiles);
+    } else {
+      HC(launch_guard(d_scal_, tiles_cap_, pool_words_, stream_));
+    }
     HC(launch_tile_desc(gates, ng, d_plan_, d_tilebase_, d_woff_, d_wlen_, d_pool_, prog_->d_lutp,
-                        d_tiles_, ntiles, stream_));
-    HC(launch_fill_u64(d_tstate_, 2 * ntiles, 0ull, stream_));
+                        d_tiles_, d_scal_ + S_TOT_A, ntiles, stream_));
+    HC(launch_fill_u64_dev(d_tstate_, d_scal_ + S_TOT_A, 2, 2 * ntiles, 0ull, stream_));
     HC(launch_fill_u64((unsigned long long*)(d_scal_ + S_TICKET), 2, 0ull, stream_));  // ticket, ovf
     LevelArgs a;
     a.gates = gates;
@@ -465,7 +487,7 @@ void Job::run_fused(std::vector<hipEvent_t>& ev_eval) {
     a.lutp = prog_->d_lutp;
     a.dtab = prog_->d_dtab;
     a.out_base = d_outoff_;
-    a.ntiles = ntiles;
+    a.ntiles = d_scal_ + S_TOT_A;
     a.woff = d_woff_;
     a.wlen = d_wlen_;
     a.pool = d_pool_;
@@ -486,14 +508,18 @@ void Job::run_fused(std::vector<hipEvent_t>& ev_eval) {
     a.blk = tiles_per_ticket ? tiles_per_ticket : 1u;
     a.prefetch = prefetch;
     a.pad = 0;
-    hipEvent_t e_a, e_b;
-    HC(hipEventCreate(&e_a));
-    HC(hipEventCreate(&e_b));
-    ev_eval.push_back(e_a);
-    ev_eval.push_back(e_b);
-    HC(hipEventRecord(e_a, stream_));
-    HC(launch_level(a, nb, stream_));
-    HC(hipEventRecord(e_b, stream_));
+    if (!device_driven) {
+      hipEvent_t e_a, e_b;
+      HC(hipEventCreate(&e_a));
+      HC(hipEventCreate(&e_b));
+      ev_eval.push_back(e_a);
+      ev_eval.push_back(e_b);
+      HC(hipEventRecord(e_a, stream_));
+      HC(launch_level(a, nb, stream_));
+      HC(hipEventRecord(e_b, stream_));
+    } else {
+      HC(launch_level(a, nb, stream_));
+    }
     HC(launch_bump(bump, d_scal_ + S_TOT_B, stream_));
   }
 }
@@ -545,6 +571,35 @@ void Job::run() {
   cap_ = tile_cap(seg_target ? seg_target : kDefaultSeg);
   level_blocks_ = level_blocks_opt ? (int)level_blocks_opt : level_max_blocks(cap_);
   stats_.level_blocks = (uint64_t)level_blocks_;
+  const std::vector<uint64_t> sig = {cap_, (uint64_t)level_blocks_, seg_target, tiles_per_ticket,
+                                     prefetch, (uint64_t)mode};
+  last_replay_ = false;
+  if (graph && gexec_ && sig == graph_sig_) {
+    // replay the captured level sequence: no host round trip between levels
+    auto w0 = std::chrono::steady_clock::now();
+    hipEvent_t e0, e1;
+    HC(hipEventCreate(&e0));
+    HC(hipEventCreate(&e1));
+    HC(hipEventRecord(e0, stream_));
+    HC(hipGraphLaunch(gexec_, stream_));
+    HC(hipEventRecord(e1, stream_));
+    HC(hipEventSynchronize(e1));
+    float ms = 0;
+    HC(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    uint64_t h[16];
+    HC(hipMemcpyAsync(h, d_scal_, sizeof h, hipMemcpyDeviceToHost, stream_));
+    HC(hipStreamSynchronize(stream_));
+    if (h[S_ERR] & 8u)
+      throw GlsError(GLS_E_STATE, "device-driven run outgrew the capacities of its first run");
+    stats_.eval_ms = ms;
+    stats_.eval_launches = 1;
+    last_replay_ = true;
+    finish_stats(h, ms);
+    stats_.wall_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - w0).count();
+    return;
+  }
   auto w0 = std::chrono::steady_clock::now();
   hipEvent_t e0, e1;
   HC(hipEventCreate(&e0));
@@ -589,14 +644,52 @@ void Job::run() {
   uint64_t h[16];
   HC(hipMemcpyAsync(h, d_scal_, sizeof h, hipMemcpyDeviceToHost, stream_));
   HC(hipStreamSynchronize(stream_));
+  finish_stats(h, ms);
+  stats_.wall_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - w0).count();
+  if (graph && mode == 0 && sig != graph_sig_) {
+    build_graph();
+    graph_sig_ = sig;
+  }
+}
+
+// Capture one device-driven execution of every level into a graph.
+void Job::build_graph() {
+  if (gexec_) {
+    HC(hipGraphExecDestroy(gexec_));
+    gexec_ = nullptr;
+  }
+  tiles_cap_ = std::max<uint64_t>(1, max_tiles_);
+  ensure((void**)&d_tiles_, cap_tiles_, tiles_cap_ * sizeof(TileDesc));
+  ensure((void**)&d_tstate_, cap_tstate_, tiles_cap_ * 16);
+  hipGraph_t g = nullptr;
+  HC(hipStreamBeginCapture(stream_, hipStreamCaptureModeRelaxed));
+  try {
+    HC(hipMemcpyAsync(d_woff_, d_woff_init_, n_nets_ * 8ull, hipMemcpyDeviceToDevice, stream_));
+    HC(hipMemcpyAsync(d_wlen_, d_wlen_init_, n_nets_ * 4ull, hipMemcpyDeviceToDevice, stream_));
+    HC(launch_set_scal(d_scal_, stim_words_, stream_));
+    HC(launch_fill_u64(d_saif_, (uint64_t)n_nets_ * 3, 0ull, stream_));
+    HC(launch_saif_nets(d_saif_nets_, d_saif_job_, (uint32_t)saif_net_list_.size(), d_woff_,
+                        d_wlen_, d_pool_, d_saif_, d_win_, stream_));
+    std::vector<hipEvent_t> none;
+    run_fused(none, true);
+  } catch (...) {
+    (void)hipStreamEndCapture(stream_, &g);
+    if (g) (void)hipGraphDestroy(g);
+    throw;
+  }
+  HC(hipStreamEndCapture(stream_, &g));
+  hipError_t e = hipGraphInstantiate(&gexec_, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  HC(e);
+}
+
+void Job::finish_stats(const uint64_t* h, double ms) {
   if (h[S_ERR] & 1u) throw GlsError(GLS_E_DEVICE, "tile look-back did not complete (spin limit)");
   if (h[S_ERR] & 2u)
     throw GlsError(GLS_E_UNSUPPORTED, "a tile exceeded the overflow stack buffer (event density too skewed)");
   if (h[S_ERR] & 4u)
     throw GlsError(GLS_E_RANGE, "a gate's inputs carry 2^30 or more events");
-  auto w1 = std::chrono::steady_clock::now();
   stats_.kernel_ms = ms;
-  stats_.wall_ms = std::chrono::duration<double, std::milli>(w1 - w0).count();
   stats_.n_events = h[S_EVC];
   stats_.pool_words_used = h[S_BUMP];
   // algorithmic bytes: every input waveform read once per reading gate, every

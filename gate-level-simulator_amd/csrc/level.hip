@@ -30,6 +30,7 @@
 // src/simulate.cpp:1365-1506 (Process, GetValueAt).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdint>
 
 #include "device.h"
@@ -39,7 +40,7 @@ namespace gls {
 constexpr int kLanes = 64;
 constexpr uint32_t kSpinLimit = 1u << 24;
 
-enum : unsigned { ERR_LOOKBACK = 1u, ERR_OVERFLOW = 2u, ERR_RANGE = 4u };
+enum : unsigned { ERR_LOOKBACK = 1u, ERR_OVERFLOW = 2u, ERR_RANGE = 4u, ERR_CAPACITY = 8u };
 
 __device__ __forceinline__ uint32_t edge_cls(uint32_t pre, uint32_t cur) {
   // CheckDelay (src/simulate.cpp:1178-1201); codes 0=0 1=X 2=Z 3=1
@@ -618,16 +619,17 @@ __global__ __launch_bounds__(kLanes) void k_level(LevelArgs A) {
   const int lane = threadIdx.x;
   unsigned long long wave_ev = 0;  // gate events of this wavefront (lane 0)
   unsigned long long sacc[3] = {0, 0, 0};
+  const uint64_t ntiles = *A.ntiles;  // uniform: one scalar load
   uint64_t tk = take_block(A), blk_end = tk + A.blk;
   bool fits = false;
-  if (tk < A.ntiles) {
+  if (tk < ntiles) {
     fits = tile_words(A.tiles[tk]) <= A.cap;
     stage_tile(A, A.tiles[tk], fits, s_in, s_lut, s_dt);
   }
   bool have_prev = false;
   uint64_t prev_tk = 0;
   Agg prev_incl = Agg{0, 0, 0, 0, 0};
-  while (tk < A.ntiles) {
+  while (tk < ntiles) {
     const TileDesc& D = A.tiles[tk];
     wait_vm();
     if (fits) {  // INF sentinel after every input range
@@ -658,7 +660,7 @@ __global__ __launch_bounds__(kLanes) void k_level(LevelArgs A) {
         tn = take_block(A);
         blk_end = tn + A.blk;
       }
-      if (tn < A.ntiles) {
+      if (tn < ntiles) {
         fits_n = tile_words(A.tiles[tn]) <= A.cap;
         stage_tile(A, A.tiles[tn], fits_n, s_in, s_lut, s_dt);
       }
@@ -670,12 +672,12 @@ __global__ __launch_bounds__(kLanes) void k_level(LevelArgs A) {
         tn = take_block(A);
         blk_end = tn + A.blk;
       }
-      if (tn < A.ntiles) {
+      if (tn < ntiles) {
         fits_n = tile_words(A.tiles[tn]) <= A.cap;
         stage_tile(A, A.tiles[tn], fits_n, s_in, s_lut, s_dt);
       }
     }
-    if (tn >= A.ntiles || A.tiles[tn].out_net != D.out_net) flush_saif(A, D.out_net, sacc);
+    if (tn >= ntiles || A.tiles[tn].out_net != D.out_net) flush_saif(A, D.out_net, sacc);
     have_prev = true;
     prev_tk = tk;
     __syncthreads();
@@ -726,11 +728,11 @@ __global__ void k_tile_desc(const GateRec* __restrict__ gates, uint32_t ng,
                             const uint64_t* __restrict__ tile_base,
                             const uint64_t* __restrict__ woff, const uint32_t* __restrict__ wlen,
                             const uint32_t* __restrict__ pool, const uint32_t* __restrict__ lutp,
-                            TileDesc* __restrict__ tiles, uint64_t ntiles) {
+                            TileDesc* __restrict__ tiles, const uint64_t* __restrict__ ntiles) {
   const uint64_t x = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const uint64_t t = x >> 2;
   const int j = (int)(x & 3);
-  if (t >= ntiles) return;
+  if (t >= *ntiles) return;
   // gate of tile t: binary search over tile_base
   uint32_t lo = 0, hi = ng;
   while (hi - lo > 1) {
@@ -798,11 +800,11 @@ hipError_t launch_plan2(const GateRec* gates, uint32_t ng, const uint32_t* wlen,
 hipError_t launch_tile_desc(const GateRec* gates, uint32_t ng, const GatePlan* plan,
                             const uint64_t* tile_base, const uint64_t* woff, const uint32_t* wlen,
                             const uint32_t* pool, const uint32_t* lutp, TileDesc* tiles,
-                            uint64_t ntiles, hipStream_t st) {
+                            const uint64_t* ntiles_dev, uint64_t ntiles, hipStream_t st) {
   if (ntiles == 0) return hipSuccess;
   const uint64_t threads = ntiles * 4;
   k_tile_desc<<<(unsigned)((threads + 255) / 256), 256, 0, st>>>(gates, ng, plan, tile_base, woff,
-                                                                  wlen, pool, lutp, tiles, ntiles);
+                                                                  wlen, pool, lutp, tiles, ntiles_dev);
   return hipGetLastError();
 }
 
@@ -813,8 +815,53 @@ hipError_t launch_fill_u64(unsigned long long* p, uint64_t n, unsigned long long
   return hipGetLastError();
 }
 
+// Device-driven levels: when the tile count does not fit the tile arrays, or
+// the level's output reservations do not fit the pool, the level is skipped
+// (tile count 0) and the error flag says why.
+__global__ void k_guard(uint64_t* scal, uint64_t tiles_cap, uint64_t pool_words) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  // scal: 0 bump, 1 tiles of the level, 2 output words reserved, 7 error flags
+  if (scal[1] > tiles_cap || scal[0] + scal[2] > pool_words) {
+    scal[1] = 0;
+    scal[2] = 0;
+    atomicOr((unsigned int*)(scal + 7), ERR_CAPACITY);
+  }
+}
+
+hipError_t launch_guard(uint64_t* scal, uint64_t tiles_cap, uint64_t pool_words, hipStream_t st) {
+  k_guard<<<1, 64, 0, st>>>(scal, tiles_cap, pool_words);
+  return hipGetLastError();
+}
+
+// p[i] = v for i < mult * (*n)
+__global__ void k_fill_u64_dev(unsigned long long* p, const uint64_t* n, uint64_t mult,
+                               uint64_t cap, unsigned long long v) {
+  const uint64_t lim = min(cap, mult * *n);
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < lim;
+       i += (uint64_t)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+
+hipError_t launch_fill_u64_dev(unsigned long long* p, const uint64_t* n, uint64_t mult, uint64_t cap,
+                               unsigned long long v, hipStream_t st) {
+  if (cap == 0) return hipSuccess;
+  const uint64_t blocks = std::min<uint64_t>((cap + 255) / 256, 4096);
+  k_fill_u64_dev<<<(unsigned)blocks, 256, 0, st>>>(p, n, mult, cap, v);
+  return hipGetLastError();
+}
+
+// Start-of-run scalars: bump pointer past the stimulus words, all else 0.
+__global__ void k_set_scal(uint64_t* scal, uint64_t bump) {
+  if (blockIdx.x == 0 && threadIdx.x < 16) scal[threadIdx.x] = threadIdx.x == 0 ? bump : 0;
+}
+
+hipError_t launch_set_scal(uint64_t* scal, uint64_t bump, hipStream_t st) {
+  k_set_scal<<<1, 64, 0, st>>>(scal, bump);
+  return hipGetLastError();
+}
+
 hipError_t launch_level(const LevelArgs& a, unsigned nblocks, hipStream_t st) {
-  if (a.ntiles == 0) return hipSuccess;
+  if (nblocks == 0) return hipSuccess;
   hipLaunchKernelGGL(k_level, dim3(nblocks), dim3(kLanes), level_lds_bytes(a.cap), st, a);
   return hipGetLastError();
 }

@@ -86,6 +86,9 @@ typedef struct {
   uint32_t level_blocks; /* workgroups per fused level launch (0: occupancy limit) */
   uint32_t tiles_per_ticket; /* fused tiles a workgroup takes at once (0: one) */
   uint32_t no_prefetch;  /* 1: fused kernel loads the next tile only after storing the current */
+  uint32_t graph;        /* 1: fused path only; after the first execution the whole level
+                            sequence is captured into a hipGraph with no host round trip
+                            between levels, and later executions replay it */
 } gls_sim_opts;
 
 /* Level-kernel variants: one fused LDS-tiled kernel per level, or the

@@ -147,3 +147,37 @@ def test_server_rejects_jobs_over_budget(ctx):
     j = srv.submit(d, s, 0, 200001)
     assert srv.wait(j) == -3  # GLS_E_OOM: the estimate can never fit
     srv.close()
+
+
+def test_graph_replays_match_eager(ctx):
+    d, s = golden_cases.make("c1_small")
+    on, off = golden_cases.CASES["c1_small"]["window"]
+    g = GOLDEN["c1_small"]
+    d2, s2 = golden_cases.make("c0_small")
+    r = ctx.prepare(d, s, on, off, keep_waves=True, graph=True)
+    for i in range(4):
+        say("graph execution", i)
+        r.execute()  # the first one runs eagerly and captures, later ones replay
+        assert "%016x" % r.waves_hash() == g["waves_hash"], i
+        assert oracle_bind.sha256(r.saif_text()) == g["saif_sha256"], i
+        if i:
+            assert r.stats()["eval_launches"] == 1  # one graph launch
+        # eager work and synchronisation between the replays
+        e = ctx.simulate(d2, s2, 0, 200001, keep_waves=True)
+        assert "%016x" % e.waves_hash() == GOLDEN["c0_small"]["waves_hash"]
+        e.close()
+    r.close()
+
+
+def test_graph_batch_replays(ctx):
+    insts = mixed_instances()
+    args = ([i[0] for i in insts], [i[1] for i in insts], [i[2] for i in insts],
+            [i[3] for i in insts])
+    ref = ctx.prepare_batch(*args, keep_waves=True).execute()
+    b = ctx.prepare_batch(*args, keep_waves=True, graph=True)
+    for i in range(3):
+        say("graph batch execution", i)
+        b.execute()
+        for k in range(len(insts)):
+            assert b.instance_waves_hash(k) == ref.instance_waves_hash(k), (i, k)
+            assert b.instance_saif_text(k) == ref.instance_saif_text(k), (i, k)
