@@ -325,11 +325,11 @@ static void apply_opts(Job& job, const gls_sim_opts* opts) {
   if (opts->seg_events) job.seg_target = opts->seg_events;
   job.mode = (int)opts->path;
   job.level_blocks_opt = opts->level_blocks;
-  job.tiles_per_ticket = opts->tiles_per_ticket;
   job.prefetch = opts->no_prefetch ? 0u : 1u;
   if (opts->graph && opts->path != GLS_PATH_FUSED)
     throw GlsError(GLS_E_UNSUPPORTED, "graph replay needs the fused path");
   job.graph = opts->graph ? 1u : 0u;
+  job.stack_mode = opts->hbm_stacks ? 1u : 0u;
 }
 
 int gls_prepare_batch(gls_ctx* c, size_t n, const gls_design* const* designs,

@@ -69,39 +69,44 @@ struct alignas(16) TileDesc {
 };
 static_assert(sizeof(TileDesc) == 128, "TileDesc layout");
 
-// Arguments of the fused level kernel.
+// What the walk kernel leaves for one tile, 32 bytes: its output entries
+// (deduplicated inside the tile) sit in the level scratch; the per-gate
+// carry pass joins consecutive tiles.
+struct alignas(16) TileOut {
+  uint32_t cnt;         // entries after dedup inside the tile
+  uint32_t ft, lt;      // times of the first and last entries
+  uint32_t flags;       // first value | last value << 2 | has entries << 4
+  uint32_t t0, t1, tx;  // SAIF durations between the tile's own entries (clipped)
+  uint32_t pad;
+};
+static_assert(sizeof(TileOut) == 32, "TileOut layout");
+
+// Arguments of the fused level (walk) kernel.
 struct LevelArgs {
-  const GateRec* gates;       // gates of this level
-  const GatePlan* plan;       // per gate
   const TileDesc* tiles;      // per tile
-  const uint32_t* lutp;       // truth tables packed 2 bits per entry, 16 words each
+  const uint32_t* pool;       // input waveforms
   const uint16_t* dtab;       // delay tables (GateRec::dt_off)
-  const uint64_t* out_base;   // per gate: output region (pool word offset)
+  const uint8_t* luts;        // full truth tables (fan-in 4)
   const uint64_t* ntiles;     // tiles of the level (device memory)
-  uint64_t* woff;
-  uint32_t* wlen;
-  uint32_t* pool;
-  const uint8_t* luts;
-  unsigned long long* tstate;  // 2 words per tile: aggregate, inclusive prefix
-  unsigned int* ticket;
-  uint32_t* ovf;               // stacks of tiles that do not fit LDS
+  const uint64_t* scr_base;   // per gate: scratch word of its first tile's region
+  uint32_t* scratch;          // tile outputs
+  TileOut* tout;              // per tile
+  uint32_t* ovf;              // stacks of tiles that do not fit LDS
   unsigned long long ovf_cap;
   unsigned long long* ovf_used;
-  unsigned long long* saif;
   unsigned long long* ev_count;
   unsigned int* err;
   const unsigned long long* win;  // SAIF window (start, end) per instance
   uint32_t cap;                // LDS input capacity per tile (words)
-  uint32_t blk;                // consecutive tiles handed out per ticket
   uint32_t prefetch;           // 1: stage the next tile before storing this one
-  uint32_t pad;
+  uint32_t* stk;               // null: stacks in LDS; else (cap + 64) words per workgroup
 };
 
 // LDS bytes of one fused-level workgroup for an input capacity of `cap`
-// words: inputs (cap + 128: one sentinel per input range, and the tile's
-// output staging), stacks (cap + 64), truth table (256 B), delay table (2 KiB).
-inline size_t level_lds_bytes(uint32_t cap) {
-  return (2ull * cap + 192) * 4 + 256 + 2 * dt_size(4);
+// words: inputs (cap + 128: one sentinel per input range), stacks (cap + 64,
+// when they live in LDS), truth table (256 B), delay table (2 KiB).
+inline size_t level_lds_bytes(uint32_t cap, bool stacks_in_lds) {
+  return ((uint64_t)cap + 128 + (stacks_in_lds ? cap + 64 : 0)) * 4 + 256 + 2 * dt_size(4);
 }
 
 // Per time-segment summary written by the evaluation kernel.

@@ -110,12 +110,12 @@ class Job {
   uint32_t seg_target = 0;        // input events per time segment (0: mode default)
   int mode = 0;                   // 0: fused LDS-tiled level kernel, 1: two-pass kernels
   uint32_t level_blocks_opt = 0;  // workgroups per fused launch (0: occupancy limit)
-  uint32_t tiles_per_ticket = 0;  // fused tiles handed out per ticket (0: one)
   uint32_t prefetch = 1;          // stage the next tile before storing the current one
   uint64_t ovf_words_opt = 0;     // overflow stack words (0: 2^24)
   // 1: after a plain execution, capture the whole level sequence (no host
   // round trips) into a hipGraph and replay it on later executions
   uint32_t graph = 0;
+  uint32_t stack_mode = 0;  // 0: preemption stacks in LDS, 1: per-workgroup scratch in HBM
   bool graph_replayed() const { return last_replay_; }
   static constexpr uint32_t kDefaultSeg = 32;  // input events per time segment
 
@@ -155,10 +155,11 @@ class Job {
   uint32_t* d_scratch_ = nullptr;
   // fused path
   uint64_t *d_ntile_ = nullptr, *d_bound_ = nullptr, *d_tilebase_ = nullptr;
-  size_t cap_tiles_ = 0, cap_tstate_ = 0;
+  size_t cap_tiles_ = 0, cap_tout_ = 0, cap_toff_ = 0;
   GatePlan* d_plan_ = nullptr;
   TileDesc* d_tiles_ = nullptr;
-  unsigned long long* d_tstate_ = nullptr;
+  TileOut* d_tout_ = nullptr;
+  uint32_t* d_toff_ = nullptr;
   uint32_t* d_ovf_ = nullptr;
   uint64_t ovf_words_ = 0;
   int level_blocks_ = 0;
@@ -166,7 +167,9 @@ class Job {
   std::vector<void*> owned_;
   uint64_t* d_woff_init_ = nullptr;
   uint32_t* d_wlen_init_ = nullptr;
-  uint64_t max_tiles_ = 0, tiles_cap_ = 0;
+  uint64_t max_tiles_ = 0, tiles_cap_ = 0, max_scratch_ = 0, scratch_cap_ = 0;
+  uint32_t* d_stk_ = nullptr;
+  size_t cap_stk_ = 0;
   hipGraphExec_t gexec_ = nullptr;
   std::vector<uint64_t> graph_sig_;
   bool last_replay_ = false;

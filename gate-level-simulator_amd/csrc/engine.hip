@@ -45,12 +45,20 @@ hipError_t launch_tile_desc(const GateRec* gates, uint32_t ng, const GatePlan* p
                             const uint32_t* pool, const uint32_t* lutp, TileDesc* tiles,
                             const uint64_t* ntiles_dev, uint64_t ntiles, hipStream_t st);
 hipError_t launch_fill_u64(unsigned long long* p, uint64_t n, unsigned long long v, hipStream_t st);
-hipError_t launch_fill_u64_dev(unsigned long long* p, const uint64_t* n, uint64_t mult, uint64_t cap,
-                               unsigned long long v, hipStream_t st);
-hipError_t launch_guard(uint64_t* scal, uint64_t tiles_cap, uint64_t pool_words, hipStream_t st);
+hipError_t launch_guard(uint64_t* scal, uint64_t tiles_cap, uint64_t scratch_cap, hipStream_t st);
+hipError_t launch_pool_guard(uint64_t* scal, uint64_t pool_words, hipStream_t st);
+hipError_t launch_gate_carry(const GateRec* gates, uint32_t ng, const uint64_t* tile_base,
+                             const uint64_t* ntile, const TileOut* tout, uint32_t* tile_off,
+                             uint64_t* gate_cnt, uint32_t* wlen, unsigned long long* saif,
+                             const unsigned long long* win, const unsigned int* err,
+                             hipStream_t st);
+hipError_t launch_compact(const TileDesc* tiles, const uint64_t* ntiles_dev, uint64_t ntiles_cap,
+                          const TileOut* tout, const uint32_t* tile_off, const uint64_t* scr_base,
+                          const uint32_t* scratch, const uint64_t* gate_base, uint32_t* pool,
+                          uint64_t* woff, uint32_t* wlen, const unsigned int* err, hipStream_t st);
 hipError_t launch_set_scal(uint64_t* scal, uint64_t bump, hipStream_t st);
 hipError_t launch_level(const LevelArgs& a, unsigned nblocks, hipStream_t st);
-int level_max_blocks(uint32_t cap);
+int level_max_blocks(uint32_t cap, bool stacks_in_lds);
 
 // LDS input capacity of a fused tile: 64 segments of `seg` input events,
 // plus variance and halo headroom.
@@ -300,7 +308,7 @@ Job::~Job() {
   if (gexec_) (void)hipGraphExecDestroy(gexec_);
   for (void* p : owned_)
     if (p) (void)hipFree(p);
-  void* grown[] = {d_segmap_, d_sums_, d_carry_, d_scratch_, d_tiles_, d_tstate_};
+  void* grown[] = {d_segmap_, d_sums_, d_carry_, d_scratch_, d_tiles_, d_tout_, d_toff_, d_stk_};
   for (void* p : grown)
     if (p) (void)hipFree(p);
 }
@@ -400,10 +408,10 @@ void Job::prepare(const std::vector<SimInstance>& inst, uint64_t pool_words) {
   uint32_t maxw = 1;
   for (size_t l = 0; l + 1 < level_start_.size(); ++l)
     maxw = std::max(maxw, level_start_[l + 1] - level_start_[l]);
-  for (uint64_t** p : {&d_outoff_, &d_ntile_, &d_bound_, &d_tilebase_})
+  for (uint64_t** p : {&d_outoff_, &d_ntile_, &d_bound_, &d_tilebase_, &d_scrbase_})
     *p = (uint64_t*)dmalloc(maxw * 8ull);
   if (mode == 1)
-    for (uint64_t** p : {&d_nseg_, &d_scr_, &d_segbase_, &d_scrbase_, &d_outlen_})
+    for (uint64_t** p : {&d_nseg_, &d_scr_, &d_segbase_, &d_outlen_})
       *p = (uint64_t*)dmalloc(maxw * 8ull);
   d_part_ = (uint64_t*)dmalloc(scan_part_size(maxw) * 8ull);
   d_gnseg_ = (uint32_t*)dmalloc(maxw * 4ull);
@@ -434,7 +442,7 @@ void Job::prepare(const std::vector<SimInstance>& inst, uint64_t pool_words) {
   HC(hipMemcpyAsync(d_win_, win_.data(), win_.size() * 8, hipMemcpyHostToDevice, stream_));
   HC(hipStreamSynchronize(stream_));
   cap_ = tile_cap(seg_target ? seg_target : kDefaultSeg);
-  level_blocks_ = level_blocks_opt ? (int)level_blocks_opt : level_max_blocks(cap_);
+  level_blocks_ = level_blocks_opt ? (int)level_blocks_opt : level_max_blocks(cap_, stack_mode == 0);
   stats_ = JobStats();
   stats_.level_blocks = (uint64_t)level_blocks_;
   stats_.n_nets = n_nets_;
@@ -442,11 +450,15 @@ void Job::prepare(const std::vector<SimInstance>& inst, uint64_t pool_words) {
   stats_.n_pins = prog_->n_pins;
 }
 
-// Fused path.  Plain: one host round trip per level sizes the tile arrays and
-// checks the pool.  Device-driven (graph capture): no host round trip; the
-// tile arrays have the capacity learned by an earlier plain run, every kernel
-// reads the level's tile count from device memory, and k_guard skips a level
-// that would not fit (reported as an error after the run).
+// Fused path, per level: plan the tiles (k_plan2, two scans), describe them
+// (k_tile_desc), walk every tile independently into the level scratch
+// (k_level), join the tiles of each gate and finish its SAIF (k_gate_carry),
+// place the gates in the pool (scan) and copy the outputs there
+// (k_compact).  Plain: one host round trip per level sizes the tile arrays
+// and the scratch.  Device-driven (graph capture): no host round trip; the
+// arrays keep the capacities learned by an earlier plain run, kernels read
+// the level's counts from device memory, and k_guard skips a level that
+// would not fit (reported as an error after the run).
 void Job::run_fused(std::vector<hipEvent_t>& ev_eval, bool device_driven) {
   uint64_t* bump = d_scal_ + S_BUMP;
   unsigned long long* evc = (unsigned long long*)(d_scal_ + S_EVC);
@@ -458,56 +470,51 @@ void Job::run_fused(std::vector<hipEvent_t>& ev_eval, bool device_driven) {
     HC(launch_plan2(gates, ng, d_wlen_, seg, d_plan_, d_ntile_, d_bound_,
                     (unsigned int*)(d_scal_ + S_ERR), stream_));
     HC(launch_scan(d_ntile_, d_tilebase_, ng, d_part_, d_scal_ + S_TOT_A, nullptr, stream_));
-    HC(launch_scan(d_bound_, d_outoff_, ng, d_part_, d_scal_ + S_TOT_B, bump, stream_));
+    HC(launch_scan(d_bound_, d_scrbase_, ng, d_part_, d_scal_ + S_TOT_B, nullptr, stream_));
     uint64_t ntiles = tiles_cap_;
     if (!device_driven) {
       uint64_t h[3];
       HC(hipMemcpyAsync(h, d_scal_, 3 * sizeof(uint64_t), hipMemcpyDeviceToHost, stream_));
       HC(hipStreamSynchronize(stream_));
-      const uint64_t need = h[S_TOT_B], used = h[S_BUMP];
       ntiles = h[S_TOT_A];
-      if (used + need > pool_words_)
-        throw GlsError(GLS_E_OOM, "event pool exhausted at level " + std::to_string(l) +
-                                      " (raise the pool size or shorten the window)");
+      const uint64_t scr = h[S_TOT_B];
       if (ntiles >= 0xFFFFFFF0ull) throw GlsError(GLS_E_RANGE, "too many tiles in one level");
       ensure((void**)&d_tiles_, cap_tiles_, ntiles * sizeof(TileDesc));
-      ensure((void**)&d_tstate_, cap_tstate_, ntiles * 16);
+      ensure((void**)&d_tout_, cap_tout_, ntiles * sizeof(TileOut));
+      ensure((void**)&d_toff_, cap_toff_, ntiles * 4);
+      ensure((void**)&d_scratch_, cap_scratch_, scr * 4);
       max_tiles_ = std::max(max_tiles_, ntiles);
+      max_scratch_ = std::max(max_scratch_, scr);
     } else {
-      HC(launch_guard(d_scal_, tiles_cap_, pool_words_, stream_));
+      HC(launch_guard(d_scal_, tiles_cap_, scratch_cap_, stream_));
     }
     HC(launch_tile_desc(gates, ng, d_plan_, d_tilebase_, d_woff_, d_wlen_, d_pool_, prog_->d_lutp,
                         d_tiles_, d_scal_ + S_TOT_A, ntiles, stream_));
-    HC(launch_fill_u64_dev(d_tstate_, d_scal_ + S_TOT_A, 2, 2 * ntiles, 0ull, stream_));
     HC(launch_fill_u64((unsigned long long*)(d_scal_ + S_TICKET), 2, 0ull, stream_));  // ticket, ovf
     LevelArgs a;
-    a.gates = gates;
-    a.plan = d_plan_;
     a.tiles = d_tiles_;
-    a.lutp = prog_->d_lutp;
-    a.dtab = prog_->d_dtab;
-    a.out_base = d_outoff_;
-    a.ntiles = d_scal_ + S_TOT_A;
-    a.woff = d_woff_;
-    a.wlen = d_wlen_;
     a.pool = d_pool_;
+    a.dtab = prog_->d_dtab;
     a.luts = prog_->d_luts;
-    a.tstate = d_tstate_;
-    a.ticket = (unsigned int*)(d_scal_ + S_TICKET);
+    a.ntiles = d_scal_ + S_TOT_A;
+    a.scr_base = d_scrbase_;
+    a.scratch = d_scratch_;
+    a.tout = d_tout_;
     a.ovf = d_ovf_;
     a.ovf_cap = ovf_words_;
     a.ovf_used = (unsigned long long*)(d_scal_ + S_OVF_USED);
-    a.saif = d_saif_;
     a.ev_count = evc;
     a.err = (unsigned int*)(d_scal_ + S_ERR);
     a.win = d_win_;
     a.cap = cap_;
-    unsigned nb = (unsigned)std::min<uint64_t>(ntiles, (uint64_t)std::max(1, level_blocks_));
-    // one tile per ticket by default: a wavefront holding several tiles of a
-    // gate makes the gate's later tiles wait for all of them in turn
-    a.blk = tiles_per_ticket ? tiles_per_ticket : 1u;
     a.prefetch = prefetch;
-    a.pad = 0;
+    a.stk = nullptr;
+    const unsigned nb = (unsigned)std::min<uint64_t>(ntiles, (uint64_t)std::max(1, level_blocks_));
+    if (stack_mode == 1) {
+      const uint64_t words = (uint64_t)std::max(1, level_blocks_) * (cap_ + 64);
+      if (!device_driven) ensure((void**)&d_stk_, cap_stk_, words * 4);
+      a.stk = d_stk_;
+    }
     if (!device_driven) {
       hipEvent_t e_a, e_b;
       HC(hipEventCreate(&e_a));
@@ -520,7 +527,15 @@ void Job::run_fused(std::vector<hipEvent_t>& ev_eval, bool device_driven) {
     } else {
       HC(launch_level(a, nb, stream_));
     }
-    HC(launch_bump(bump, d_scal_ + S_TOT_B, stream_));
+    // gate counts go to d_bound_ (free again once the scratch bases are scanned)
+    HC(launch_gate_carry(gates, ng, d_tilebase_, d_ntile_, d_tout_, d_toff_, d_bound_, d_wlen_,
+                         d_saif_, d_win_, (const unsigned int*)(d_scal_ + S_ERR), stream_));
+    HC(launch_scan(d_bound_, d_outoff_, ng, d_part_, d_scal_ + S_TOT_OUT, bump, stream_));
+    HC(launch_pool_guard(d_scal_, pool_words_, stream_));
+    HC(launch_compact(d_tiles_, d_scal_ + S_TOT_A, ntiles, d_tout_, d_toff_, d_scrbase_, d_scratch_,
+                      d_outoff_, d_pool_, d_woff_, d_wlen_, (const unsigned int*)(d_scal_ + S_ERR),
+                      stream_));
+    HC(launch_bump(bump, d_scal_ + S_TOT_OUT, stream_));
   }
 }
 
@@ -569,10 +584,10 @@ void Job::run_two_pass(std::vector<hipEvent_t>& ev_eval) {
 void Job::run() {
   HC(hipSetDevice(device_));
   cap_ = tile_cap(seg_target ? seg_target : kDefaultSeg);
-  level_blocks_ = level_blocks_opt ? (int)level_blocks_opt : level_max_blocks(cap_);
+  level_blocks_ = level_blocks_opt ? (int)level_blocks_opt : level_max_blocks(cap_, stack_mode == 0);
   stats_.level_blocks = (uint64_t)level_blocks_;
-  const std::vector<uint64_t> sig = {cap_, (uint64_t)level_blocks_, seg_target, tiles_per_ticket,
-                                     prefetch, (uint64_t)mode};
+  const std::vector<uint64_t> sig = {cap_, (uint64_t)level_blocks_, seg_target, prefetch,
+                                     (uint64_t)mode, stack_mode};
   last_replay_ = false;
   if (graph && gexec_ && sig == graph_sig_) {
     // replay the captured level sequence: no host round trip between levels
@@ -659,8 +674,11 @@ void Job::build_graph() {
     gexec_ = nullptr;
   }
   tiles_cap_ = std::max<uint64_t>(1, max_tiles_);
+  scratch_cap_ = std::max<uint64_t>(1, max_scratch_);
   ensure((void**)&d_tiles_, cap_tiles_, tiles_cap_ * sizeof(TileDesc));
-  ensure((void**)&d_tstate_, cap_tstate_, tiles_cap_ * 16);
+  ensure((void**)&d_tout_, cap_tout_, tiles_cap_ * sizeof(TileOut));
+  ensure((void**)&d_toff_, cap_toff_, tiles_cap_ * 4);
+  ensure((void**)&d_scratch_, cap_scratch_, scratch_cap_ * 4);
   hipGraph_t g = nullptr;
   HC(hipStreamBeginCapture(stream_, hipStreamCaptureModeRelaxed));
   try {
@@ -684,7 +702,8 @@ void Job::build_graph() {
 }
 
 void Job::finish_stats(const uint64_t* h, double ms) {
-  if (h[S_ERR] & 1u) throw GlsError(GLS_E_DEVICE, "tile look-back did not complete (spin limit)");
+  if (h[S_ERR] & 16u)
+    throw GlsError(GLS_E_OOM, "event pool exhausted (raise the pool size or shorten the window)");
   if (h[S_ERR] & 2u)
     throw GlsError(GLS_E_UNSUPPORTED, "a tile exceeded the overflow stack buffer (event density too skewed)");
   if (h[S_ERR] & 4u)

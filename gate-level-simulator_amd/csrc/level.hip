@@ -1,30 +1,32 @@
-// Fused per-level gate evaluation kernel (gfx950).
+// Fused per-level gate evaluation (gfx950): walk, join, compact.
 //
 // Work unit: a tile = 64 consecutive time segments of one gate, one segment
 // per lane of a wavefront.  Segment c of a gate covers the events
 // [c*stride, (c+1)*stride) of the gate's longest input (the "leader"), so
-// segment boundaries are exact event times.  Per tile:
-//   1. the input waveform ranges the tile needs (found beforehand by
-//      k_tile_desc: one event before the tile for the starting values and a
-//      D_max look-ahead halo after it) are copied into LDS with batched,
-//      coalesced loads;
+// segment boundaries are exact event times.
+//
+// k_level (walk), one tile at a time per wavefront, tiles independent:
+//   1. the tile's input ranges (one event before the tile for the starting
+//      values, a D_max look-ahead halo after it), its delay table and its
+//      truth table are copied into LDS with direct-to-LDS loads, each range
+//      followed by an INF sentinel;
 //   2. every lane merges its segment's input events in time order (branch
-//      free: each LDS range ends with an INF sentinel), evaluates the 4-state
-//      truth table (packed in registers for fan-in <= 3), classifies the
-//      output edge, picks the SDF delay with one lookup in the gate's delay
-//      table (AddDelay folded on the host, copied to LDS per tile) and applies
-//      the preemption rule on a stack in LDS, counting value changes as it
-//      goes; events in the halo only pre-empt;
-//   3. a wavefront scan combines the lanes' kept lists (dedup of equal
-//      adjacent values across lanes), a decoupled look-back over the gate's
-//      earlier tiles supplies the carry (last value, output count), and the
-//      tile's dense output is written with coalesced stores from LDS;
-//   4. SAIF T0/T1/TX durations are reduced per wavefront and added with one
-//      atomic per counter.
-// Tiles are handed out by an atomic ticket in increasing order, so a tile
-// only ever waits for tiles already taken by running wavefronts.  The wait
-// is bounded: past the spin limit the kernel raises an error flag and
-// finishes instead of hanging.
+//      free), evaluates the 4-state truth table (packed in registers for
+//      fan-in <= 3), classifies the output edge, picks the SDF delay with one
+//      lookup in the gate's delay table (AddDelay folded on the host) and
+//      applies the preemption rule on a stack (LDS, or a per-workgroup HBM
+//      scratch), counting value changes as it goes; events in the halo only
+//      pre-empt;
+//   3. a wavefront scan (DPP) combines the lanes' kept lists (dedup of equal
+//      adjacent values across lanes is an associative monoid), the lanes
+//      store the tile's entries into the level scratch and lane 0 writes the
+//      tile summary (count, first/last value and time, SAIF durations).
+//   The next tile's loads are issued before the current tile is stored.
+// k_gate_carry: per gate, joins its tiles in order (a tile whose first entry
+//   repeats the value before it drops that entry), places them, and finishes
+//   the gate's SAIF record.
+// k_compact: copies each tile's joined entries into the gate's dense
+//   waveform in the pool (coalesced), so the pool holds exactly the outputs.
 //
 // Event rules: src/simulate.cpp:919-1201 (Calculate, AddDelay, CheckDelay),
 // src/simulate.cpp:1365-1506 (Process, GetValueAt).
@@ -38,9 +40,8 @@
 namespace gls {
 
 constexpr int kLanes = 64;
-constexpr uint32_t kSpinLimit = 1u << 24;
 
-enum : unsigned { ERR_LOOKBACK = 1u, ERR_OVERFLOW = 2u, ERR_RANGE = 4u, ERR_CAPACITY = 8u };
+enum : unsigned { ERR_OVERFLOW = 2u, ERR_RANGE = 4u, ERR_CAPACITY = 8u, ERR_POOL = 16u };
 
 __device__ __forceinline__ uint32_t edge_cls(uint32_t pre, uint32_t cur) {
   // CheckDelay (src/simulate.cpp:1178-1201); codes 0=0 1=X 2=Z 3=1
@@ -67,77 +68,62 @@ __device__ __forceinline__ Agg agg_combine(const Agg& a, const Agg& b) {
   return r;
 }
 
-__device__ __forceinline__ Agg agg_shfl_up(const Agg& a, int d) {
+// Cross-lane moves with DPP (no LDS round trip).  Lanes without a source
+// lane (outside the row, or rows left out by ROW_MASK) receive 0, which is
+// the identity of the sums below and, for all three words, the empty Agg.
+// Control codes: row_shr:d 0x110+d, row_shl:d 0x100+d, wave_shr:1 0x138,
+// row_bcast:15 0x142, row_bcast:31 0x143.
+template <int CTRL, int ROW_MASK = 0xF>
+__device__ __forceinline__ uint32_t dpp(uint32_t x) {
+  return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, CTRL, ROW_MASK, 0xF, false);
+}
+
+__device__ __forceinline__ uint32_t agg_flags(const Agg& a) {
+  return a.fv | (a.lv << 2) | (a.has << 4);
+}
+__device__ __forceinline__ Agg agg_from(uint32_t cnt, uint32_t lt, uint32_t f) {
   Agg r;
-  r.cnt = __shfl_up(a.cnt, d, 64);
-  r.lt = __shfl_up(a.lt, d, 64);
-  uint32_t f = a.fv | (a.lv << 2) | (a.has << 4);
-  f = __shfl_up(f, d, 64);
+  r.cnt = cnt;
+  r.lt = lt;
   r.fv = f & 3u;
   r.lv = (f >> 2) & 3u;
   r.has = (f >> 4) & 1u;
   return r;
 }
 
-__device__ __forceinline__ Agg agg_shfl(const Agg& a, int src) {
-  Agg r;
-  r.cnt = __shfl(a.cnt, src, 64);
-  r.lt = __shfl(a.lt, src, 64);
-  uint32_t f = a.fv | (a.lv << 2) | (a.has << 4);
-  f = __shfl(f, src, 64);
-  r.fv = f & 3u;
-  r.lv = (f >> 2) & 3u;
-  r.has = (f >> 4) & 1u;
-  return r;
+template <int CTRL, int ROW_MASK = 0xF>
+__device__ __forceinline__ Agg agg_dpp(const Agg& a) {
+  return agg_from(dpp<CTRL, ROW_MASK>(a.cnt), dpp<CTRL, ROW_MASK>(a.lt),
+                  dpp<CTRL, ROW_MASK>(agg_flags(a)));
 }
 
-// value of lane+d, or the empty aggregate past the last lane
-__device__ __forceinline__ Agg agg_shfl_down_ordered(const Agg& a, int d, int lane) {
-  Agg r;
-  r.cnt = __shfl_down(a.cnt, d, 64);
-  r.lt = __shfl_down(a.lt, d, 64);
-  uint32_t f = a.fv | (a.lv << 2) | (a.has << 4);
-  f = __shfl_down(f, d, 64);
-  r.fv = f & 3u;
-  r.lv = (f >> 2) & 3u;
-  r.has = (f >> 4) & 1u;
-  if (lane + d >= 64) {
-    r.has = 0;
-    r.cnt = 0;
-  }
-  return r;
+__device__ __forceinline__ Agg agg_readlane(const Agg& a, int lane) {
+  return agg_from(__builtin_amdgcn_readlane(a.cnt, lane), __builtin_amdgcn_readlane(a.lt, lane),
+                  __builtin_amdgcn_readlane(agg_flags(a), lane));
 }
 
-// aggregate word: valid<<63 | lt<<33 | has<<32 | lv<<30 | fv<<28 | cnt(28).
-// A tile's count is bounded by its input events: the LDS capacity, or the
-// overflow buffer size (< 2^28 words) for tiles that do not fit LDS.
-__device__ __forceinline__ unsigned long long pack_agg(const Agg& a) {
-  return (1ull << 63) | ((unsigned long long)a.lt << 33) | ((unsigned long long)a.has << 32) |
-         ((unsigned long long)a.lv << 30) | ((unsigned long long)a.fv << 28) |
-         (a.cnt & 0x0FFFFFFFu);
+// Inclusive scan of the lanes' aggregates in lane order: within rows of 16
+// by row shifts, then row 0's total into row 1 and row 2's into row 3
+// (row_bcast:15), then lane 31's prefix into rows 2 and 3 (row_bcast:31).
+__device__ __forceinline__ Agg agg_wave_scan(Agg x) {
+  x = agg_combine(agg_dpp<0x111>(x), x);
+  x = agg_combine(agg_dpp<0x112>(x), x);
+  x = agg_combine(agg_dpp<0x114>(x), x);
+  x = agg_combine(agg_dpp<0x118>(x), x);
+  x = agg_combine(agg_dpp<0x142, 0xA>(x), x);
+  x = agg_combine(agg_dpp<0x143, 0xC>(x), x);
+  return x;
 }
-__device__ __forceinline__ Agg unpack_agg(unsigned long long w) {
-  Agg a;
-  a.cnt = (uint32_t)(w & 0x0FFFFFFFu);
-  a.fv = (uint32_t)(w >> 28) & 3u;
-  a.lv = (uint32_t)(w >> 30) & 3u;
-  a.has = (uint32_t)(w >> 32) & 1u;
-  a.lt = (uint32_t)(w >> 33) & 0x3FFFFFFFu;
-  return a;
-}
-// inclusive-prefix word: valid<<63 | lt<<33 | has<<32 | lv<<30 | cnt(30)
-__device__ __forceinline__ unsigned long long pack_inc(const Agg& a) {
-  return (1ull << 63) | ((unsigned long long)a.lt << 33) | ((unsigned long long)a.has << 32) |
-         ((unsigned long long)a.lv << 30) | (a.cnt & 0x3FFFFFFFu);
-}
-__device__ __forceinline__ Agg unpack_inc(unsigned long long w) {
-  Agg a;
-  a.cnt = (uint32_t)(w & 0x3FFFFFFFu);
-  a.lv = (uint32_t)(w >> 30) & 3u;
-  a.has = (uint32_t)(w >> 32) & 1u;
-  a.lt = (uint32_t)(w >> 33) & 0x3FFFFFFFu;
-  a.fv = 0;  // not needed to the right of an inclusive prefix
-  return a;
+
+// Sum over the wavefront (uniform result).
+__device__ __forceinline__ uint32_t wave_sum(uint32_t x) {
+  x += dpp<0x111>(x);
+  x += dpp<0x112>(x);
+  x += dpp<0x114>(x);
+  x += dpp<0x118>(x);
+  x += dpp<0x142, 0xA>(x);
+  x += dpp<0x143, 0xC>(x);
+  return __builtin_amdgcn_readlane(x, 63);
 }
 
 __device__ __forceinline__ void saif_acc(unsigned long long* acc, uint32_t v, unsigned long long a,
@@ -384,11 +370,16 @@ __device__ __forceinline__ Lane walk_k(const LevelArgs& A, const TileDesc& D, ui
     pos += n + 1;
     total += n;
   }
-  if (fits) return lane_walk<K, true>(X, D.ti, s_lut, s_dt, s_st);
+  if (fits) {
+    // stacks in LDS, or in this workgroup's slice of the stack scratch
+    if (A.stk) return lane_walk<K, true>(X, D.ti, s_lut, s_dt, A.stk + (uint64_t)blockIdx.x * (A.cap + kLanes));
+    return lane_walk<K, true>(X, D.ti, s_lut, s_dt, s_st);
+  }
   unsigned long long base = 0;
   const unsigned long long need = (unsigned long long)total + kLanes;
   if (lane == 0) base = atomicAdd(A.ovf_used, need);
-  base = __shfl(base, 0, 64);
+  base = ((unsigned long long)(uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)(base >> 32)) << 32) |
+         (uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)base);  // lane 0's value
   if (base + need > A.ovf_cap) {
     if (lane == 0) atomicOr(A.err, ERR_OVERFLOW);
     base = 0;  // results are invalid (the error is reported); keep every wavefront moving
@@ -409,226 +400,116 @@ __device__ __forceinline__ Lane walk_k(const LevelArgs& A, const TileDesc& D, ui
 // Wavefront scan of the lane aggregates and the gate prefix before the tile.
 // chain: this wavefront finished the gate's previous tile just before, and
 // prev_incl is its inclusive prefix (no look-back needed).
+// The tile's output inside the tile: wavefront scan of the lane aggregates
+// (dedup across lane boundaries is an associative monoid).
 struct TileSum {
-  Agg carry;          // everything before this lane's first entry (gate prefix + earlier lanes)
-  Agg incl;           // inclusive prefix of the gate through this tile
-  uint32_t tile_out0; // output index of the tile's first entry
-  uint32_t e_first;   // the lane's first stack entry
+  Agg carry;         // everything before this lane's first entry, inside the tile
+  Agg tile;          // the whole tile
+  uint32_t e_first;  // the lane's first stack entry
+  uint32_t ft;       // time of the tile's first entry
 };
 
-__device__ __forceinline__ TileSum combine_tile(const LevelArgs& A, const TileDesc& D,
-                                                uint64_t tk, const Lane& ln, bool chain,
-                                                const Agg& prev_incl) {
-  const int lane = threadIdx.x;
-  const uint32_t ti = D.ti;
+__device__ __forceinline__ TileSum combine_tile(const Lane& ln) {
   const uint32_t sp = ln.sp;
-  const uint32_t* st = ln.st;
-  const uint32_t e_first = sp ? st[0] : 0u;
+  const uint32_t e_first = sp ? ln.st[0] : 0u;
   Agg mine;
   mine.cnt = ln.cnt;
   mine.has = sp > 0 ? 1u : 0u;
   mine.fv = e_first & 3u;
   mine.lv = ln.top & 3u;
   mine.lt = ln.top >> 2;
-  // wavefront inclusive scan of lane aggregates
-  Agg inc = mine;
-#pragma unroll
-  for (int d = 1; d < 64; d <<= 1) {
-    Agg o = agg_shfl_up(inc, d);
-    if (lane >= d) inc = agg_combine(o, inc);
-  }
-  Agg lane_excl = agg_shfl_up(inc, 1);
-  if (lane == 0) lane_excl.has = 0, lane_excl.cnt = 0;
-  const Agg tile_agg = agg_shfl(inc, 63);
-
-  // decoupled look-back over the gate's earlier tiles, 64 predecessors per
-  // round: lane i reads tile tk-1-i (tiles before the gate's first one act
-  // as an empty inclusive prefix).  A look-back word carries its whole
-  // payload, so relaxed device-scope atomics suffice; release/acquire would
-  // write back and invalidate the XCD's L2 on every tile.
-  unsigned long long* ts = A.tstate;
-  Agg excl;
-  excl.has = 0;
-  excl.cnt = 0;
-  excl.fv = excl.lv = excl.lt = 0;
-  if (ti == 0) {
-    if (lane == 0)
-      __hip_atomic_store(&ts[2 * tk + 1], pack_inc(tile_agg), __ATOMIC_RELAXED,
-                         __HIP_MEMORY_SCOPE_AGENT);
-  } else if (chain) {
-    excl = prev_incl;
-    if (lane == 0)
-      __hip_atomic_store(&ts[2 * tk + 1], pack_inc(agg_combine(excl, tile_agg)), __ATOMIC_RELAXED,
-                         __HIP_MEMORY_SCOPE_AGENT);
-  } else {
-    if (lane == 0)
-      __hip_atomic_store(&ts[2 * tk], pack_agg(tile_agg), __ATOMIC_RELAXED,
-                         __HIP_MEMORY_SCOPE_AGENT);
-    const uint64_t first = tk - ti;  // the gate's first tile
-    Agg acc;                         // combination of the tiles after the window
-    acc.has = 0;
-    acc.cnt = 0;
-    acc.fv = acc.lv = acc.lt = 0;
-    uint64_t top_tile = tk - 1;  // tile read by lane 0 in this round
-    uint32_t spins = 0;
-    for (;;) {
-      const bool in_gate = top_tile >= first + lane;  // tile top_tile-lane belongs to the gate
-      const uint64_t p = top_tile - lane;
-      unsigned long long wi = 0, wa = 0;
-      if (in_gate) {
-        wi = __hip_atomic_load(&ts[2 * p + 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        if (!(wi >> 63))
-          wa = __hip_atomic_load(&ts[2 * p], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      }
-      // status: 2 = inclusive prefix (or before the gate), 1 = aggregate, 0 = not ready
-      const uint32_t status = !in_gate ? 2u : ((wi >> 63) ? 2u : ((wa >> 63) ? 1u : 0u));
-      const uint64_t have_inc = __ballot(status == 2u);
-      const uint64_t not_ready = __ballot(status == 0u);
-      const int ip = have_inc ? __ffsll((long long)have_inc) - 1 : 64;  // nearest inclusive
-      const uint64_t need = ip >= 64 ? ~0ull : ((2ull << ip) - 1);      // lanes 0..ip
-      if (not_ready & need) {
-        if (++spins > kSpinLimit) {
-          if (lane == 0) atomicOr(A.err, ERR_LOOKBACK);
-          break;
-        }
-        __builtin_amdgcn_s_sleep(1);
-        continue;
-      }
-      // ordered reduction: lane i holds tile top_tile-i; earlier tiles are
-      // at higher lanes, so x_i = x_{i+d} (earlier) + x_i
-      Agg x;
-      if (lane < ip) x = unpack_agg(wa);
-      else if (lane == ip) x = in_gate ? unpack_inc(wi) : Agg{0, 0, 0, 0, 0};
-      else x = Agg{0, 0, 0, 0, 0};
-#pragma unroll
-      for (int d = 1; d < 64; d <<= 1) {
-        Agg y = agg_shfl_down_ordered(x, d, lane);
-        x = agg_combine(y, x);
-      }
-      x = agg_shfl(x, 0);
-      acc = agg_combine(x, acc);
-      if (ip < 64) break;
-      top_tile -= 64;  // no inclusive prefix in this window: look further back
-    }
-    excl = acc;
-    if (lane == 0)
-      __hip_atomic_store(&ts[2 * tk + 1], pack_inc(agg_combine(excl, tile_agg)), __ATOMIC_RELAXED,
-                         __HIP_MEMORY_SCOPE_AGENT);
-  }
+  const Agg inc = agg_wave_scan(mine);
   TileSum r;
-  r.incl = agg_combine(excl, tile_agg);
-  r.carry = agg_combine(excl, lane_excl);
-  r.tile_out0 = excl.cnt;
+  r.carry = agg_dpp<0x138>(inc);  // lane - 1 (empty for lane 0)
+  r.tile = agg_readlane(inc, 63);
   r.e_first = e_first;
+  const uint64_t has = __ballot(sp > 0);
+  r.ft = has ? (uint32_t)__builtin_amdgcn_readlane(e_first, __ffsll((long long)has) - 1) >> 2 : 0u;
   return r;
 }
 
-// Emit: dedup against the carry; each lane stores its run of the tile's
-// output straight into the gate's output region.  SAIF durations are added
-// to sacc (lane 0), which the caller flushes per gate.
-__device__ __forceinline__ void emit_tile(const LevelArgs& A, const TileDesc& D, const Lane& ln,
-                                          const TileSum& T, unsigned long long (&sacc)[3],
+// Store the tile's entries (deduplicated inside the tile) at dst, sum the
+// SAIF durations between them (32 bits per lane: times < 2^30; the lanes'
+// intervals are disjoint pieces of the tile's span, so the sums fit too) and
+// write the tile summary.
+__device__ __forceinline__ void emit_tile(const LevelArgs& A, uint64_t tk, const Lane& ln,
+                                          const TileSum& T, uint32_t* dst,
+                                          unsigned long long w_start, unsigned long long w_end,
                                           unsigned long long& wave_ev) {
   const int lane = threadIdx.x;
-  const uint32_t ti = D.ti;
   const uint32_t sp = ln.sp;
   const uint32_t* st = ln.st;
   const Agg& carry = T.carry;
-  const Agg& incl = T.incl;
-  const uint32_t e_first = T.e_first;
-  const uint64_t gbase = A.out_base[D.g];
-  const uint32_t tile_out0 = T.tile_out0;
-  const unsigned long long w_start = A.win[2 * D.job], w_end = A.win[2 * D.job + 1];
   const uint32_t s32 = (uint32_t)min(w_start, 0x7FFFFFFFull);
   const uint32_t e32 = (uint32_t)min(w_end, 0x7FFFFFFFull);
   uint32_t pvv = carry.has ? carry.lv : 4u;
-  uint32_t pct = max(min(carry.has ? carry.lt : (e_first >> 2), e32), s32);
+  uint32_t pct = max(min(carry.has ? carry.lt : (T.e_first >> 2), e32), s32);
   const uint32_t ct0 = pct;
   uint32_t a0 = 0, a1 = 0;
-  {
-    uint32_t w = carry.has ? carry.cnt - tile_out0 : 0u;  // lane's first slot in the tile output
-    for (uint32_t i = 0; i < sp; ++i) {
-      const uint32_t e = st[i];
-      const uint32_t v = e & 3u;
-      const uint32_t ct = max(min(e >> 2, e32), s32);
-      const uint32_t len = ct - pct;
-      a0 += pvv == 0u ? len : 0u;
-      a1 += pvv == 3u ? len : 0u;
-      if (v != pvv) {
-        A.pool[gbase + tile_out0 + w] = e;
-        ++w;
-      }
-      pvv = v;
-      pct = ct;
+  uint32_t w = carry.has ? carry.cnt : 0u;  // lane's first slot in the tile output
+  for (uint32_t i = 0; i < sp; ++i) {
+    const uint32_t e = st[i];
+    const uint32_t v = e & 3u;
+    const uint32_t ct = max(min(e >> 2, e32), s32);
+    const uint32_t len = ct - pct;
+    a0 += pvv == 0u ? len : 0u;
+    a1 += pvv == 3u ? len : 0u;
+    if (v != pvv) {
+      dst[w] = e;
+      ++w;
     }
+    pvv = v;
+    pct = ct;
   }
-  // SAIF durations and event count of the tile, reduced over the wavefront
-  unsigned long long acc[3] = {a0, a1, (unsigned long long)(pct - ct0 - a0 - a1)};
-#pragma unroll
-  for (int k = 0; k < 3; ++k)
-    for (int o = 32; o > 0; o >>= 1) acc[k] += __shfl_down(acc[k], o, 64);
-  unsigned long long ev = ln.nown;
-  for (int o = 32; o > 0; o >>= 1) ev += __shfl_down(ev, o, 64);
+  const uint32_t t0 = wave_sum(a0), t1 = wave_sum(a1), tx = wave_sum(pct - ct0 - a0 - a1);
+  const uint32_t ev = wave_sum(ln.nown);
   if (lane == 0) {
-    const uint32_t ntile_g = (D.nseg + kLanes - 1) / kLanes;
-    if (ti + 1 == ntile_g) {  // last tile: waveform length and the tail duration
-      A.wlen[D.out_net] = incl.cnt;
-      if (incl.has)
-        saif_acc(acc, incl.lv, incl.lt, w_end > incl.lt ? w_end : incl.lt, w_start, w_end);
-    }
-    if (ti == 0) A.woff[D.out_net] = gbase;
-    for (int k = 0; k < 3; ++k) sacc[k] += acc[k];
+    TileOut o;
+    o.cnt = T.tile.cnt;
+    o.ft = T.ft;
+    o.lt = T.tile.lt;
+    o.flags = agg_flags(T.tile);
+    o.t0 = t0;
+    o.t1 = t1;
+    o.tx = tx;
+    o.pad = 0;
+    A.tout[tk] = o;
     wave_ev += ev;
   }
 }
 
-// Add a wavefront's SAIF durations of one net to the counters (lane 0).
-__device__ __forceinline__ void flush_saif(const LevelArgs& A, uint32_t net,
-                                           unsigned long long (&sacc)[3]) {
-  if (threadIdx.x == 0) {
-    unsigned long long* sa = A.saif + 3ull * net;
-    for (int k = 0; k < 3; ++k)
-      if (sacc[k]) atomicAdd(sa + k, sacc[k]);
-  }
-  sacc[0] = sacc[1] = sacc[2] = 0;
+// Scratch word of a tile's output region: its gate's region, then the
+// tile's first own event in the gate's merged input order plus one slack
+// word per earlier tile (a tile keeps at most its own events + 1 entries).
+__device__ __forceinline__ uint64_t tile_scratch(const uint64_t* scr_base, const TileDesc& D) {
+  uint64_t p = scr_base[D.g] + D.ti;
+  for (uint32_t j = 0; j < D.k; ++j) p += D.L[j];
+  return p;
 }
 
-// First tile of the next block of A.blk consecutive tiles.  Blocks are
-// handed out in increasing order, so a tile only ever waits for tiles that
-// running wavefronts hold; one device-scope atomic per block instead of per
-// tile keeps the counter from serialising the grid.
-__device__ __forceinline__ uint64_t take_block(const LevelArgs& A) {
-  unsigned int t = 0;
-  if (threadIdx.x == 0) t = atomicAdd(A.ticket, 1u);
-  return (uint64_t)__builtin_amdgcn_readfirstlane(__shfl(t, 0, 64)) * A.blk;
-}
-
-// Persistent, software-pipelined level kernel: while tile n is combined and
-// stored, the inputs of tile n+1 are already on their way into LDS.  Every
-// wavefront exits when the ticket passes the last tile.
+// Persistent, software-pipelined walk kernel.  Tiles are independent here
+// (no tile waits for another), so the schedule is static: workgroup b takes
+// tiles b, b + grid, b + 2 grid, ...; the next tile's inputs are staged into
+// LDS while the current tile is stored.
 __global__ __launch_bounds__(kLanes) void k_level(LevelArgs A) {
-  // dynamic LDS: inputs (cap + 128 words), stacks (cap + 64), truth table
-  // (64 words), delay table (512 words)
+  // dynamic LDS: inputs (cap + 128 words), stacks (cap + 64, unless they are
+  // in HBM), truth table (64 words), delay table (512 words)
   extern __shared__ uint32_t s_dyn[];
   uint32_t* s_in = s_dyn;
-  uint32_t* s_st = s_dyn + A.cap + 2 * kLanes;
-  uint32_t* s_lut = s_st + A.cap + kLanes;
+  uint32_t* s_st = A.stk ? nullptr : s_dyn + A.cap + 2 * kLanes;
+  uint32_t* s_lut = A.stk ? s_dyn + A.cap + 2 * kLanes : s_st + A.cap + kLanes;
   uint32_t* s_dt = s_lut + 64;
   const uint8_t* lut8 = (const uint8_t*)s_lut;
   const uint16_t* dt16 = (const uint16_t*)s_dt;
   const int lane = threadIdx.x;
   unsigned long long wave_ev = 0;  // gate events of this wavefront (lane 0)
-  unsigned long long sacc[3] = {0, 0, 0};
   const uint64_t ntiles = *A.ntiles;  // uniform: one scalar load
-  uint64_t tk = take_block(A), blk_end = tk + A.blk;
+  uint64_t tk = blockIdx.x;
   bool fits = false;
   if (tk < ntiles) {
     fits = tile_words(A.tiles[tk]) <= A.cap;
     stage_tile(A, A.tiles[tk], fits, s_in, s_lut, s_dt);
   }
-  bool have_prev = false;
-  uint64_t prev_tk = 0;
-  Agg prev_incl = Agg{0, 0, 0, 0, 0};
   while (tk < ntiles) {
     const TileDesc& D = A.tiles[tk];
     wait_vm();
@@ -641,6 +522,9 @@ __global__ __launch_bounds__(kLanes) void k_level(LevelArgs A) {
       }
     }
     __syncthreads();
+    // the emit's scalars, loaded now so the walk covers their latency
+    uint32_t* dst = A.scratch + tile_scratch(A.scr_base, D);
+    const unsigned long long w_start = A.win[2 * D.job], w_end = A.win[2 * D.job + 1];
     Lane ln;
     switch (D.k) {
       case 1: ln = walk_k<1>(A, D, tk, fits, s_in, s_st, lut8, dt16); break;
@@ -649,42 +533,113 @@ __global__ __launch_bounds__(kLanes) void k_level(LevelArgs A) {
       default: ln = walk_k<4>(A, D, tk, fits, s_in, s_st, lut8, dt16); break;
     }
     __syncthreads();  // inputs, truth and delay tables no longer read
-    const bool chain = have_prev && prev_tk + 1 == tk && D.ti > 0;
-    // the look-back may wait for other wavefronts: it runs before this
-    // wavefront takes its next tile, so no tile is ever held by a waiting one
-    const TileSum T = combine_tile(A, D, tk, ln, chain, prev_incl);
-    uint64_t tn = tk + 1;
+    const TileSum T = combine_tile(ln);
+    const uint64_t tn = tk + gridDim.x;
     bool fits_n = false;
-    if (A.prefetch) {  // stage the next tile while this one is stored
-      if (tn >= blk_end) {
-        tn = take_block(A);
-        blk_end = tn + A.blk;
-      }
-      if (tn < ntiles) {
-        fits_n = tile_words(A.tiles[tn]) <= A.cap;
-        stage_tile(A, A.tiles[tn], fits_n, s_in, s_lut, s_dt);
-      }
+    if (A.prefetch && tn < ntiles) {  // stage the next tile while this one is stored
+      fits_n = tile_words(A.tiles[tn]) <= A.cap;
+      stage_tile(A, A.tiles[tn], fits_n, s_in, s_lut, s_dt);
     }
-    emit_tile(A, D, ln, T, sacc, wave_ev);
-    prev_incl = T.incl;
-    if (!A.prefetch) {
-      if (tn >= blk_end) {
-        tn = take_block(A);
-        blk_end = tn + A.blk;
-      }
-      if (tn < ntiles) {
-        fits_n = tile_words(A.tiles[tn]) <= A.cap;
-        stage_tile(A, A.tiles[tn], fits_n, s_in, s_lut, s_dt);
-      }
+    emit_tile(A, tk, ln, T, dst, w_start, w_end, wave_ev);
+    if (!A.prefetch && tn < ntiles) {
+      __syncthreads();
+      fits_n = tile_words(A.tiles[tn]) <= A.cap;
+      stage_tile(A, A.tiles[tn], fits_n, s_in, s_lut, s_dt);
     }
-    if (tn >= ntiles || A.tiles[tn].out_net != D.out_net) flush_saif(A, D.out_net, sacc);
-    have_prev = true;
-    prev_tk = tk;
     __syncthreads();
     tk = tn;
     fits = fits_n;
   }
   if (lane == 0 && wave_ev) atomicAdd(A.ev_count, wave_ev);
+}
+
+// Per gate, in tile order: join the tiles' outputs (a tile whose first
+// entry repeats the last value before it drops that entry), place each
+// tile's output (offset inside the gate, bit 31: first entry dropped), and
+// add up the SAIF durations: inside tiles, between tiles, and from the last
+// entry to the window end.  One thread per gate.
+__global__ void k_gate_carry(const GateRec* __restrict__ gates, uint32_t ng,
+                             const uint64_t* __restrict__ tile_base,
+                             const uint64_t* __restrict__ ntile, const TileOut* __restrict__ tout,
+                             uint32_t* __restrict__ tile_off, uint64_t* __restrict__ gate_cnt,
+                             uint32_t* __restrict__ wlen, unsigned long long* __restrict__ saif,
+                             const unsigned long long* __restrict__ win,
+                             const unsigned int* __restrict__ err) {
+  const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= ng) return;
+  const GateRec& G = gates[g];
+  if (*err & ERR_CAPACITY) {  // the level was skipped: its tiles were never walked
+    gate_cnt[g] = 0;
+    wlen[G.out_net] = 0;
+    return;
+  }
+  const unsigned long long start = win[2 * G.job], end = win[2 * G.job + 1];
+  const uint64_t t0 = tile_base[g], n = ntile[g];
+  uint32_t cnt = 0, lv = 0, lt = 0, has = 0;
+  unsigned long long acc[3] = {0, 0, 0};
+  for (uint64_t t = t0; t < t0 + n; ++t) {
+    const TileOut S = tout[t];
+    if (!((S.flags >> 4) & 1u)) {  // no entry in this tile
+      tile_off[t] = cnt;
+      continue;
+    }
+    const uint32_t skip = (has && lv == (S.flags & 3u)) ? 1u : 0u;
+    tile_off[t] = cnt | (skip << 31);
+    if (has) saif_acc(acc, lv, lt, S.ft, start, end);
+    acc[0] += S.t0;
+    acc[1] += S.t1;
+    acc[2] += S.tx;
+    cnt += S.cnt - skip;
+    lv = (S.flags >> 2) & 3u;
+    lt = S.lt;
+    has = 1;
+  }
+  if (has) saif_acc(acc, lv, lt, end > lt ? end : (unsigned long long)lt, start, end);
+  gate_cnt[g] = cnt;
+  wlen[G.out_net] = cnt;
+  unsigned long long* sa = saif + 3ull * G.out_net;
+  sa[0] = acc[0];
+  sa[1] = acc[1];
+  sa[2] = acc[2];
+}
+
+// Copy every tile's joined output from the scratch into the gate's dense
+// waveform in the pool (one wavefront per tile, coalesced).  When the level
+// does not fit the pool (error flag set by k_pool_guard) nothing is copied
+// and the level's waveforms are left empty.
+__global__ void k_compact(const TileDesc* __restrict__ tiles, const uint64_t* __restrict__ ntiles,
+                          const TileOut* __restrict__ tout, const uint32_t* __restrict__ tile_off,
+                          const uint64_t* __restrict__ scr_base, const uint32_t* __restrict__ scratch,
+                          const uint64_t* __restrict__ gate_base, uint32_t* __restrict__ pool,
+                          uint64_t* __restrict__ woff, uint32_t* __restrict__ wlen,
+                          const unsigned int* __restrict__ err) {
+  const int lane = threadIdx.x & 63;
+  const uint64_t waves = (uint64_t)gridDim.x * (blockDim.x / 64);
+  const bool failed = (*err & ERR_CAPACITY) != 0;
+  const uint64_t nt = *ntiles;
+  for (uint64_t t = (uint64_t)blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64; t < nt; t += waves) {
+    const TileDesc& D = tiles[t];
+    if (failed) {
+      if (D.ti == 0 && lane == 0) {
+        woff[D.out_net] = 0;
+        wlen[D.out_net] = 0;
+      }
+      continue;
+    }
+    if (D.ti == 0 && lane == 0) woff[D.out_net] = gate_base[D.g];
+    const TileOut S = tout[t];
+    if (!((S.flags >> 4) & 1u)) continue;
+    const uint32_t off = tile_off[t] & 0x7FFFFFFFu, skip = tile_off[t] >> 31;
+    const uint32_t n = S.cnt - skip;
+    const uint32_t* src = scratch + tile_scratch(scr_base, D) + skip;
+    uint32_t* dst = pool + gate_base[D.g] + off;
+    for (uint32_t q = lane; q < n; q += 64) dst[q] = src[q];
+  }
+}
+
+// Start-of-run scalars: bump pointer past the stimulus words, all else 0.
+__global__ void k_set_scal(uint64_t* scal, uint64_t bump) {
+  if (blockIdx.x == 0 && threadIdx.x < 16) scal[threadIdx.x] = threadIdx.x == 0 ? bump : 0;
 }
 
 // per gate: leader, stride, segments, tiles, output bound
@@ -705,7 +660,7 @@ __global__ void k_plan2(const GateRec* __restrict__ gates, uint32_t ng,
       lead = j;
     }
   }
-  // output counts travel in 30-bit fields of the look-back words
+  // a tile's offset inside its gate's waveform travels in 31 bits
   if (N + 1 >= (1ull << 30)) atomicOr(err, ERR_RANGE);
   // leader events per segment so that a segment holds ~seg_target input events
   uint64_t s = N > 0 ? (uint64_t)seg_target * nl / N : 1;
@@ -717,8 +672,11 @@ __global__ void k_plan2(const GateRec* __restrict__ gates, uint32_t ng,
   P.lead = lead;
   P.pad = 0;
   plan[g] = P;
-  ntile[g] = (ns + kLanes - 1) / kLanes;
-  bound[g] = N + 1;
+  const uint32_t nt = (ns + kLanes - 1) / kLanes;
+  ntile[g] = nt;
+  // scratch words of the gate's tile outputs: every tile keeps at most its
+  // own input events + 1 entries
+  bound[g] = N + nt + 1;
 }
 
 // per (tile, input): the tile's descriptor (gate, tile index within the
@@ -815,45 +773,62 @@ hipError_t launch_fill_u64(unsigned long long* p, uint64_t n, unsigned long long
   return hipGetLastError();
 }
 
-// Device-driven levels: when the tile count does not fit the tile arrays, or
-// the level's output reservations do not fit the pool, the level is skipped
-// (tile count 0) and the error flag says why.
-__global__ void k_guard(uint64_t* scal, uint64_t tiles_cap, uint64_t pool_words) {
+// Device-driven levels: when the tile count or the level's scratch does not
+// fit the arrays sized by the first run, the level is skipped (tile count 0)
+// and the error flag says why.
+__global__ void k_guard(uint64_t* scal, uint64_t tiles_cap, uint64_t scratch_cap) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  // scal: 0 bump, 1 tiles of the level, 2 output words reserved, 7 error flags
-  if (scal[1] > tiles_cap || scal[0] + scal[2] > pool_words) {
+  // scal: 1 tiles of the level, 2 scratch words of the level, 7 error flags
+  if (scal[1] > tiles_cap || scal[2] > scratch_cap) {
     scal[1] = 0;
     scal[2] = 0;
     atomicOr((unsigned int*)(scal + 7), ERR_CAPACITY);
   }
 }
 
-hipError_t launch_guard(uint64_t* scal, uint64_t tiles_cap, uint64_t pool_words, hipStream_t st) {
-  k_guard<<<1, 64, 0, st>>>(scal, tiles_cap, pool_words);
+// After the level's output size is known: when it does not fit the pool,
+// flag the error (the compaction then leaves the level empty) and reserve
+// nothing.  scal: 0 bump, 3 output words of the level.
+__global__ void k_pool_guard(uint64_t* scal, uint64_t pool_words) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  if (scal[0] + scal[3] > pool_words) {
+    scal[3] = 0;
+    atomicOr((unsigned int*)(scal + 7), ERR_CAPACITY | ERR_POOL);
+  }
+}
+
+hipError_t launch_pool_guard(uint64_t* scal, uint64_t pool_words, hipStream_t st) {
+  k_pool_guard<<<1, 64, 0, st>>>(scal, pool_words);
   return hipGetLastError();
 }
 
-// p[i] = v for i < mult * (*n)
-__global__ void k_fill_u64_dev(unsigned long long* p, const uint64_t* n, uint64_t mult,
-                               uint64_t cap, unsigned long long v) {
-  const uint64_t lim = min(cap, mult * *n);
-  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < lim;
-       i += (uint64_t)gridDim.x * blockDim.x)
-    p[i] = v;
-}
-
-hipError_t launch_fill_u64_dev(unsigned long long* p, const uint64_t* n, uint64_t mult, uint64_t cap,
-                               unsigned long long v, hipStream_t st) {
-  if (cap == 0) return hipSuccess;
-  const uint64_t blocks = std::min<uint64_t>((cap + 255) / 256, 4096);
-  k_fill_u64_dev<<<(unsigned)blocks, 256, 0, st>>>(p, n, mult, cap, v);
+hipError_t launch_gate_carry(const GateRec* gates, uint32_t ng, const uint64_t* tile_base,
+                             const uint64_t* ntile, const TileOut* tout, uint32_t* tile_off,
+                             uint64_t* gate_cnt, uint32_t* wlen, unsigned long long* saif,
+                             const unsigned long long* win, const unsigned int* err,
+                             hipStream_t st) {
+  if (ng == 0) return hipSuccess;
+  k_gate_carry<<<(ng + 255) / 256, 256, 0, st>>>(gates, ng, tile_base, ntile, tout, tile_off,
+                                                 gate_cnt, wlen, saif, win, err);
   return hipGetLastError();
 }
 
-// Start-of-run scalars: bump pointer past the stimulus words, all else 0.
-__global__ void k_set_scal(uint64_t* scal, uint64_t bump) {
-  if (blockIdx.x == 0 && threadIdx.x < 16) scal[threadIdx.x] = threadIdx.x == 0 ? bump : 0;
+hipError_t launch_compact(const TileDesc* tiles, const uint64_t* ntiles_dev, uint64_t ntiles_cap,
+                          const TileOut* tout, const uint32_t* tile_off, const uint64_t* scr_base,
+                          const uint32_t* scratch, const uint64_t* gate_base, uint32_t* pool,
+                          uint64_t* woff, uint32_t* wlen, const unsigned int* err, hipStream_t st) {
+  if (ntiles_cap == 0) return hipSuccess;
+  const uint64_t blocks = std::min<uint64_t>((ntiles_cap + 3) / 4, 8192);  // 4 wavefronts each
+  k_compact<<<(unsigned)blocks, 256, 0, st>>>(tiles, ntiles_dev, tout, tile_off, scr_base, scratch,
+                                              gate_base, pool, woff, wlen, err);
+  return hipGetLastError();
 }
+
+hipError_t launch_guard(uint64_t* scal, uint64_t tiles_cap, uint64_t scratch_cap, hipStream_t st) {
+  k_guard<<<1, 64, 0, st>>>(scal, tiles_cap, scratch_cap);
+  return hipGetLastError();
+}
+
 
 hipError_t launch_set_scal(uint64_t* scal, uint64_t bump, hipStream_t st) {
   k_set_scal<<<1, 64, 0, st>>>(scal, bump);
@@ -862,15 +837,16 @@ hipError_t launch_set_scal(uint64_t* scal, uint64_t bump, hipStream_t st) {
 
 hipError_t launch_level(const LevelArgs& a, unsigned nblocks, hipStream_t st) {
   if (nblocks == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_level, dim3(nblocks), dim3(kLanes), level_lds_bytes(a.cap), st, a);
+  hipLaunchKernelGGL(k_level, dim3(nblocks), dim3(kLanes), level_lds_bytes(a.cap, a.stk == nullptr),
+                     st, a);
   return hipGetLastError();
 }
 
-int level_max_blocks(uint32_t cap) {
+int level_max_blocks(uint32_t cap, bool stacks_in_lds) {
   int per_cu = 0, dev = 0, cus = 0;
   if (hipGetDevice(&dev) != hipSuccess) return 2048;
   if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_level, kLanes,
-                                                   level_lds_bytes(cap)) != hipSuccess)
+                                                   level_lds_bytes(cap, stacks_in_lds)) != hipSuccess)
     per_cu = 4;
   if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
     cus = 256;

@@ -84,11 +84,12 @@ typedef struct {
   uint32_t keep_waves;   /* 1: keep waveforms for gls_run_waveform */
   uint32_t path;         /* GLS_PATH_FUSED (default) or GLS_PATH_TWO_PASS */
   uint32_t level_blocks; /* workgroups per fused level launch (0: occupancy limit) */
-  uint32_t tiles_per_ticket; /* fused tiles a workgroup takes at once (0: one) */
   uint32_t no_prefetch;  /* 1: fused kernel loads the next tile only after storing the current */
   uint32_t graph;        /* 1: fused path only; after the first execution the whole level
                             sequence is captured into a hipGraph with no host round trip
                             between levels, and later executions replay it */
+  uint32_t hbm_stacks;   /* 1: the fused kernel keeps its preemption stacks in a per-workgroup
+                            HBM scratch instead of LDS (half the LDS, more resident workgroups) */
 } gls_sim_opts;
 
 /* Level-kernel variants: one fused LDS-tiled kernel per level, or the

@@ -78,9 +78,9 @@ def test_batch_segment_and_ticket_variants(ctx):
     args = ([i[0] for i in insts], [i[1] for i in insts], [i[2] for i in insts],
             [i[3] for i in insts])
     ref = ctx.prepare_batch(*args, keep_waves=True).execute()
-    for seg, tpt, pf in ((2, 1, True), (48, 3, False), (1 << 20, 1, True)):
-        say("batch variant", seg, tpt, pf)
-        r = ctx.prepare_batch(*args, keep_waves=True, seg_events=seg, tiles_per_ticket=tpt,
+    for seg, hbm, pf in ((2, False, True), (48, True, False), (1 << 20, False, True)):
+        say("batch variant", seg, hbm, pf)
+        r = ctx.prepare_batch(*args, keep_waves=True, seg_events=seg, hbm_stacks=hbm,
                               prefetch=pf).execute()
         for k in range(len(insts)):
             assert r.instance_waves_hash(k) == ref.instance_waves_hash(k), (seg, k)

@@ -104,19 +104,20 @@ def test_segment_size_invariance(ctx):
     assert "%016x" % ref[0] == GOLDEN["c1_small"]["waves_hash"]
 
 
-def test_tiles_per_ticket_invariance(ctx):
-    # a workgroup takes 1..64 consecutive tiles at once: the prefix carry moves
-    # from the look-back to registers, and SAIF flushes once per gate run
+def test_kernel_option_invariance(ctx):
+    # launch width, staging order and where the preemption stacks live do
+    # not change results; a one-workgroup grid walks every tile in turn
     d, s = golden_cases.make("c1_small")
     start, end = golden_cases.CASES["c1_small"]["window"]
     g = GOLDEN["c1_small"]
-    for seg, tpt, blocks, pf in ((8, 1, 0, True), (8, 3, 0, True), (8, 64, 0, False),
-                                 (32, 16, 7, True), (2, 5, 1, False), (64, 1, 0, False)):
-        say("tiles_per_ticket", seg, tpt, blocks, pf)
+    for seg, blocks, pf, hbm in ((8, 0, True, False), (8, 0, True, True), (8, 3, False, False),
+                                 (32, 7, True, True), (2, 1, False, True), (64, 0, False, False),
+                                 (32, 0, True, True)):
+        say("options", seg, blocks, pf, hbm)
         r = ctx.simulate(d, s, start, end, seg_events=seg, keep_waves=True, level_blocks=blocks,
-                         tiles_per_ticket=tpt, prefetch=pf)
-        assert "%016x" % r.waves_hash() == g["waves_hash"], (seg, tpt, blocks, pf)
-        assert oracle_bind.sha256(r.saif_text()) == g["saif_sha256"], (seg, tpt, blocks, pf)
+                         prefetch=pf, hbm_stacks=hbm)
+        assert "%016x" % r.waves_hash() == g["waves_hash"], (seg, blocks, pf, hbm)
+        assert oracle_bind.sha256(r.saif_text()) == g["saif_sha256"], (seg, blocks, pf, hbm)
 
 
 def test_cli_dropin(tmp_path):
