@@ -330,6 +330,7 @@ static void apply_opts(Job& job, const gls_sim_opts* opts) {
     throw GlsError(GLS_E_UNSUPPORTED, "graph replay needs the fused path");
   job.graph = opts->graph ? 1u : 0u;
   job.stack_mode = opts->hbm_stacks ? 1u : 0u;
+  job.pool_grow = opts->pool_grow ? 1u : 0u;
 }
 
 int gls_prepare_batch(gls_ctx* c, size_t n, const gls_design* const* designs,
@@ -408,6 +409,8 @@ int gls_run_stats_get(const gls_run* r, gls_run_stats* st) {
   st->eval_ms = j.eval_ms;
   st->eval_launches = j.eval_launches;
   st->level_blocks = j.level_blocks;
+  st->pool_bytes = j.pool_words * 4;
+  st->pool_used_bytes = j.pool_words_used * 4;
   API_END
 }
 

@@ -29,6 +29,7 @@ struct JobStats {
   uint64_t eval_launches = 0;
   uint64_t level_blocks = 0;    // workgroups per fused level launch
   uint64_t pool_words_used = 0;
+  uint64_t pool_words = 0;      // event pool capacity after the run
 };
 
 // Host and device image of one compiled program for one delay time scale:
@@ -116,6 +117,7 @@ class Job {
   // round trips) into a hipGraph and replay it on later executions
   uint32_t graph = 0;
   uint32_t stack_mode = 0;  // 0: preemption stacks in LDS, 1: per-workgroup scratch in HBM
+  uint32_t pool_grow = 0;   // 1: a given pool size is the starting size and grows as needed
   bool graph_replayed() const { return last_replay_; }
   static constexpr uint32_t kDefaultSeg = 32;  // input events per time segment
 
@@ -126,6 +128,7 @@ class Job {
   void finish_stats(const uint64_t* scal, double ms);
   void run_two_pass(std::vector<hipEvent_t>& ev_eval);
   void* dmalloc(size_t bytes);
+  bool grow_pool(uint64_t need, uint64_t used);
   int device_;
   hipStream_t stream_;
   std::shared_ptr<DevProgram> prog_;
@@ -135,6 +138,7 @@ class Job {
   std::vector<uint32_t> level_start_;
   std::vector<uint32_t> saif_net_list_, saif_net_job_;
   uint64_t pool_words_ = 0, stim_words_ = 0, dev_bytes_ = 0;
+  bool pool_auto_ = false;  // pool sized by the job (grows between levels)
   // device buffers of the job
   uint64_t* d_woff_ = nullptr;
   uint32_t* d_wlen_ = nullptr;
@@ -168,6 +172,7 @@ class Job {
   uint64_t* d_woff_init_ = nullptr;
   uint32_t* d_wlen_init_ = nullptr;
   uint64_t max_tiles_ = 0, tiles_cap_ = 0, max_scratch_ = 0, scratch_cap_ = 0;
+  bool graph_scratch_in_pool_ = false;  // captured levels keep their scratch at the pool top
   uint32_t* d_stk_ = nullptr;
   size_t cap_stk_ = 0;
   hipGraphExec_t gexec_ = nullptr;

@@ -308,7 +308,8 @@ Job::~Job() {
   if (gexec_) (void)hipGraphExecDestroy(gexec_);
   for (void* p : owned_)
     if (p) (void)hipFree(p);
-  void* grown[] = {d_segmap_, d_sums_, d_carry_, d_scratch_, d_tiles_, d_tout_, d_toff_, d_stk_};
+  void* grown[] = {d_segmap_, d_sums_, d_carry_, d_scratch_, d_tiles_, d_tout_, d_toff_, d_stk_,
+                   d_pool_};
   for (void* p : grown)
     if (p) (void)hipFree(p);
 }
@@ -331,6 +332,39 @@ void Job::ensure(void** p, size_t& cap, size_t need) {
   HC(hipMalloc(p, n));
   dev_bytes_ += n - cap;
   cap = n;
+}
+
+// Grows an automatically sized pool to at least `need` words, keeping its
+// first `used` words (the waveforms so far; offsets into the pool stay
+// valid).  Doubles when the device has room, else takes exactly `need`.
+// Returns whether the pool now holds `need` words; a caller-sized pool never
+// grows.  A captured graph holds the old pool address and is dropped.
+bool Job::grow_pool(uint64_t need, uint64_t used) {
+  if (need <= pool_words_) return true;
+  if (!pool_auto_) return false;
+  size_t free_b = 0, total_b = 0;
+  HC(hipMemGetInfo(&free_b, &total_b));
+  const uint64_t room = (uint64_t)(free_b * 0.9) / 4;  // the old pool stays during the copy
+  uint64_t n = std::max<uint64_t>(need, 2 * pool_words_);
+  if (n > room) n = need;
+  if (n > room) return false;
+  uint32_t* p = nullptr;
+  if (hipMalloc((void**)&p, n * sizeof(uint32_t)) != hipSuccess) {
+    (void)hipGetLastError();
+    return false;
+  }
+  if (used) HC(hipMemcpyAsync(p, d_pool_, used * 4, hipMemcpyDeviceToDevice, stream_));
+  HC(hipStreamSynchronize(stream_));
+  HC(hipFree(d_pool_));
+  d_pool_ = p;
+  dev_bytes_ += (n - pool_words_) * sizeof(uint32_t);
+  pool_words_ = n;
+  if (gexec_) {
+    HC(hipGraphExecDestroy(gexec_));
+    gexec_ = nullptr;
+    graph_sig_.clear();
+  }
+  return true;
 }
 
 void Job::prepare(const std::vector<SimInstance>& inst, uint64_t pool_words) {
@@ -419,12 +453,20 @@ void Job::prepare(const std::vector<SimInstance>& inst, uint64_t pool_words) {
   ovf_words_ = ovf_words_opt ? ovf_words_opt : (1ull << 24);  // stacks of tiles that do not fit LDS
   d_ovf_ = (uint32_t*)dmalloc(ovf_words_ * 4);
 
-  size_t free_b = 0, total_b = 0;
-  HC(hipMemGetInfo(&free_b, &total_b));
-  if (pool_words == 0) pool_words = (uint64_t)(free_b * 0.85) / 4;
+  // event pool: a caller-given size is fixed unless pool_grow is set; size 0
+  // starts at four times the stimulus (at least 64 MiB); a growable pool is
+  // enlarged between levels as needed
+  pool_auto_ = pool_words == 0 || pool_grow;
+  if (pool_words == 0) {
+    size_t free_b = 0, total_b = 0;
+    HC(hipMemGetInfo(&free_b, &total_b));
+    pool_words = std::min<uint64_t>((uint64_t)(free_b * 0.85) / 4,
+                                    std::max<uint64_t>(1ull << 24, 4 * stim_words_));
+  }
   if (pool_words < stim_words_ + 1024) pool_words = stim_words_ + 1024;
   pool_words_ = pool_words;
-  d_pool_ = (uint32_t*)dmalloc(pool_words_ * sizeof(uint32_t));
+  HC(hipMalloc((void**)&d_pool_, pool_words_ * sizeof(uint32_t)));
+  dev_bytes_ += pool_words_ * sizeof(uint32_t);
 
   if (!words.empty())
     HC(hipMemcpyAsync(d_pool_, words.data(), words.size() * 4, hipMemcpyHostToDevice, stream_));
@@ -472,21 +514,38 @@ void Job::run_fused(std::vector<hipEvent_t>& ev_eval, bool device_driven) {
     HC(launch_scan(d_ntile_, d_tilebase_, ng, d_part_, d_scal_ + S_TOT_A, nullptr, stream_));
     HC(launch_scan(d_bound_, d_scrbase_, ng, d_part_, d_scal_ + S_TOT_B, nullptr, stream_));
     uint64_t ntiles = tiles_cap_;
+    // The level scratch lives in the unused top of the event pool when twice
+    // its size is free above the bump (the level's outputs, at most the
+    // scratch size, then land below it); otherwise in an allocation of its
+    // own.  The pool guard keeps the outputs clear of a scratch in the pool.
+    uint32_t* scratch = d_scratch_;
+    uint64_t pool_limit = pool_words_;
     if (!device_driven) {
       uint64_t h[3];
       HC(hipMemcpyAsync(h, d_scal_, 3 * sizeof(uint64_t), hipMemcpyDeviceToHost, stream_));
       HC(hipStreamSynchronize(stream_));
       ntiles = h[S_TOT_A];
-      const uint64_t scr = h[S_TOT_B];
+      const uint64_t scr = h[S_TOT_B], used = h[S_BUMP];
       if (ntiles >= 0xFFFFFFF0ull) throw GlsError(GLS_E_RANGE, "too many tiles in one level");
       ensure((void**)&d_tiles_, cap_tiles_, ntiles * sizeof(TileDesc));
       ensure((void**)&d_tout_, cap_tout_, ntiles * sizeof(TileOut));
       ensure((void**)&d_toff_, cap_toff_, ntiles * 4);
-      ensure((void**)&d_scratch_, cap_scratch_, scr * 4);
+      grow_pool(used + 2 * scr, used);
+      if (used <= pool_words_ && 2 * scr <= pool_words_ - used) {
+        pool_limit = pool_words_ - scr;
+        scratch = d_pool_ + pool_limit;
+      } else {
+        ensure((void**)&d_scratch_, cap_scratch_, scr * 4);
+        scratch = d_scratch_;
+      }
       max_tiles_ = std::max(max_tiles_, ntiles);
       max_scratch_ = std::max(max_scratch_, scr);
     } else {
       HC(launch_guard(d_scal_, tiles_cap_, scratch_cap_, stream_));
+      if (graph_scratch_in_pool_) {
+        pool_limit = pool_words_ - scratch_cap_;
+        scratch = d_pool_ + pool_limit;
+      }
     }
     HC(launch_tile_desc(gates, ng, d_plan_, d_tilebase_, d_woff_, d_wlen_, d_pool_, prog_->d_lutp,
                         d_tiles_, d_scal_ + S_TOT_A, ntiles, stream_));
@@ -498,7 +557,7 @@ void Job::run_fused(std::vector<hipEvent_t>& ev_eval, bool device_driven) {
     a.luts = prog_->d_luts;
     a.ntiles = d_scal_ + S_TOT_A;
     a.scr_base = d_scrbase_;
-    a.scratch = d_scratch_;
+    a.scratch = scratch;
     a.tout = d_tout_;
     a.ovf = d_ovf_;
     a.ovf_cap = ovf_words_;
@@ -531,8 +590,8 @@ void Job::run_fused(std::vector<hipEvent_t>& ev_eval, bool device_driven) {
     HC(launch_gate_carry(gates, ng, d_tilebase_, d_ntile_, d_tout_, d_toff_, d_bound_, d_wlen_,
                          d_saif_, d_win_, (const unsigned int*)(d_scal_ + S_ERR), stream_));
     HC(launch_scan(d_bound_, d_outoff_, ng, d_part_, d_scal_ + S_TOT_OUT, bump, stream_));
-    HC(launch_pool_guard(d_scal_, pool_words_, stream_));
-    HC(launch_compact(d_tiles_, d_scal_ + S_TOT_A, ntiles, d_tout_, d_toff_, d_scrbase_, d_scratch_,
+    HC(launch_pool_guard(d_scal_, pool_limit, stream_));
+    HC(launch_compact(d_tiles_, d_scal_ + S_TOT_A, ntiles, d_tout_, d_toff_, d_scrbase_, scratch,
                       d_outoff_, d_pool_, d_woff_, d_wlen_, (const unsigned int*)(d_scal_ + S_ERR),
                       stream_));
     HC(launch_bump(bump, d_scal_ + S_TOT_OUT, stream_));
@@ -559,7 +618,7 @@ void Job::run_two_pass(std::vector<hipEvent_t>& ev_eval) {
     ensure((void**)&d_carry_, cap_carry_, nsegs * sizeof(SegCarry));
     ensure((void**)&d_scratch_, cap_scratch_, nscr * 4);
     // the level's output is at most its scratch size; refuse before writing
-    if (used + nscr > pool_words_)
+    if (!grow_pool(used + nscr, used))
       throw GlsError(GLS_E_OOM, "event pool exhausted at level " + std::to_string(l) +
                                     " (raise the pool size or shorten the window)");
     HC(launch_segmap(ng, d_segbase_, d_gnseg_, d_segmap_, stream_));
@@ -678,7 +737,11 @@ void Job::build_graph() {
   ensure((void**)&d_tiles_, cap_tiles_, tiles_cap_ * sizeof(TileDesc));
   ensure((void**)&d_tout_, cap_tout_, tiles_cap_ * sizeof(TileOut));
   ensure((void**)&d_toff_, cap_toff_, tiles_cap_ * 4);
-  ensure((void**)&d_scratch_, cap_scratch_, scratch_cap_ * 4);
+  // the replay's bump sequence is the eager run's: a scratch at the pool top
+  // stays clear of every level's outputs when it fits above the final bump
+  const uint64_t used = stats_.pool_words_used;
+  graph_scratch_in_pool_ = used <= pool_words_ && scratch_cap_ <= pool_words_ - used;
+  if (!graph_scratch_in_pool_) ensure((void**)&d_scratch_, cap_scratch_, scratch_cap_ * 4);
   hipGraph_t g = nullptr;
   HC(hipStreamBeginCapture(stream_, hipStreamCaptureModeRelaxed));
   try {
@@ -711,6 +774,7 @@ void Job::finish_stats(const uint64_t* h, double ms) {
   stats_.kernel_ms = ms;
   stats_.n_events = h[S_EVC];
   stats_.pool_words_used = h[S_BUMP];
+  stats_.pool_words = pool_words_;
   // algorithmic bytes: every input waveform read once per reading gate, every
   // output written once, 3 u64 SAIF counters per net, CSR metadata
   std::vector<uint32_t> wlen(n_nets_);

@@ -35,7 +35,8 @@ class SimOpts(ctypes.Structure):
     _fields_ = [("pool_bytes", ctypes.c_uint64), ("seg_events", ctypes.c_uint32),
                 ("keep_waves", ctypes.c_uint32), ("path", ctypes.c_uint32),
                 ("level_blocks", ctypes.c_uint32), ("no_prefetch", ctypes.c_uint32),
-                ("graph", ctypes.c_uint32), ("hbm_stacks", ctypes.c_uint32)]
+                ("graph", ctypes.c_uint32), ("hbm_stacks", ctypes.c_uint32),
+                ("pool_grow", ctypes.c_uint32)]
 
 
 PATH_FUSED = 0
@@ -48,7 +49,8 @@ class RunStats(ctypes.Structure):
                 ("n_nets", ctypes.c_uint64), ("n_gates", ctypes.c_uint64),
                 ("n_pins", ctypes.c_uint64), ("kernel_ms", ctypes.c_double),
                 ("wall_ms", ctypes.c_double), ("eval_ms", ctypes.c_double),
-                ("eval_launches", ctypes.c_uint64), ("level_blocks", ctypes.c_uint64)]
+                ("eval_launches", ctypes.c_uint64), ("level_blocks", ctypes.c_uint64),
+                ("pool_bytes", ctypes.c_uint64), ("pool_used_bytes", ctypes.c_uint64)]
 
 
 TUNE_MAX = 16
@@ -381,22 +383,24 @@ class Context(_Handle):
 
     @staticmethod
     def _opts(pool_bytes, seg_events, keep_waves, path, level_blocks=0, prefetch=True,
-              graph=False, hbm_stacks=False):
+              graph=False, hbm_stacks=False, pool_grow=False):
         return SimOpts(pool_bytes, seg_events, 1 if keep_waves else 0, path, level_blocks,
-                       0 if prefetch else 1, 1 if graph else 0, 1 if hbm_stacks else 0)
+                       0 if prefetch else 1, 1 if graph else 0, 1 if hbm_stacks else 0,
+                       1 if pool_grow else 0)
 
     def prepare(self, design, stim, dumpon, dumpoff, pool_bytes=0, seg_events=0, keep_waves=False,
                 path=PATH_FUSED, level_blocks=0, prefetch=True, graph=False,
-                hbm_stacks=False):
+                hbm_stacks=False, pool_grow=False):
         h = ctypes.c_void_p()
-        o = self._opts(pool_bytes, seg_events, keep_waves, path, level_blocks, prefetch, graph, hbm_stacks)
+        o = self._opts(pool_bytes, seg_events, keep_waves, path, level_blocks, prefetch, graph,
+                       hbm_stacks, pool_grow)
         _check(load().gls_prepare(self.ptr, design.ptr, stim.ptr, dumpon, dumpoff, ctypes.byref(o),
                                   ctypes.byref(h)))
         return Run(h.value)
 
     def prepare_batch(self, designs, stims, dumpons, dumpoffs, pool_bytes=0, seg_events=0,
                       keep_waves=False, level_blocks=0, prefetch=True,
-                      graph=False, hbm_stacks=False):
+                      graph=False, hbm_stacks=False, pool_grow=False):
         n = len(designs)
         assert len(stims) == n and len(dumpons) == n and len(dumpoffs) == n
         D = (ctypes.c_void_p * n)(*[d.ptr for d in designs])
@@ -405,16 +409,17 @@ class Context(_Handle):
         off = (_U64 * n)(*dumpoffs)
         h = ctypes.c_void_p()
         o = self._opts(pool_bytes, seg_events, keep_waves, PATH_FUSED, level_blocks, prefetch,
-                       graph, hbm_stacks)
+                       graph, hbm_stacks, pool_grow)
         _check(load().gls_prepare_batch(self.ptr, n, D, S, on, off, ctypes.byref(o),
                                         ctypes.byref(h)))
         return Run(h.value)
 
     def simulate(self, design, stim, dumpon, dumpoff, pool_bytes=0, seg_events=0, keep_waves=False,
                  path=PATH_FUSED, level_blocks=0, prefetch=True, graph=False,
-                 hbm_stacks=False):
+                 hbm_stacks=False, pool_grow=False):
         h = ctypes.c_void_p()
-        o = self._opts(pool_bytes, seg_events, keep_waves, path, level_blocks, prefetch, graph, hbm_stacks)
+        o = self._opts(pool_bytes, seg_events, keep_waves, path, level_blocks, prefetch, graph,
+                       hbm_stacks, pool_grow)
         _check(load().gls_simulate(self.ptr, design.ptr, stim.ptr, dumpon, dumpoff, ctypes.byref(o),
                                    ctypes.byref(h)))
         return Run(h.value)

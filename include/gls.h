@@ -79,7 +79,8 @@ int gls_ctx_create(int device, gls_ctx** out);
 void gls_ctx_destroy(gls_ctx* c);
 
 typedef struct {
-  uint64_t pool_bytes;   /* event pool size (0: 85% of free HBM) */
+  uint64_t pool_bytes;   /* event pool size, fixed (0: sized from the stimulus and grown
+                            between levels as needed, up to the free HBM) */
   uint32_t seg_events;   /* target input events per time segment (0: path default) */
   uint32_t keep_waves;   /* 1: keep waveforms for gls_run_waveform */
   uint32_t path;         /* GLS_PATH_FUSED (default) or GLS_PATH_TWO_PASS */
@@ -90,6 +91,7 @@ typedef struct {
                             between levels, and later executions replay it */
   uint32_t hbm_stacks;   /* 1: the fused kernel keeps its preemption stacks in a per-workgroup
                             HBM scratch instead of LDS (half the LDS, more resident workgroups) */
+  uint32_t pool_grow;    /* 1: pool_bytes is only the starting size; the pool grows as with 0 */
 } gls_sim_opts;
 
 /* Level-kernel variants: one fused LDS-tiled kernel per level, or the
@@ -121,6 +123,8 @@ typedef struct {
   double eval_ms;      /* summed device time of the gate-evaluation kernel */
   uint64_t eval_launches;
   uint64_t level_blocks; /* workgroups per fused level launch */
+  uint64_t pool_bytes;      /* event pool capacity after the run */
+  uint64_t pool_used_bytes; /* pool bytes holding waveforms at the end of the run */
 } gls_run_stats;
 int gls_run_stats_get(const gls_run* r, gls_run_stats* st);
 /* SAIF writer (DumpSaif, src/simulate.cu:113-481 / src/simulate.cpp:37-380). */

@@ -144,6 +144,31 @@ def test_repeated_execute_is_identical(ctx):
     assert hashes == {int(GOLDEN["c0_small"]["waves_hash"], 16)}
 
 
+def test_pool_grows_between_levels(ctx):
+    # a 4 KiB starting pool that may grow: doubled between levels (the
+    # waveforms so far copied over) until every level fits, same results
+    d, s = golden_cases.make("c1_small")
+    start, end = golden_cases.CASES["c1_small"]["window"]
+    g = GOLDEN["c1_small"]
+    for path in (gls.PATH_FUSED, gls.PATH_TWO_PASS):
+        say("pool growth", path)
+        r = ctx.simulate(d, s, start, end, pool_bytes=4096, pool_grow=True, keep_waves=True,
+                         path=path)
+        st = r.stats()
+        assert st["pool_bytes"] > 4096 and st["pool_used_bytes"] <= st["pool_bytes"]
+        assert "%016x" % r.waves_hash() == g["waves_hash"], path
+        assert oracle_bind.sha256(r.saif_text()) == g["saif_sha256"], path
+        r.close()
+    # growth during the eager run that precedes the capture; replays reuse
+    # the grown pool
+    r = ctx.prepare(d, s, start, end, pool_bytes=4096, pool_grow=True, keep_waves=True, graph=True)
+    for i in range(3):
+        r.execute()
+        assert "%016x" % r.waves_hash() == g["waves_hash"], i
+    assert r.stats()["eval_launches"] == 1
+    r.close()
+
+
 def test_pool_exhaustion_is_reported(ctx):
     # the pool is clamped to the stimulus plus 1024 words: level 0 cannot fit
     d, s = golden_cases.make("c0_small")
