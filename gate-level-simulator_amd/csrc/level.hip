@@ -136,7 +136,8 @@ __device__ __forceinline__ void saif_acc(unsigned long long* acc, uint32_t v, un
   acc[2] += (v == 1 || v == 2) ? len : 0ull;
 }
 
-__device__ __forceinline__ uint32_t lb_words(const uint32_t* a, uint32_t n, uint32_t key) {
+template <typename P>
+__device__ __forceinline__ uint32_t lb_words(P a, uint32_t n, uint32_t key) {
   uint32_t lo = 0, hi = n;
   while (lo < hi) {
     uint32_t mid = (lo + hi) >> 1;
@@ -146,17 +147,24 @@ __device__ __forceinline__ uint32_t lb_words(const uint32_t* a, uint32_t n, uint
   return lo;
 }
 
+typedef __attribute__((address_space(3))) uint32_t lds_u32;
+
+// Pointer to a tile's input ranges: LDS (staged) or HBM (too long for LDS).
+// The LDS form is a 32-bit local pointer, so its reads are ds_read.
+template <bool LDS> struct InPtr { typedef const uint32_t* T; };
+template <> struct InPtr<true> { typedef const lds_u32* T; };
+
 // Uniform (per tile) data of one gate.
-template <int K>
+template <int K, bool LDS>
 struct TileCtx {
-  const uint32_t* src[K];  // input ranges (LDS or HBM), index relative to B
+  typename InPtr<LDS>::T src[K];  // input ranges, index relative to B
   uint32_t B[K], E[K], L[K];
   uint32_t nseg, stride, lead, dmax;
   uint32_t lut[4];          // packed truth table, fan-in <= 3
 };
 
-template <int K>
-__device__ __forceinline__ uint32_t lut_eval(const TileCtx<K>& X, const uint8_t* s_lut,
+template <int K, bool LDS>
+__device__ __forceinline__ uint32_t lut_eval(const TileCtx<K, LDS>& X, const uint8_t* s_lut,
                                              uint32_t idx) {
   if (K <= 2) return (X.lut[0] >> (2 * idx)) & 3u;
   if (K == 3) {
@@ -183,19 +191,24 @@ __device__ __forceinline__ uint32_t fell_bits(uint32_t p, uint32_t c) {
 }
 
 // Remove the top entry of a lane's stack, keeping the count of value changes.
-__device__ __forceinline__ void stack_pop(const uint32_t* st, uint32_t& sp, uint32_t& top,
+template <typename SP>
+__device__ __forceinline__ void stack_pop(SP st, uint32_t& sp, uint32_t& top,
                                           uint32_t& cnt) {
   --sp;
-  const uint32_t nt = sp ? st[sp - 1] : 0u;
+  // st[0] when the stack empties: the entry just removed, in bounds, unused
+  const uint32_t below = st[sp ? sp - 1 : 0u];
+  const uint32_t nt = sp ? below : 0u;
   cnt -= (sp == 0 || ((nt ^ top) & 3u)) ? 1u : 0u;
   top = nt;
 }
 
 // What one lane's walk over its segment leaves behind: its stack of kept
 // candidates (st[0..sp)), the top entry, the number of value changes in the
-// stack, and the number of gate events it owned.
+// stack, and the number of gate events it owned.  SP: pointer to the stack
+// (LDS or HBM).
+template <typename SP>
 struct Lane {
-  uint32_t* st;
+  SP st;
   uint32_t sp, top, cnt, nown;
 };
 
@@ -213,8 +226,12 @@ __device__ __forceinline__ void wait_vm() { __builtin_amdgcn_s_waitcnt(0x0F70); 
 __device__ __forceinline__ uint32_t range_start(uint32_t L) { return L > 0 ? L - 1 : 0u; }
 
 __device__ __forceinline__ uint32_t tile_words(const TileDesc& D) {
+  // loops over the fan-in run to 4 with constant indices so a descriptor
+  // held in registers is never indexed at run time
   uint32_t total = 0;
-  for (uint32_t j = 0; j < D.k; ++j) total += D.E[j] - range_start(D.L[j]);
+#pragma unroll
+  for (uint32_t j = 0; j < 4; ++j)
+    if (j < D.k) total += D.E[j] - range_start(D.L[j]);
   return total;
 }
 
@@ -226,7 +243,9 @@ __device__ __forceinline__ void stage_tile(const LevelArgs& A, const TileDesc& D
   const uint32_t lane = threadIdx.x;
   if (fits) {
     uint32_t pos = 0;
-    for (uint32_t j = 0; j < D.k; ++j) {
+#pragma unroll
+    for (uint32_t j = 0; j < 4; ++j) {
+      if (j >= D.k) break;
       const uint32_t n = D.E[j] - range_start(D.L[j]);
       const uint32_t* src = A.pool + D.in_off[j];
       for (uint32_t i = 0; i < n; i += kLanes)
@@ -243,15 +262,16 @@ __device__ __forceinline__ void stage_tile(const LevelArgs& A, const TileDesc& D
 
 // One lane's walk over its time segment: merge the inputs in time order,
 // evaluate, pick the delay, keep the preemption stack.
-template <int K, bool LDS>
-__device__ __forceinline__ Lane lane_walk(const TileCtx<K>& X, uint32_t ti, const uint8_t* s_lut,
-                                          const uint16_t* s_dt, uint32_t* stk_base) {
+template <int K, bool LDS, typename SP>
+__device__ __forceinline__ Lane<SP> lane_walk(const TileCtx<K, LDS>& X, uint32_t ti,
+                                              const uint8_t* s_lut, const uint16_t* s_dt,
+                                              SP stk_base) {
   const int lane = threadIdx.x;
   const uint32_t c = ti * kLanes + lane;
   const uint32_t kX = 1u;
   const uint32_t INF = 0xFFFFFFFFu;
   uint32_t r[K], h[K], bound[K];
-  const uint32_t* srcL = X.src[0];
+  typename InPtr<LDS>::T srcL = X.src[0];
   uint32_t BL = X.B[0];
 #pragma unroll
   for (int j = 0; j < K; ++j) {
@@ -261,7 +281,7 @@ __device__ __forceinline__ Lane lane_walk(const TileCtx<K>& X, uint32_t ti, cons
       BL = X.B[j];
     }
   }
-  Lane ln;
+  Lane<SP> ln;
   ln.st = stk_base;
   // sp: stack depth, top: top entry, cnt: value changes in the stack
   uint32_t sp = 0, top = 0, cnt = 0, nown = 0, hmin = INF;
@@ -287,7 +307,7 @@ __device__ __forceinline__ Lane lane_walk(const TileCtx<K>& X, uint32_t ti, cons
       // LDS ranges end with an INF sentinel
       h[j] = (LDS || r[j] < bound[j]) ? X.src[j][r[j]] : INF;
     }
-    uint32_t* st = stk_base + soff;
+    SP st = stk_base + soff;
     ln.st = st;
     uint32_t fprev = c == 0 ? kX : lut_eval<K>(X, s_lut, idx);
     if (c == 0) {  // every output starts with (0, X) (src/simulate.cpp:882-887)
@@ -345,36 +365,57 @@ __device__ __forceinline__ Lane lane_walk(const TileCtx<K>& X, uint32_t ti, cons
   return ln;
 }
 
-// Tile context and walk for fan-in K.  Tiles that do not fit LDS read their
-// inputs from HBM and keep their stacks in the overflow buffer.
-template <int K>
-__device__ __forceinline__ Lane walk_k(const LevelArgs& A, const TileDesc& D, uint64_t tk,
-                                       bool fits, const uint32_t* s_in, uint32_t* s_st,
-                                       const uint8_t* s_lut, const uint16_t* s_dt) {
+// Tile context and walk for fan-in K, then `finish` with the lanes' stacks.
+// Tiles that fit LDS read their inputs there (LDS pointers) and keep their
+// stacks in LDS or in this workgroup's slice of the stack scratch; tiles that
+// do not fit read their inputs from HBM and keep their stacks in the
+// overflow buffer.
+template <int K, typename F>
+__device__ __forceinline__ void walk_k(const LevelArgs& A, const TileDesc& D, bool fits,
+                                       uint32_t* s_in, uint32_t* s_st, const uint8_t* s_lut,
+                                       const uint16_t* s_dt, F&& finish) {
   const int lane = threadIdx.x;
-  TileCtx<K> X;
-  X.nseg = D.nseg;
-  X.stride = D.stride;
-  X.lead = D.lead;
-  X.dmax = D.dmax;
-#pragma unroll
-  for (int q = 0; q < 4; ++q) X.lut[q] = D.lutw[q];
-  uint32_t pos = 0, total = 0;
+  uint32_t B[K], E[K], L[K], pos[K], total = 0, p = 0;
 #pragma unroll
   for (int j = 0; j < K; ++j) {
-    X.L[j] = D.L[j];
-    X.E[j] = D.E[j];
-    X.B[j] = range_start(D.L[j]);
-    const uint32_t n = X.E[j] - X.B[j];
-    X.src[j] = fits ? s_in + pos : A.pool + D.in_off[j];
-    pos += n + 1;
-    total += n;
+    L[j] = D.L[j];
+    E[j] = D.E[j];
+    B[j] = range_start(L[j]);
+    pos[j] = p;
+    p += E[j] - B[j] + 1;
+    total += E[j] - B[j];
   }
+  auto fill = [&](auto& X) {
+    X.nseg = D.nseg;
+    X.stride = D.stride;
+    X.lead = D.lead;
+    X.dmax = D.dmax;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) X.lut[q] = D.lutw[q];
+#pragma unroll
+    for (int j = 0; j < K; ++j) {
+      X.L[j] = L[j];
+      X.E[j] = E[j];
+      X.B[j] = B[j];
+    }
+  };
   if (fits) {
-    // stacks in LDS, or in this workgroup's slice of the stack scratch
-    if (A.stk) return lane_walk<K, true>(X, D.ti, s_lut, s_dt, A.stk + (uint64_t)blockIdx.x * (A.cap + kLanes));
-    return lane_walk<K, true>(X, D.ti, s_lut, s_dt, s_st);
+    TileCtx<K, true> X;
+    fill(X);
+    const lds_u32* in = (const lds_u32*)s_in;
+#pragma unroll
+    for (int j = 0; j < K; ++j) X.src[j] = in + pos[j];
+    if (A.stk) {
+      finish(lane_walk<K, true>(X, D.ti, s_lut, s_dt, A.stk + (uint64_t)blockIdx.x * (A.cap + kLanes)));
+    } else {
+      finish(lane_walk<K, true>(X, D.ti, s_lut, s_dt, (lds_u32*)s_st));
+    }
+    return;
   }
+  TileCtx<K, false> X;
+  fill(X);
+#pragma unroll
+  for (int j = 0; j < K; ++j) X.src[j] = A.pool + D.in_off[j];
   unsigned long long base = 0;
   const unsigned long long need = (unsigned long long)total + kLanes;
   if (lane == 0) base = atomicAdd(A.ovf_used, need);
@@ -384,13 +425,14 @@ __device__ __forceinline__ Lane walk_k(const LevelArgs& A, const TileDesc& D, ui
     if (lane == 0) atomicOr(A.err, ERR_OVERFLOW);
     base = 0;  // results are invalid (the error is reported); keep every wavefront moving
     if (need > A.ovf_cap) {
-      Lane ln;
+      Lane<uint32_t*> ln;
       ln.st = A.ovf;
       ln.sp = ln.top = ln.cnt = ln.nown = 0;
-      return ln;
+      finish(ln);
+      return;
     }
   }
-  return lane_walk<K, false>(X, D.ti, s_lut, s_dt, A.ovf + base);
+  finish(lane_walk<K, false>(X, D.ti, s_lut, s_dt, A.ovf + base));
 }
 
 // Combine the lanes' stacks into the tile's output: wavefront scan of the
@@ -409,7 +451,8 @@ struct TileSum {
   uint32_t ft;       // time of the tile's first entry
 };
 
-__device__ __forceinline__ TileSum combine_tile(const Lane& ln) {
+template <typename SP>
+__device__ __forceinline__ TileSum combine_tile(const Lane<SP>& ln) {
   const uint32_t sp = ln.sp;
   const uint32_t e_first = sp ? ln.st[0] : 0u;
   Agg mine;
@@ -432,13 +475,14 @@ __device__ __forceinline__ TileSum combine_tile(const Lane& ln) {
 // SAIF durations between them (32 bits per lane: times < 2^30; the lanes'
 // intervals are disjoint pieces of the tile's span, so the sums fit too) and
 // write the tile summary.
-__device__ __forceinline__ void emit_tile(const LevelArgs& A, uint64_t tk, const Lane& ln,
+template <typename SP>
+__device__ __forceinline__ void emit_tile(const LevelArgs& A, uint64_t tk, const Lane<SP>& ln,
                                           const TileSum& T, uint32_t* dst,
                                           unsigned long long w_start, unsigned long long w_end,
                                           unsigned long long& wave_ev) {
   const int lane = threadIdx.x;
   const uint32_t sp = ln.sp;
-  const uint32_t* st = ln.st;
+  const SP st = ln.st;
   const Agg& carry = T.carry;
   const uint32_t s32 = (uint32_t)min(w_start, 0x7FFFFFFFull);
   const uint32_t e32 = (uint32_t)min(w_end, 0x7FFFFFFFull);
@@ -478,12 +522,32 @@ __device__ __forceinline__ void emit_tile(const LevelArgs& A, uint64_t tk, const
   }
 }
 
+// A tile descriptor is read with one coalesced load (lane i < 32 holds
+// dword i) and broadcast to every lane with readlane.  The fields then live
+// in registers: no per-field loads (whose waits, s_waitcnt vmcnt, would also
+// wait for the staging DMA in flight) on the walk's path.
+__device__ __forceinline__ uint32_t desc_word(const TileDesc* p) {
+  const uint32_t lane = threadIdx.x;
+  return lane < 32 ? ((const uint32_t*)p)[lane] : 0u;
+}
+
+__device__ __forceinline__ TileDesc desc_of(uint32_t w) {
+  uint32_t f[32];
+#pragma unroll
+  for (int i = 0; i < 32; ++i) f[i] = __builtin_amdgcn_readlane(w, i);
+  TileDesc D;
+  __builtin_memcpy(&D, f, sizeof D);
+  return D;
+}
+
 // Scratch word of a tile's output region: its gate's region, then the
 // tile's first own event in the gate's merged input order plus one slack
 // word per earlier tile (a tile keeps at most its own events + 1 entries).
 __device__ __forceinline__ uint64_t tile_scratch(const uint64_t* scr_base, const TileDesc& D) {
   uint64_t p = scr_base[D.g] + D.ti;
-  for (uint32_t j = 0; j < D.k; ++j) p += D.L[j];
+#pragma unroll
+  for (uint32_t j = 0; j < 4; ++j)
+    if (j < D.k) p += D.L[j];
   return p;
 }
 
@@ -506,16 +570,21 @@ __global__ __launch_bounds__(kLanes) void k_level(LevelArgs A) {
   const uint64_t ntiles = *A.ntiles;  // uniform: one scalar load
   uint64_t tk = blockIdx.x;
   bool fits = false;
+  uint32_t dw = 0;  // descriptor of tile tk, one dword per lane
   if (tk < ntiles) {
-    fits = tile_words(A.tiles[tk]) <= A.cap;
-    stage_tile(A, A.tiles[tk], fits, s_in, s_lut, s_dt);
+    dw = desc_word(A.tiles + tk);
+    const TileDesc D0 = desc_of(dw);
+    fits = tile_words(D0) <= A.cap;
+    stage_tile(A, D0, fits, s_in, s_lut, s_dt);
   }
   while (tk < ntiles) {
-    const TileDesc& D = A.tiles[tk];
+    const TileDesc D = desc_of(dw);
     wait_vm();
     if (fits) {  // INF sentinel after every input range
       uint32_t pos = 0;
-      for (uint32_t j = 0; j < D.k; ++j) {
+#pragma unroll
+      for (uint32_t j = 0; j < 4; ++j) {
+        if (j >= D.k) break;
         pos += D.E[j] - range_start(D.L[j]);
         if ((uint32_t)lane == j) s_in[pos] = 0xFFFFFFFFu;
         ++pos;
@@ -525,30 +594,36 @@ __global__ __launch_bounds__(kLanes) void k_level(LevelArgs A) {
     // the emit's scalars, loaded now so the walk covers their latency
     uint32_t* dst = A.scratch + tile_scratch(A.scr_base, D);
     const unsigned long long w_start = A.win[2 * D.job], w_end = A.win[2 * D.job + 1];
-    Lane ln;
-    switch (D.k) {
-      case 1: ln = walk_k<1>(A, D, tk, fits, s_in, s_st, lut8, dt16); break;
-      case 2: ln = walk_k<2>(A, D, tk, fits, s_in, s_st, lut8, dt16); break;
-      case 3: ln = walk_k<3>(A, D, tk, fits, s_in, s_st, lut8, dt16); break;
-      default: ln = walk_k<4>(A, D, tk, fits, s_in, s_st, lut8, dt16); break;
-    }
-    __syncthreads();  // inputs, truth and delay tables no longer read
-    const TileSum T = combine_tile(ln);
     const uint64_t tn = tk + gridDim.x;
+    // the next descriptor's load completes during the walk
+    const uint32_t dw_n = tn < ntiles ? desc_word(A.tiles + tn) : 0u;
     bool fits_n = false;
-    if (A.prefetch && tn < ntiles) {  // stage the next tile while this one is stored
-      fits_n = tile_words(A.tiles[tn]) <= A.cap;
-      stage_tile(A, A.tiles[tn], fits_n, s_in, s_lut, s_dt);
+    auto finish = [&](const auto& ln) {
+      __syncthreads();  // inputs, truth and delay tables no longer read
+      const TileSum T = combine_tile(ln);
+      if (A.prefetch && tn < ntiles) {  // stage the next tile while this one is stored
+        const TileDesc Dn = desc_of(dw_n);
+        fits_n = tile_words(Dn) <= A.cap;
+        stage_tile(A, Dn, fits_n, s_in, s_lut, s_dt);
+      }
+      emit_tile(A, tk, ln, T, dst, w_start, w_end, wave_ev);
+    };
+    switch (D.k) {
+      case 1: walk_k<1>(A, D, fits, s_in, s_st, lut8, dt16, finish); break;
+      case 2: walk_k<2>(A, D, fits, s_in, s_st, lut8, dt16, finish); break;
+      case 3: walk_k<3>(A, D, fits, s_in, s_st, lut8, dt16, finish); break;
+      default: walk_k<4>(A, D, fits, s_in, s_st, lut8, dt16, finish); break;
     }
-    emit_tile(A, tk, ln, T, dst, w_start, w_end, wave_ev);
     if (!A.prefetch && tn < ntiles) {
       __syncthreads();
-      fits_n = tile_words(A.tiles[tn]) <= A.cap;
-      stage_tile(A, A.tiles[tn], fits_n, s_in, s_lut, s_dt);
+      const TileDesc Dn = desc_of(dw_n);
+      fits_n = tile_words(Dn) <= A.cap;
+      stage_tile(A, Dn, fits_n, s_in, s_lut, s_dt);
     }
     __syncthreads();
     tk = tn;
     fits = fits_n;
+    dw = dw_n;
   }
   if (lane == 0 && wave_ev) atomicAdd(A.ev_count, wave_ev);
 }
