@@ -76,6 +76,11 @@ void hip_check(hipError_t e, const char* what) {
 
 #define HC(x) hip_check((x), #x)
 
+// Words allocated past the end of the event pool: the fused kernel stages
+// input ranges with 16-byte copies that may read up to 3 words beyond a
+// range's last word.
+static constexpr uint64_t kPoolPad = 16;
+
 // scalar slots in d_scal_
 enum { S_BUMP = 0, S_TOT_A = 1, S_TOT_B = 2, S_TOT_OUT = 3, S_EVC = 4, S_TICKET = 5,
        S_OVF_USED = 6, S_ERR = 7 };
@@ -349,7 +354,7 @@ bool Job::grow_pool(uint64_t need, uint64_t used) {
   if (n > room) n = need;
   if (n > room) return false;
   uint32_t* p = nullptr;
-  if (hipMalloc((void**)&p, n * sizeof(uint32_t)) != hipSuccess) {
+  if (hipMalloc((void**)&p, (n + kPoolPad) * sizeof(uint32_t)) != hipSuccess) {
     (void)hipGetLastError();
     return false;
   }
@@ -465,8 +470,8 @@ void Job::prepare(const std::vector<SimInstance>& inst, uint64_t pool_words) {
   }
   if (pool_words < stim_words_ + 1024) pool_words = stim_words_ + 1024;
   pool_words_ = pool_words;
-  HC(hipMalloc((void**)&d_pool_, pool_words_ * sizeof(uint32_t)));
-  dev_bytes_ += pool_words_ * sizeof(uint32_t);
+  HC(hipMalloc((void**)&d_pool_, (pool_words_ + kPoolPad) * sizeof(uint32_t)));
+  dev_bytes_ += (pool_words_ + kPoolPad) * sizeof(uint32_t);
 
   if (!words.empty())
     HC(hipMemcpyAsync(d_pool_, words.data(), words.size() * 4, hipMemcpyHostToDevice, stream_));

@@ -219,6 +219,12 @@ __device__ __forceinline__ void dma_word(const uint32_t* g, uint32_t* l) {
   __builtin_amdgcn_global_load_lds(g, (lds_void*)l, 4, 0, 0);
 }
 
+// One 16-byte direct-to-LDS load per lane (gfx950): lane i's four words
+// land at l + 4 i.  g is 16-byte aligned.
+__device__ __forceinline__ void dma_x4(const uint32_t* g, uint32_t* l) {
+  __builtin_amdgcn_global_load_lds(g, (lds_void*)l, 16, 0, 0);
+}
+
 // Wait until this wavefront has no vector-memory access in flight (direct-to-
 // LDS loads included): s_waitcnt vmcnt(0).
 __device__ __forceinline__ void wait_vm() { __builtin_amdgcn_s_waitcnt(0x0F70); }
@@ -235,22 +241,41 @@ __device__ __forceinline__ uint32_t tile_words(const TileDesc& D) {
   return total;
 }
 
-// Issue the loads of tile D into LDS: its input ranges (range j at
-// sum_{i<j}(n_i + 1), leaving one word for the sentinel) when they fit, its
-// delay table and (fan-in 4) its truth table.  Completion: wait_vm().
+// LDS layout of a tile's input ranges: range j (n_j words from pool word
+// G_j) starts at P_j = R_j + G_j mod 4 with R_j a multiple of 4 words, so it
+// is copied with 16-byte loads from the aligned word G_j - G_j mod 4;
+// R_{j+1} = the multiple of 4 at or above P_j + n_j + 1 keeps the sentinel
+// word after each range.  The layout needs at most sum(n_j) + 7 K words.
+__device__ __forceinline__ void range_layout(const TileDesc& D, uint32_t P[4]) {
+  uint32_t R = 0;
+#pragma unroll
+  for (uint32_t j = 0; j < 4; ++j) {
+    const uint32_t n = D.E[j] - range_start(D.L[j]);
+    P[j] = R + ((uint32_t)D.in_off[j] & 3u);
+    if (j < D.k) R = (P[j] + n + 4) & ~3u;
+  }
+}
+
+// Issue the loads of tile D into LDS: its input ranges when they fit (laid
+// out by range_layout; the 16-byte copies may also bring up to 3 words
+// before and after a range, inside its slot), its delay table and (fan-in 4)
+// its truth table.  Completion: wait_vm().
 __device__ __forceinline__ void stage_tile(const LevelArgs& A, const TileDesc& D, bool fits,
                                            uint32_t* s_in, uint32_t* s_lut, uint32_t* s_dt) {
   const uint32_t lane = threadIdx.x;
   if (fits) {
-    uint32_t pos = 0;
+    uint32_t P[4];
+    range_layout(D, P);
 #pragma unroll
     for (uint32_t j = 0; j < 4; ++j) {
       if (j >= D.k) break;
       const uint32_t n = D.E[j] - range_start(D.L[j]);
-      const uint32_t* src = A.pool + D.in_off[j];
-      for (uint32_t i = 0; i < n; i += kLanes)
-        if (i + lane < n) dma_word(src + i + lane, s_in + pos + i);
-      pos += n + 1;
+      const uint32_t a = (uint32_t)D.in_off[j] & 3u;
+      const uint32_t m = (a + n + 3) & ~3u;  // words copied, a multiple of 4
+      const uint32_t* src = A.pool + (D.in_off[j] - a);
+      uint32_t* l = s_in + (P[j] - a);
+      for (uint32_t i = 0; i < m; i += 4 * kLanes)
+        if (i + 4 * lane < m) dma_x4(src + i + 4 * lane, l + i);
     }
   }
   const uint32_t nw = dt_size(D.k) / 2;
@@ -375,14 +400,13 @@ __device__ __forceinline__ void walk_k(const LevelArgs& A, const TileDesc& D, bo
                                        uint32_t* s_in, uint32_t* s_st, const uint8_t* s_lut,
                                        const uint16_t* s_dt, F&& finish) {
   const int lane = threadIdx.x;
-  uint32_t B[K], E[K], L[K], pos[K], total = 0, p = 0;
+  uint32_t B[K], E[K], L[K], pos[4], total = 0;
+  range_layout(D, pos);
 #pragma unroll
   for (int j = 0; j < K; ++j) {
     L[j] = D.L[j];
     E[j] = D.E[j];
     B[j] = range_start(L[j]);
-    pos[j] = p;
-    p += E[j] - B[j] + 1;
     total += E[j] - B[j];
   }
   auto fill = [&](auto& X) {
@@ -491,19 +515,27 @@ __device__ __forceinline__ void emit_tile(const LevelArgs& A, uint64_t tk, const
   const uint32_t ct0 = pct;
   uint32_t a0 = 0, a1 = 0;
   uint32_t w = carry.has ? carry.cnt : 0u;  // lane's first slot in the tile output
-  for (uint32_t i = 0; i < sp; ++i) {
-    const uint32_t e = st[i];
-    const uint32_t v = e & 3u;
-    const uint32_t ct = max(min(e >> 2, e32), s32);
-    const uint32_t len = ct - pct;
-    a0 += pvv == 0u ? len : 0u;
-    a1 += pvv == 3u ? len : 0u;
-    if (v != pvv) {
-      dst[w] = e;
-      ++w;
+  // four stack reads in flight per step (indices clamped into the stack)
+  for (uint32_t i = 0; i < sp; i += 4) {
+    uint32_t e4[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) e4[q] = st[min(i + q, sp - 1)];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      if (i + q >= sp) break;
+      const uint32_t e = e4[q];
+      const uint32_t v = e & 3u;
+      const uint32_t ct = max(min(e >> 2, e32), s32);
+      const uint32_t len = ct - pct;
+      a0 += pvv == 0u ? len : 0u;
+      a1 += pvv == 3u ? len : 0u;
+      if (v != pvv) {
+        dst[w] = e;
+        ++w;
+      }
+      pvv = v;
+      pct = ct;
     }
-    pvv = v;
-    pct = ct;
   }
   const uint32_t t0 = wave_sum(a0), t1 = wave_sum(a1), tx = wave_sum(pct - ct0 - a0 - a1);
   const uint32_t ev = wave_sum(ln.nown);
@@ -581,13 +613,12 @@ __global__ __launch_bounds__(kLanes) void k_level(LevelArgs A) {
     const TileDesc D = desc_of(dw);
     wait_vm();
     if (fits) {  // INF sentinel after every input range
-      uint32_t pos = 0;
+      uint32_t P[4];
+      range_layout(D, P);
 #pragma unroll
       for (uint32_t j = 0; j < 4; ++j) {
         if (j >= D.k) break;
-        pos += D.E[j] - range_start(D.L[j]);
-        if ((uint32_t)lane == j) s_in[pos] = 0xFFFFFFFFu;
-        ++pos;
+        if ((uint32_t)lane == j) s_in[P[j] + D.E[j] - range_start(D.L[j])] = 0xFFFFFFFFu;
       }
     }
     __syncthreads();
