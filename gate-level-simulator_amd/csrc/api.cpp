@@ -331,6 +331,7 @@ static void apply_opts(Job& job, const gls_sim_opts* opts) {
   job.graph = opts->graph ? 1u : 0u;
   job.stack_mode = opts->hbm_stacks ? 1u : 0u;
   job.pool_grow = opts->pool_grow ? 1u : 0u;
+  job.ovf_words_opt = opts->ovf_words;
 }
 
 int gls_prepare_batch(gls_ctx* c, size_t n, const gls_design* const* designs,

@@ -65,7 +65,8 @@ struct alignas(16) TileDesc {
   uint32_t lutw[4];                // packed truth table words (fan-in <= 3)
   uint32_t g, ti, nseg, stride;    // gate (index in the level), tile of the gate, segments
   uint32_t lead, k, dt_off, lut;   // leader input, fan-in, delay table, truth table id
-  uint32_t dmax, out_net, job, pad;
+  uint32_t dmax, out_net, job;
+  uint32_t out_cap;  // words of the tile's output region in the level scratch
 };
 static_assert(sizeof(TileDesc) == 128, "TileDesc layout");
 

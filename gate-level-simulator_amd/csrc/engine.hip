@@ -776,6 +776,8 @@ void Job::finish_stats(const uint64_t* h, double ms) {
     throw GlsError(GLS_E_UNSUPPORTED, "a tile exceeded the overflow stack buffer (event density too skewed)");
   if (h[S_ERR] & 4u)
     throw GlsError(GLS_E_RANGE, "a gate's inputs carry 2^30 or more events");
+  if (h[S_ERR] & 32u)
+    throw GlsError(GLS_E_STATE, "a gate's input waveform was out of time order; results discarded");
   stats_.kernel_ms = ms;
   stats_.n_events = h[S_EVC];
   stats_.pool_words_used = h[S_BUMP];

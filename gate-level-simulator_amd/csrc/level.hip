@@ -35,13 +35,19 @@
 #include <algorithm>
 #include <cstdint>
 
+#include <type_traits>
+
 #include "device.h"
 
 namespace gls {
 
 constexpr int kLanes = 64;
 
-enum : unsigned { ERR_OVERFLOW = 2u, ERR_RANGE = 4u, ERR_CAPACITY = 8u, ERR_POOL = 16u };
+// ERR_ORDER: a bound that holds for time-ordered inputs was hit (a lane's
+// stack region or a tile's output region would have been overrun); the
+// write was dropped.
+enum : unsigned { ERR_OVERFLOW = 2u, ERR_RANGE = 4u, ERR_CAPACITY = 8u, ERR_POOL = 16u,
+                  ERR_ORDER = 32u };
 
 __device__ __forceinline__ uint32_t edge_cls(uint32_t pre, uint32_t cur) {
   // CheckDelay (src/simulate.cpp:1178-1201); codes 0=0 1=X 2=Z 3=1
@@ -287,10 +293,17 @@ __device__ __forceinline__ void stage_tile(const LevelArgs& A, const TileDesc& D
 
 // One lane's walk over its time segment: merge the inputs in time order,
 // evaluate, pick the delay, keep the preemption stack.
+// stk_words: size of the tile's stack buffer at stk_base.  A lane's stack
+// region runs from its offset to the next lane's; for time-ordered inputs a
+// lane never pushes more than its region holds.  Stacks in HBM are guarded
+// anyway: a push that would leave the buffer (inputs out of order)
+// overwrites the buffer's last word instead and sets bad.  LDS stacks are
+// not guarded (an LDS access outside the workgroup's allocation is
+// discarded by the hardware; all inputs are time-ordered by construction).
 template <int K, bool LDS, typename SP>
 __device__ __forceinline__ Lane<SP> lane_walk(const TileCtx<K, LDS>& X, uint32_t ti,
                                               const uint8_t* s_lut, const uint16_t* s_dt,
-                                              SP stk_base) {
+                                              SP stk_base, uint32_t stk_words, uint32_t& bad) {
   const int lane = threadIdx.x;
   const uint32_t c = ti * kLanes + lane;
   const uint32_t kX = 1u;
@@ -306,6 +319,7 @@ __device__ __forceinline__ Lane<SP> lane_walk(const TileCtx<K, LDS>& X, uint32_t
       BL = X.B[j];
     }
   }
+  constexpr bool kGuard = !std::is_same<SP, lds_u32*>::value;
   Lane<SP> ln;
   ln.st = stk_base;
   // sp: stack depth, top: top entry, cnt: value changes in the stack
@@ -313,13 +327,13 @@ __device__ __forceinline__ Lane<SP> lane_walk(const TileCtx<K, LDS>& X, uint32_t
   if (c < X.nseg) {
     // segment bounds from the leader
     uint32_t tau_c = 0, tau_n = 0x3FFFFFFFu, halo_end = 0x3FFFFFFFu;
+    uint32_t soff = lane, idx = 0;
     const uint32_t pl = c * X.stride;
     if (c > 0) tau_c = srcL[pl - BL] >> 2;
     if (c + 1 < X.nseg) {
       tau_n = srcL[pl + X.stride - BL] >> 2;
       halo_end = tau_n + X.dmax;
     }
-    uint32_t soff = lane, idx = 0;
 #pragma unroll
     for (int j = 0; j < K; ++j) {
       uint32_t pj;
@@ -332,7 +346,11 @@ __device__ __forceinline__ Lane<SP> lane_walk(const TileCtx<K, LDS>& X, uint32_t
       // LDS ranges end with an INF sentinel
       h[j] = (LDS || r[j] < bound[j]) ? X.src[j][r[j]] : INF;
     }
-    SP st = stk_base + soff;
+    // guarded stacks: the lane's entries stay below the buffer end (a lane
+    // whose offset is already past it keeps the buffer's last word)
+    const uint32_t s0 = kGuard ? min(soff, stk_words - 1) : soff;
+    const uint32_t stk_lim = stk_words - s0;
+    SP st = stk_base + s0;
     ln.st = st;
     uint32_t fprev = c == 0 ? kX : lut_eval<K>(X, s_lut, idx);
     if (c == 0) {  // every output starts with (0, X) (src/simulate.cpp:882-887)
@@ -372,9 +390,12 @@ __device__ __forceinline__ Lane<SP> lane_walk(const TileCtx<K, LDS>& X, uint32_t
           // preemption: drop queued entries at or after o (src/simulate.cpp:991-1007)
           while (sp > 0 && (top >> 2) >= o) stack_pop(st, sp, top, cnt);
           const uint32_t e = (o << 2) | f;
-          cnt += (sp == 0 || ((e ^ top) & 3u)) ? 1u : 0u;
-          st[sp++] = e;
-          top = e;
+          const bool room = !kGuard || sp < stk_lim;
+          bad |= room ? 0u : 1u;
+          cnt += ((sp == 0 || ((e ^ top) & 3u)) && room) ? 1u : 0u;
+          st[room ? sp : stk_lim - 1] = e;
+          sp += room ? 1u : 0u;
+          top = room ? e : top;
         } else {
           hmin = min(hmin, o);
         }
@@ -429,11 +450,14 @@ __device__ __forceinline__ void walk_k(const LevelArgs& A, const TileDesc& D, bo
     const lds_u32* in = (const lds_u32*)s_in;
 #pragma unroll
     for (int j = 0; j < K; ++j) X.src[j] = in + pos[j];
+    uint32_t bad = 0;
     if (A.stk) {
-      finish(lane_walk<K, true>(X, D.ti, s_lut, s_dt, A.stk + (uint64_t)blockIdx.x * (A.cap + kLanes)));
+      finish(lane_walk<K, true>(X, D.ti, s_lut, s_dt, A.stk + (uint64_t)blockIdx.x * (A.cap + kLanes),
+                                A.cap + kLanes, bad));
     } else {
-      finish(lane_walk<K, true>(X, D.ti, s_lut, s_dt, (lds_u32*)s_st));
+      finish(lane_walk<K, true>(X, D.ti, s_lut, s_dt, (lds_u32*)s_st, A.cap + kLanes, bad));
     }
+    if (bad) atomicOr(A.err, ERR_ORDER);
     return;
   }
   TileCtx<K, false> X;
@@ -446,26 +470,20 @@ __device__ __forceinline__ void walk_k(const LevelArgs& A, const TileDesc& D, bo
   base = ((unsigned long long)(uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)(base >> 32)) << 32) |
          (uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)base);  // lane 0's value
   if (base + need > A.ovf_cap) {
+    // no room: the tile's output is left empty (so the level's waveforms stay
+    // in time order) and the run reports the error
     if (lane == 0) atomicOr(A.err, ERR_OVERFLOW);
-    base = 0;  // results are invalid (the error is reported); keep every wavefront moving
-    if (need > A.ovf_cap) {
-      Lane<uint32_t*> ln;
-      ln.st = A.ovf;
-      ln.sp = ln.top = ln.cnt = ln.nown = 0;
-      finish(ln);
-      return;
-    }
+    Lane<uint32_t*> ln;
+    ln.st = A.ovf;
+    ln.sp = ln.top = ln.cnt = ln.nown = 0;
+    finish(ln);
+    return;
   }
-  finish(lane_walk<K, false>(X, D.ti, s_lut, s_dt, A.ovf + base));
+  uint32_t bad = 0;
+  finish(lane_walk<K, false>(X, D.ti, s_lut, s_dt, A.ovf + base, (uint32_t)need, bad));
+  if (bad) atomicOr(A.err, ERR_ORDER);
 }
 
-// Combine the lanes' stacks into the tile's output: wavefront scan of the
-// lane aggregates, decoupled look-back over the gate's earlier tiles, dedup
-// against the carry, stores, SAIF durations (summed per lane in 32 bits:
-// times < 2^30) and event count.
-// Wavefront scan of the lane aggregates and the gate prefix before the tile.
-// chain: this wavefront finished the gate's previous tile just before, and
-// prev_incl is its inclusive prefix (no look-back needed).
 // The tile's output inside the tile: wavefront scan of the lane aggregates
 // (dedup across lane boundaries is an associative monoid).
 struct TileSum {
@@ -498,10 +516,12 @@ __device__ __forceinline__ TileSum combine_tile(const Lane<SP>& ln) {
 // Store the tile's entries (deduplicated inside the tile) at dst, sum the
 // SAIF durations between them (32 bits per lane: times < 2^30; the lanes'
 // intervals are disjoint pieces of the tile's span, so the sums fit too) and
-// write the tile summary.
+// write the tile summary.  out_cap: words of the tile's output region (never
+// reached for time-ordered inputs; stores beyond it are dropped and the
+// count is clamped, with ERR_ORDER).
 template <typename SP>
 __device__ __forceinline__ void emit_tile(const LevelArgs& A, uint64_t tk, const Lane<SP>& ln,
-                                          const TileSum& T, uint32_t* dst,
+                                          const TileSum& T, uint32_t* dst, uint32_t out_cap,
                                           unsigned long long w_start, unsigned long long w_end,
                                           unsigned long long& wave_ev) {
   const int lane = threadIdx.x;
@@ -530,7 +550,7 @@ __device__ __forceinline__ void emit_tile(const LevelArgs& A, uint64_t tk, const
       a0 += pvv == 0u ? len : 0u;
       a1 += pvv == 3u ? len : 0u;
       if (v != pvv) {
-        dst[w] = e;
+        if (w < out_cap) dst[w] = e;
         ++w;
       }
       pvv = v;
@@ -541,7 +561,8 @@ __device__ __forceinline__ void emit_tile(const LevelArgs& A, uint64_t tk, const
   const uint32_t ev = wave_sum(ln.nown);
   if (lane == 0) {
     TileOut o;
-    o.cnt = T.tile.cnt;
+    o.cnt = min(T.tile.cnt, out_cap);
+    if (T.tile.cnt > out_cap) atomicOr(A.err, ERR_ORDER);
     o.ft = T.ft;
     o.lt = T.tile.lt;
     o.flags = agg_flags(T.tile);
@@ -637,7 +658,7 @@ __global__ __launch_bounds__(kLanes) void k_level(LevelArgs A) {
         fits_n = tile_words(Dn) <= A.cap;
         stage_tile(A, Dn, fits_n, s_in, s_lut, s_dt);
       }
-      emit_tile(A, tk, ln, T, dst, w_start, w_end, wave_ev);
+      emit_tile(A, tk, ln, T, dst, D.out_cap, w_start, w_end, wave_ev);
     };
     switch (D.k) {
       case 1: walk_k<1>(A, D, fits, s_in, s_st, lut8, dt16, finish); break;
@@ -822,31 +843,39 @@ __global__ void k_tile_desc(const GateRec* __restrict__ gates, uint32_t ng,
     D.dmax = G.dmax;
     D.out_net = G.out_net;
     D.job = G.job;
-    D.pad = 0;
   }
+  // own input words of the tile (up to the next tile's first words), summed
+  // over the tile's four threads: its output region in the level scratch
+  // holds that many entries plus one
+  uint32_t span = 0;
   if (j >= G.k) {
     D.L[j] = 0;
     D.E[j] = 0;
     D.in_off[j] = 0;
-    return;
+  } else {
+    const uint32_t* lw = pool + woff[G.in_net[P.lead]];
+    const uint32_t* iw = pool + woff[G.in_net[j]];
+    const uint32_t n = wlen[G.in_net[j]];
+    const uint32_t c0 = ti * kLanes;
+    const uint32_t clast = min(c0 + kLanes - 1, P.nseg - 1);
+    uint32_t L = 0, E = n, Ln = n;
+    if (c0 > 0) {
+      const uint32_t tau_lo = lw[c0 * P.stride] >> 2;
+      L = (uint32_t)j == P.lead ? c0 * P.stride : lb_words(iw, n, tau_lo << 2);
+    }
+    if (clast + 1 < P.nseg) {
+      const uint32_t tau_nx = lw[(clast + 1) * P.stride] >> 2;
+      Ln = (uint32_t)j == P.lead ? (clast + 1) * P.stride : lb_words(iw, n, tau_nx << 2);
+      E = lb_words(iw, n, (tau_nx + G.dmax) << 2);
+    }
+    D.L[j] = L;
+    D.E[j] = E;
+    D.in_off[j] = woff[G.in_net[j]] + (L > 0 ? L - 1 : 0u);
+    span = Ln - L;
   }
-  const uint32_t* lw = pool + woff[G.in_net[P.lead]];
-  const uint32_t* iw = pool + woff[G.in_net[j]];
-  const uint32_t n = wlen[G.in_net[j]];
-  const uint32_t c0 = ti * kLanes;
-  const uint32_t clast = min(c0 + kLanes - 1, P.nseg - 1);
-  uint32_t L = 0, E = n;
-  if (c0 > 0) {
-    const uint32_t tau_lo = lw[c0 * P.stride] >> 2;
-    L = (uint32_t)j == P.lead ? c0 * P.stride : lb_words(iw, n, tau_lo << 2);
-  }
-  if (clast + 1 < P.nseg) {
-    const uint32_t tau_hi = (lw[(clast + 1) * P.stride] >> 2) + G.dmax;
-    E = lb_words(iw, n, tau_hi << 2);
-  }
-  D.L[j] = L;
-  D.E[j] = E;
-  D.in_off[j] = woff[G.in_net[j]] + (L > 0 ? L - 1 : 0u);
+  span += __shfl_xor(span, 1);
+  span += __shfl_xor(span, 2);
+  if (j == 0) D.out_cap = span + 1;
 }
 
 __global__ void k_fill_u64(unsigned long long* p, uint64_t n, unsigned long long v) {

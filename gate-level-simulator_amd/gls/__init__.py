@@ -36,7 +36,7 @@ class SimOpts(ctypes.Structure):
                 ("keep_waves", ctypes.c_uint32), ("path", ctypes.c_uint32),
                 ("level_blocks", ctypes.c_uint32), ("no_prefetch", ctypes.c_uint32),
                 ("graph", ctypes.c_uint32), ("hbm_stacks", ctypes.c_uint32),
-                ("pool_grow", ctypes.c_uint32)]
+                ("pool_grow", ctypes.c_uint32), ("ovf_words", ctypes.c_uint32)]
 
 
 PATH_FUSED = 0
@@ -383,24 +383,24 @@ class Context(_Handle):
 
     @staticmethod
     def _opts(pool_bytes, seg_events, keep_waves, path, level_blocks=0, prefetch=True,
-              graph=False, hbm_stacks=False, pool_grow=False):
+              graph=False, hbm_stacks=False, pool_grow=False, ovf_words=0):
         return SimOpts(pool_bytes, seg_events, 1 if keep_waves else 0, path, level_blocks,
                        0 if prefetch else 1, 1 if graph else 0, 1 if hbm_stacks else 0,
-                       1 if pool_grow else 0)
+                       1 if pool_grow else 0, ovf_words)
 
     def prepare(self, design, stim, dumpon, dumpoff, pool_bytes=0, seg_events=0, keep_waves=False,
                 path=PATH_FUSED, level_blocks=0, prefetch=True, graph=False,
-                hbm_stacks=False, pool_grow=False):
+                hbm_stacks=False, pool_grow=False, ovf_words=0):
         h = ctypes.c_void_p()
         o = self._opts(pool_bytes, seg_events, keep_waves, path, level_blocks, prefetch, graph,
-                       hbm_stacks, pool_grow)
+                       hbm_stacks, pool_grow, ovf_words)
         _check(load().gls_prepare(self.ptr, design.ptr, stim.ptr, dumpon, dumpoff, ctypes.byref(o),
                                   ctypes.byref(h)))
         return Run(h.value)
 
     def prepare_batch(self, designs, stims, dumpons, dumpoffs, pool_bytes=0, seg_events=0,
                       keep_waves=False, level_blocks=0, prefetch=True,
-                      graph=False, hbm_stacks=False, pool_grow=False):
+                      graph=False, hbm_stacks=False, pool_grow=False, ovf_words=0):
         n = len(designs)
         assert len(stims) == n and len(dumpons) == n and len(dumpoffs) == n
         D = (ctypes.c_void_p * n)(*[d.ptr for d in designs])
@@ -409,17 +409,17 @@ class Context(_Handle):
         off = (_U64 * n)(*dumpoffs)
         h = ctypes.c_void_p()
         o = self._opts(pool_bytes, seg_events, keep_waves, PATH_FUSED, level_blocks, prefetch,
-                       graph, hbm_stacks, pool_grow)
+                       graph, hbm_stacks, pool_grow, ovf_words)
         _check(load().gls_prepare_batch(self.ptr, n, D, S, on, off, ctypes.byref(o),
                                         ctypes.byref(h)))
         return Run(h.value)
 
     def simulate(self, design, stim, dumpon, dumpoff, pool_bytes=0, seg_events=0, keep_waves=False,
                  path=PATH_FUSED, level_blocks=0, prefetch=True, graph=False,
-                 hbm_stacks=False, pool_grow=False):
+                 hbm_stacks=False, pool_grow=False, ovf_words=0):
         h = ctypes.c_void_p()
         o = self._opts(pool_bytes, seg_events, keep_waves, path, level_blocks, prefetch, graph,
-                       hbm_stacks, pool_grow)
+                       hbm_stacks, pool_grow, ovf_words)
         _check(load().gls_simulate(self.ptr, design.ptr, stim.ptr, dumpon, dumpoff, ctypes.byref(o),
                                    ctypes.byref(h)))
         return Run(h.value)

@@ -92,6 +92,8 @@ typedef struct {
   uint32_t hbm_stacks;   /* 1: the fused kernel keeps its preemption stacks in a per-workgroup
                             HBM scratch instead of LDS (half the LDS, more resident workgroups) */
   uint32_t pool_grow;    /* 1: pool_bytes is only the starting size; the pool grows as with 0 */
+  uint32_t ovf_words;    /* fused path: words of the stack buffer for tiles too long for LDS, per
+                            level (0: 2^24); a level that needs more fails with GLS_E_UNSUPPORTED */
 } gls_sim_opts;
 
 /* Level-kernel variants: one fused LDS-tiled kernel per level, or the

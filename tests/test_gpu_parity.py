@@ -91,8 +91,8 @@ def test_segment_size_invariance(ctx):
     d, s = golden_cases.make("c1_small")
     start, end = golden_cases.CASES["c1_small"]["window"]
     ref = None
-    # seg 1: many tiles per gate (long look-back chains); 1<<20: every tile
-    # overflows LDS and takes the HBM fallback
+    # seg 1: many tiles per gate; 1<<20: one tile per gate (the largest LDS
+    # tiles; tiles beyond LDS are covered by test_tiles_beyond_lds_match_lds_tiles)
     for path, seg in ((0, 1), (0, 8), (0, 32), (0, 256), (0, 5000), (0, 1 << 20),
                       (1, 1), (1, 64), (1, 1 << 20)):
         say("segments", path, seg)
@@ -167,6 +167,44 @@ def test_pool_grows_between_levels(ctx):
         assert "%016x" % r.waves_hash() == g["waves_hash"], i
     assert r.stats()["eval_launches"] == 1
     r.close()
+
+
+def dense_case():
+    """A small netlist under a dense stimulus (about 40k events per input):
+    every gate's inputs carry more words than the largest LDS tile holds."""
+    d = gls.Design.generate(0, 77, 60, 4, 8)
+    s = gls.Stimulus.generate(d, 78, 400000, 10.0, 0.01)
+    return d, s, 0, 400001
+
+
+def test_tiles_beyond_lds_match_lds_tiles(ctx):
+    # seg 2^20: one tile per gate, far over the LDS capacity, so every tile
+    # reads its inputs from HBM and keeps its stacks in the overflow buffer;
+    # results equal the LDS-tiled and two-pass runs
+    d, s, on, off = dense_case()
+    runs = []
+    for seg, path in ((1 << 20, gls.PATH_FUSED), (0, gls.PATH_FUSED), (0, gls.PATH_TWO_PASS)):
+        say("dense", seg, path)
+        r = ctx.simulate(d, s, on, off, seg_events=seg, path=path, keep_waves=True)
+        runs.append((r.waves_hash(), r.saif_text()))
+        r.close()
+    assert runs[0] == runs[1] == runs[2]
+
+
+def test_overflow_buffer_exhaustion_is_reported(ctx):
+    # tiles beyond LDS and a 1024-word overflow stack buffer: tiles that find
+    # no room leave their output empty (the level's waveforms stay in time
+    # order for the next level), the run fails with GLS_E_UNSUPPORTED and
+    # the context stays usable
+    d, s, on, off = dense_case()
+    with pytest.raises(gls.GlsError) as e:
+        ctx.simulate(d, s, on, off, seg_events=1 << 20, ovf_words=1024)
+    assert e.value.status == "GLS_E_UNSUPPORTED"
+    g = GOLDEN["c1_small"]
+    d1, s1 = golden_cases.make("c1_small")
+    start, end = golden_cases.CASES["c1_small"]["window"]
+    r = ctx.simulate(d1, s1, start, end, keep_waves=True)
+    assert "%016x" % r.waves_hash() == g["waves_hash"]
 
 
 def test_pool_exhaustion_is_reported(ctx):
