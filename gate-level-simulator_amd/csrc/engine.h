@@ -112,12 +112,12 @@ class Job {
   int mode = 0;                   // 0: fused LDS-tiled level kernel, 1: two-pass kernels
   uint32_t level_blocks_opt = 0;  // workgroups per fused launch (0: occupancy limit)
   uint32_t prefetch = 1;          // stage the next tile before storing the current one
-  uint64_t ovf_words_opt = 0;     // overflow stack words (0: 2^24)
+  uint64_t ovf_words_opt = 0;     // overflow stack words (0: from 2^24, doubled as needed)
   // 1: after a plain execution, capture the whole level sequence (no host
   // round trips) into a hipGraph and replay it on later executions
   uint32_t graph = 0;
   uint32_t stack_mode = 0;  // 0: preemption stacks in LDS, 1: per-workgroup scratch in HBM
-  uint32_t pool_grow = 0;   // 1: a given pool size is the starting size and grows as needed
+  uint32_t pool_grow = 0;   // 1: given pool and overflow-buffer sizes are starting sizes that grow
   bool graph_replayed() const { return last_replay_; }
   static constexpr uint32_t kDefaultSeg = 32;  // input events per time segment
 
@@ -166,6 +166,7 @@ class Job {
   uint32_t* d_toff_ = nullptr;
   uint32_t* d_ovf_ = nullptr;
   uint64_t ovf_words_ = 0;
+  size_t cap_ovf_ = 0;
   int level_blocks_ = 0;
   uint32_t cap_ = 0;  // LDS input words per fused tile
   std::vector<void*> owned_;

@@ -150,7 +150,6 @@ void Server::worker(int slot) {
         try {
           Job job(device_, stream);
           job.seg_target = opt_.seg_events;
-          job.ovf_words_opt = std::min<uint64_t>(1ull << 24, std::max<uint64_t>(1ull << 18, pool / 16));
           job.prepare({I}, pool);
           job.run();
           job.fetch_saif(saif);

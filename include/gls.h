@@ -91,9 +91,12 @@ typedef struct {
                             between levels, and later executions replay it */
   uint32_t hbm_stacks;   /* 1: the fused kernel keeps its preemption stacks in a per-workgroup
                             HBM scratch instead of LDS (half the LDS, more resident workgroups) */
-  uint32_t pool_grow;    /* 1: pool_bytes is only the starting size; the pool grows as with 0 */
+  uint32_t pool_grow;    /* 1: pool_bytes and ovf_words are only starting sizes; both grow as
+                            with 0 */
   uint32_t ovf_words;    /* fused path: words of the stack buffer for tiles too long for LDS, per
-                            level (0: 2^24); a level that needs more fails with GLS_E_UNSUPPORTED */
+                            level (0: starts at 2^24 and doubles when a level runs out, the
+                            level is then run again); a fixed size that a level outgrows fails
+                            the run with GLS_E_UNSUPPORTED */
 } gls_sim_opts;
 
 /* Level-kernel variants: one fused LDS-tiled kernel per level, or the

@@ -207,6 +207,17 @@ def test_overflow_buffer_exhaustion_is_reported(ctx):
     assert "%016x" % r.waves_hash() == g["waves_hash"]
 
 
+def test_overflow_buffer_grows(ctx):
+    # the same 1024-word start as a growing size: each level that runs out
+    # of it runs again with twice the buffer; results equal the LDS tiles
+    d, s, on, off = dense_case()
+    ref = ctx.simulate(d, s, on, off, keep_waves=True)
+    r = ctx.simulate(d, s, on, off, seg_events=1 << 20, ovf_words=1024, pool_grow=True,
+                     keep_waves=True)
+    assert (r.waves_hash(), r.saif_text()) == (ref.waves_hash(), ref.saif_text())
+    assert r.stats()["n_events"] == ref.stats()["n_events"]
+
+
 def test_pool_exhaustion_is_reported(ctx):
     # the pool is clamped to the stimulus plus 1024 words: level 0 cannot fit
     d, s = golden_cases.make("c0_small")
