@@ -284,7 +284,8 @@ def main():
         f"{run.stats()['level_blocks']} workgroups per level launch")
     for i in range(args.warmup):
         run.execute()
-        log(f"bench: warmup {i} done ({run.stats()['kernel_ms']:.1f} ms device)")
+        log(f"bench: warmup {i} done ({run.stats()['kernel_ms']:.1f} ms device, "
+            f"{run.stats()['n_events']} gate events)")
     walls, stats = [], []
     for i in range(args.steps):
         w0 = time.perf_counter()
@@ -292,7 +293,8 @@ def main():
         walls.append(time.perf_counter() - w0)
         stats.append(run.stats())
         log(f"bench: step {i} wall {walls[-1] * 1e3:.1f} ms, device {stats[-1]['kernel_ms']:.1f} ms, "
-            f"k_eval {stats[-1]['eval_ms']:.1f} ms")
+            f"k_eval {stats[-1]['eval_ms']:.1f} ms, {stats[-1]['n_events']} gate events, "
+            f"{stats[-1]['eval_launches']} level launches")
     st = stats[-1]
     ms_per_step = sum(walls) / len(walls) * 1e3
     value = st["n_events"] / (ms_per_step / 1e3)
