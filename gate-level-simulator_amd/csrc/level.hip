@@ -789,9 +789,16 @@ __global__ void k_plan2(const GateRec* __restrict__ gates, uint32_t ng,
   }
   // a tile's offset inside its gate's waveform travels in 31 bits
   if (N + 1 >= (1ull << 30)) atomicOr(err, ERR_RANGE);
-  // leader events per segment so that a segment holds ~seg_target input events
+  // leader events per segment so that a segment holds ~seg_target input
+  // events; then the gate's tiles are filled evenly (the stride that spreads
+  // the leader's events over all 64 segments of each of those tiles, never
+  // larger), so a small gate's single tile uses every lane
   uint64_t s = N > 0 ? (uint64_t)seg_target * nl / N : 1;
   if (s < 1) s = 1;
+  if (nl > 0) {
+    const uint64_t nt0 = ((nl + s - 1) / s + kLanes - 1) / kLanes;
+    s = (nl + nt0 * kLanes - 1) / (nt0 * kLanes);
+  }
   uint32_t ns = nl > 0 ? (uint32_t)((nl + s - 1) / s) : 1u;
   GatePlan P;
   P.nseg = ns;
