@@ -263,6 +263,16 @@ int gls_stim_from_vcd(const char* path, gls_stim** out) {
   API_END
 }
 
+int gls_stim_from_vcd_threads(const char* path, int threads, gls_stim** out) {
+  API_BEGIN
+  if (!path || !out) throw GlsError(GLS_E_ARG, "null argument");
+  if (threads < 0) throw GlsError(GLS_E_ARG, "negative thread count");
+  auto s = std::make_unique<gls_stim>();
+  s->st = parse_vcd_file(path, threads);
+  *out = s.release();
+  API_END
+}
+
 int gls_stim_generate(const gls_design* d, uint64_t seed, uint32_t t_end, double mean_gap,
                       double x_prob, gls_stim** out) {
   API_BEGIN

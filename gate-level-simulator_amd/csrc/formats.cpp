@@ -1,6 +1,8 @@
 #include "formats.h"
 
 #include <algorithm>
+#include <exception>
+#include <thread>
 #include <cctype>
 #include <cstdio>
 #include <cstring>
@@ -433,9 +435,58 @@ static int vcd_code(char c) {
 
 static const uint64_t kMaxTime = (1ull << 30) - (1ull << 17);
 
-Stimulus parse_vcd_text(const std::string& s) {
-  Stimulus st;
-  // header
+namespace {
+
+struct VcdVar {
+  std::vector<int> sig;
+  int lindex = -1, rindex = -1;
+  bool vec = false;
+};
+
+// Declarations of a VCD: identifier -> signals, which signals are bus bits,
+// and where the value-change body starts.
+struct VcdHeader {
+  std::unordered_map<std::string, std::vector<VcdVar>> by_id;
+  std::vector<uint8_t> is_vec_bit;
+  size_t body = 0;
+  // identifiers of up to 3 printable characters (the usual case) map to
+  // their declarations through a flat table: no allocation or hashing per
+  // value change
+  std::vector<int> fast;                       // base-95 code -> index into slots, or -1
+  std::vector<const std::vector<VcdVar>*> slots;
+  static constexpr size_t kFast = 95 * 95 * 95 + 95 * 95 + 95;
+  static long code(const char* c, size_t len) {
+    if (len == 0 || len > 3) return -1;
+    long x = 0;
+    for (size_t i = 0; i < len; ++i) {
+      const int d = (unsigned char)c[i] - 32;
+      if (d < 1 || d > 94) return -1;
+      x = x * 95 + d;
+    }
+    return x;
+  }
+  const std::vector<VcdVar>* find(const char* c, size_t len) const {
+    const long k = code(c, len);
+    if (k >= 0) {
+      const int i = fast[(size_t)k];
+      return i < 0 ? nullptr : slots[(size_t)i];
+    }
+    auto it = by_id.find(std::string(c, len));
+    return it == by_id.end() ? nullptr : &it->second;
+  }
+};
+
+// Value changes of one stretch of the body, per signal in file order,
+// before de-duplication; first/last timestamp seen in the stretch.
+struct VcdRaw {
+  std::vector<std::vector<uint32_t>> t;
+  std::vector<std::vector<uint8_t>> v;
+  bool has_stamp = false;
+  uint64_t first_stamp = 0, last_stamp = 0;
+};
+
+VcdHeader parse_vcd_header(const std::string& s, Stimulus& st) {
+  VcdHeader H;
   size_t p = 0, n = s.size();
   auto skip_ws = [&]() { while (p < n && std::isspace((unsigned char)s[p])) ++p; };
   auto word = [&]() {
@@ -444,11 +495,7 @@ Stimulus parse_vcd_text(const std::string& s) {
     while (p < n && !std::isspace((unsigned char)s[p])) ++p;
     return s.substr(a, p - a);
   };
-  struct Var { std::vector<int> sig; int lindex = -1, rindex = -1; bool vec = false; };
-  std::unordered_map<std::string, std::vector<Var>> by_id;
   bool scope_seen = false;
-  std::vector<uint8_t> is_vec_bit;
-  std::vector<uint8_t> last_bit;  // per signal, for bus-bit dedup (initial Z)
   for (;;) {
     std::string w = word();
     if (w.empty()) throw GlsError(GLS_E_FORMAT, "vcd: missing $enddefinitions");
@@ -484,7 +531,7 @@ Stimulus parse_vcd_text(const std::string& s) {
       if (f.size() < 4) throw GlsError(GLS_E_FORMAT, "vcd: short $var");
       int size = std::stoi(f[1]);
       std::string id = f[2], ref = f[3];
-      Var v;
+      VcdVar v;
       std::string range = f.size() > 4 ? f[4] : "";
       if (ref.find('[') != std::string::npos && range.empty()) {
         range = ref.substr(ref.find('['));
@@ -512,18 +559,16 @@ Stimulus parse_vcd_text(const std::string& s) {
           v.sig.push_back((int)st.names.size());
           st.names.push_back(ref + "[" + std::to_string(i) + "]");
           st.waves.emplace_back();
-          is_vec_bit.push_back(1);
-          last_bit.push_back(VZ);
+          H.is_vec_bit.push_back(1);
           if (i == v.rindex) break;
         }
       } else {
         v.sig.push_back((int)st.names.size());
         st.names.push_back(ref);
         st.waves.emplace_back();
-        is_vec_bit.push_back(0);
-        last_bit.push_back(VZ);
+        H.is_vec_bit.push_back(0);
       }
-      by_id[id].push_back(v);
+      H.by_id[id].push_back(v);
       continue;
     }
     if (w[0] == '$') {  // $date $version $comment $upscope ...
@@ -533,19 +578,37 @@ Stimulus parse_vcd_text(const std::string& s) {
     }
     throw GlsError(GLS_E_FORMAT, "vcd: unexpected token in header: " + w);
   }
-  // body
+  H.body = p;
+  H.fast.assign(VcdHeader::kFast, -1);
+  for (auto& kv : H.by_id) {
+    const long k = VcdHeader::code(kv.first.data(), kv.first.size());
+    if (k < 0) continue;
+    H.fast[(size_t)k] = (int)H.slots.size();
+    H.slots.push_back(&kv.second);
+  }
+  return H;
+}
+
+// Body tokens of s[a, b): timestamps and value changes, recorded per signal
+// in file order (no de-duplication yet).  The stretch starts at time 0 or at
+// a timestamp of its own.
+void parse_vcd_body(const std::string& s, size_t a, size_t b, const VcdHeader& H, VcdRaw& R) {
+  const size_t nsig = H.is_vec_bit.size();
+  R.t.assign(nsig, {});
+  R.v.assign(nsig, {});
+  size_t p = a;
+  const size_t n = b;
+  auto skip_ws = [&]() { while (p < n && std::isspace((unsigned char)s[p])) ++p; };
+  auto word = [&]() {
+    skip_ws();
+    size_t q = p;
+    while (p < n && !std::isspace((unsigned char)s[p])) ++p;
+    return s.substr(q, p - q);
+  };
   uint64_t cur_t = 0;
   auto put = [&](int sig, uint8_t v) {
-    Wave& wv = st.waves[sig];
-    if (is_vec_bit[sig]) {  // bus bits are de-duplicated, starting from Z
-      if (v == last_bit[sig]) return;
-      last_bit[sig] = v;
-    }
-    if (!wv.t.empty() && wv.t.back() == (uint32_t)cur_t) {
-      wv.v.back() = v;  // several changes at one timestamp: the last wins
-      return;
-    }
-    wv.push((uint32_t)cur_t, v);
+    R.t[sig].push_back((uint32_t)cur_t);
+    R.v[sig].push_back(v);
   };
   while (p < n) {
     skip_ws();
@@ -554,12 +617,15 @@ Stimulus parse_vcd_text(const std::string& s) {
     if (c == '#') {
       ++p;
       uint64_t t = 0;
-      size_t a = p;
+      size_t q = p;
       while (p < n && std::isdigit((unsigned char)s[p])) t = t * 10 + (uint64_t)(s[p++] - '0');
-      if (p == a) throw GlsError(GLS_E_FORMAT, "vcd: bad timestamp");
+      if (p == q) throw GlsError(GLS_E_FORMAT, "vcd: bad timestamp");
       if (t > kMaxTime) throw GlsError(GLS_E_RANGE, "vcd: time beyond the 30-bit event word");
       if (t < cur_t) throw GlsError(GLS_E_FORMAT, "vcd: timestamps go backwards");
       cur_t = t;
+      if (!R.has_stamp) R.first_stamp = t;
+      R.has_stamp = true;
+      R.last_stamp = t;
       continue;
     }
     if (c == '$') {
@@ -570,21 +636,23 @@ Stimulus parse_vcd_text(const std::string& s) {
     }
     if (c == 'b' || c == 'B') {
       ++p;
-      size_t a = p;
+      size_t q = p;
       while (p < n && !std::isspace((unsigned char)s[p])) ++p;
-      std::string bits = s.substr(a, p - a);
-      std::string id = word();
-      auto it = by_id.find(id);
-      if (it == by_id.end()) throw GlsError(GLS_E_FORMAT, "vcd: unknown identifier " + id);
-      for (auto& v : it->second) {
+      std::string bits = s.substr(q, p - q);
+      skip_ws();
+      const size_t iq = p;
+      while (p < n && !std::isspace((unsigned char)s[p])) ++p;
+      const std::vector<VcdVar>* vars = H.find(s.data() + iq, p - iq);
+      if (!vars) throw GlsError(GLS_E_FORMAT, "vcd: unknown identifier " + s.substr(iq, p - iq));
+      for (auto& v : *vars) {
         size_t width = v.sig.size();
-        std::string b = bits;
-        if (b.size() < width) {
-          char pad = (b[0] == 'x' || b[0] == 'X' || b[0] == 'z' || b[0] == 'Z') ? b[0] : '0';
-          b = std::string(width - b.size(), pad) + b;
+        std::string bb = bits;
+        if (bb.size() < width) {
+          char pad = (bb[0] == 'x' || bb[0] == 'X' || bb[0] == 'z' || bb[0] == 'Z') ? bb[0] : '0';
+          bb = std::string(width - bb.size(), pad) + bb;
         }
         for (size_t k = 0; k < width; ++k) {
-          int code = vcd_code(b[b.size() - width + k]);
+          int code = vcd_code(bb[bb.size() - width + k]);
           if (code < 0) throw GlsError(GLS_E_FORMAT, "vcd: bad vector value");
           put(v.sig[k], (uint8_t)code);
         }
@@ -599,12 +667,11 @@ Stimulus parse_vcd_text(const std::string& s) {
     int code = vcd_code(c);
     if (code < 0) throw GlsError(GLS_E_FORMAT, std::string("vcd: unexpected character '") + c + "'");
     ++p;
-    size_t a = p;
+    size_t q = p;
     while (p < n && !std::isspace((unsigned char)s[p])) ++p;
-    std::string id = s.substr(a, p - a);
-    auto it = by_id.find(id);
-    if (it == by_id.end()) throw GlsError(GLS_E_FORMAT, "vcd: unknown identifier " + id);
-    for (auto& v : it->second) {
+    const std::vector<VcdVar>* vars = H.find(s.data() + q, p - q);
+    if (!vars) throw GlsError(GLS_E_FORMAT, "vcd: unknown identifier " + s.substr(q, p - q));
+    for (auto& v : *vars) {
       if (v.vec) {
         for (int sig : v.sig) put(sig, (uint8_t)code);
       } else {
@@ -612,10 +679,106 @@ Stimulus parse_vcd_text(const std::string& s) {
       }
     }
   }
+}
+
+// One signal's waveform from its value changes in file order: bus bits are
+// de-duplicated starting from Z; several changes at one timestamp, the last
+// wins.
+void finish_vcd_wave(const std::vector<VcdRaw>& parts, int sig, bool vec_bit, Wave& wv) {
+  uint8_t last = VZ;
+  size_t total = 0;
+  for (const VcdRaw& R : parts) total += R.t[sig].size();
+  wv.t.reserve(total);
+  wv.v.reserve(total);
+  for (const VcdRaw& R : parts) {
+    const std::vector<uint32_t>& T = R.t[sig];
+    const std::vector<uint8_t>& V = R.v[sig];
+    for (size_t i = 0; i < T.size(); ++i) {
+      const uint8_t v = V[i];
+      if (vec_bit) {
+        if (v == last) continue;
+        last = v;
+      }
+      if (!wv.t.empty() && wv.t.back() == T[i]) {
+        wv.v.back() = v;
+        continue;
+      }
+      wv.push(T[i], v);
+    }
+  }
+}
+
+}  // namespace
+
+// VCD reader (VCDParser::init/parse/process, src/VCDParser.cpp:244-717).
+// The header is read first; the body is cut at line-initial timestamps
+// ("\n#", as the reference's threaded parser cuts it, src/VCDParser.cpp:668-683)
+// into one stretch per thread, the stretches are tokenised in parallel, and
+// each signal's waveform is then assembled from the stretches in order, in
+// parallel over signals.  The result equals a single-threaded read: a body
+// with a $comment (which could hold "\n#") or any error in a stretch is
+// read again by one thread, which then gives the canonical error.
+Stimulus parse_vcd_text(const std::string& s, int threads) {
+  Stimulus st;
+  VcdHeader H = parse_vcd_header(s, st);
+  const size_t a = H.body, n = s.size();
+  if (threads <= 0) threads = (int)std::min<unsigned>(16, std::max(1u, std::thread::hardware_concurrency()));
+  if (n - a < (8u << 20) || s.find("$comment", a) != std::string::npos) threads = 1;
+  std::vector<size_t> cut{a};
+  for (int i = 1; i < threads; ++i) {
+    size_t target = a + (n - a) / (size_t)threads * (size_t)i;
+    size_t k = s.find("\n#", std::max(target, cut.back()));
+    if (k == std::string::npos) break;
+    if (k + 1 > cut.back()) cut.push_back(k + 1);
+  }
+  cut.push_back(n);
+  const size_t np = cut.size() - 1;
+  std::vector<VcdRaw> parts(np);
+  std::vector<std::exception_ptr> err(np);
+  auto run_part = [&](size_t i) {
+    try {
+      parse_vcd_body(s, cut[i], cut[i + 1], H, parts[i]);
+    } catch (...) {
+      err[i] = std::current_exception();
+    }
+  };
+  if (np == 1) {
+    parse_vcd_body(s, cut[0], cut[1], H, parts[0]);
+  } else {
+    std::vector<std::thread> th;
+    for (size_t i = 0; i < np; ++i) th.emplace_back(run_part, i);
+    for (auto& t : th) t.join();
+    bool bad = false;
+    uint64_t last = 0;
+    for (size_t i = 0; i < np && !bad; ++i) {
+      if (err[i]) bad = true;
+      else if (parts[i].has_stamp) {
+        if (parts[i].first_stamp < last) bad = true;  // timestamps go backwards across a cut
+        last = parts[i].last_stamp;
+      }
+    }
+    if (bad) return parse_vcd_text(s, 1);
+  }
+  const size_t nsig = H.is_vec_bit.size();
+  const size_t nthr = std::min<size_t>(np, nsig ? nsig : 1);
+  auto finish = [&](size_t w) {
+    for (size_t g = w; g < nsig; g += nthr) finish_vcd_wave(parts, (int)g, H.is_vec_bit[g] != 0, st.waves[g]);
+  };
+  if (nthr <= 1) {
+    finish(0);
+  } else {
+    std::vector<std::thread> th;
+    for (size_t w = 0; w < nthr; ++w) th.emplace_back(finish, w);
+    for (auto& t : th) t.join();
+  }
   return st;
 }
 
-Stimulus parse_vcd_file(const std::string& path) { return parse_vcd_text(read_file(path)); }
+Stimulus parse_vcd_text(const std::string& text) { return parse_vcd_text(text, 0); }
+
+Stimulus parse_vcd_file(const std::string& path, int threads) {
+  return parse_vcd_text(read_file(path), threads);
+}
 
 // ------------------------------------------------------------- writers ----
 std::string gv_text(const Netlist& nl, const Library& lib) {

@@ -18,8 +18,11 @@ Netlist parse_gv_file(const std::string& path, const Library& lib);
 Netlist parse_gv_text(const std::string& text, const Library& lib);
 void parse_sdf_file(const std::string& path, Netlist& nl);
 void parse_sdf_text(const std::string& text, Netlist& nl);
-Stimulus parse_vcd_file(const std::string& path);
+// threads: 0 = up to 16 (the body is read in parallel stretches when it is
+// large), 1 = one thread; the result does not depend on it.
+Stimulus parse_vcd_file(const std::string& path, int threads = 0);
 Stimulus parse_vcd_text(const std::string& text);
+Stimulus parse_vcd_text(const std::string& text, int threads);
 
 // Writers (used by the generator and tests).
 std::string gv_text(const Netlist& nl, const Library& lib);

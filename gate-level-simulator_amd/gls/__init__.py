@@ -106,6 +106,7 @@ SIGNATURES = [
     ("gls_design_info_get", _I, [_P, ctypes.POINTER(DesignInfo)]),
     ("gls_design_free", None, [_P]),
     ("gls_stim_from_vcd", _I, [_S, _PP]),
+    ("gls_stim_from_vcd_threads", _I, [_S, ctypes.c_int, _PP]),
     ("gls_stim_generate", _I, [_P, _U64, ctypes.c_uint32, ctypes.c_double, ctypes.c_double, _PP]),
     ("gls_stim_write_vcd", _I, [_P, _S]),
     ("gls_stim_info", _I, [_P, ctypes.POINTER(_U64), ctypes.POINTER(_U64)]),
@@ -244,9 +245,14 @@ class Stimulus(_Handle):
     _free = "gls_stim_free"
 
     @classmethod
-    def from_vcd(cls, path):
+    def from_vcd(cls, path, threads=None):
+        """threads: None = the library's choice (parallel for a large body),
+        1 = one thread, n = up to n threads; the result does not depend on it."""
         h = ctypes.c_void_p()
-        _check(load().gls_stim_from_vcd(_b(path), ctypes.byref(h)))
+        if threads is None:
+            _check(load().gls_stim_from_vcd(_b(path), ctypes.byref(h)))
+        else:
+            _check(load().gls_stim_from_vcd_threads(_b(path), int(threads), ctypes.byref(h)))
         return cls(h.value)
 
     @classmethod

@@ -62,8 +62,13 @@ int gls_design_info_get(const gls_design* d, gls_design_info* info);
 void gls_design_free(gls_design* d);
 
 /* ---------------------------------------------------------- stimulus --- */
-/* Parse a VCD (VCDParser::init/parse/process, src/VCDParser.cpp:244-717). */
+/* Parse a VCD (VCDParser::init/parse/process, src/VCDParser.cpp:244-717).
+ * A large body is read in parallel stretches cut at line-initial timestamps,
+ * as the reference's threaded parser cuts it (src/VCDParser.cpp:668-683). */
 int gls_stim_from_vcd(const char* vcd_path, gls_stim** out);
+/* Same with an explicit thread count (0: up to 16; 1: one thread); the
+ * result does not depend on it. */
+int gls_stim_from_vcd_threads(const char* vcd_path, int threads, gls_stim** out);
 /* Seeded Poisson stimulus for the design's primary inputs: initial value
  * U{0,1} at t = 0, mean toggle gap `mean_gap` ps, toggles in (0, t_end],
  * a toggle goes to X with probability x_prob. */

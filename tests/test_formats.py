@@ -132,3 +132,68 @@ def test_reference_library_cells_match_generic(tmp_path):
     d_ref = gls.Design.from_files(ref, paths[1], paths[2])
     d_gen = gls.Design.from_files(*paths[:3])
     assert d_ref.info() == d_gen.info()
+
+
+def _big_vcd(path, n_ts=480000, seed=5, backwards_at=None, comment_at=None):
+    """A VCD whose body is over 8 MiB (read in parallel stretches): a 6-bit
+    bus and a 3-bit ascending bus whose values repeat (de-duplicated per
+    bit), scalars with repeated values, several changes of one signal at one
+    timestamp, and timestamps repeated on consecutive lines."""
+    import random
+    rng = random.Random(seed)
+    out = ["$timescale 1ps $end", "$scope module top $end",
+           "$var wire 6 ! bus [5:0] $end", '$var wire 1 " a $end', "$var wire 1 # b $end",
+           "$var wire 3 % nib [0:2] $end", "$upscope $end", "$enddefinitions $end",
+           "$dumpvars", "b000000 !", '0"', "x#", "b101 %", "$end"]
+    t = 0
+    for i in range(n_ts):
+        if backwards_at is not None and i == backwards_at:
+            t -= 3
+        elif rng.random() > 0.05:
+            t += rng.randint(1, 40)
+        out.append("#%d" % t)
+        if comment_at is not None and i == comment_at:
+            out.append("$comment a note $end")
+        for _ in range(rng.randint(1, 3)):
+            k = rng.random()
+            if k < 0.35:
+                out.append("b%s !" % "".join(rng.choice("01xz") for _ in range(rng.randint(1, 6))))
+            elif k < 0.6:
+                out.append(rng.choice("01xz") + '"')
+            elif k < 0.8:
+                out.append(rng.choice("01x") + "#")
+            else:
+                out.append("b%s %%" % "".join(rng.choice("01") for _ in range(3)))
+    path.write_text("\n".join(out) + "\n")
+    return path
+
+
+def _vcd_back(stim, path):
+    stim.write_vcd(str(path))
+    return path.read_text()
+
+
+def test_parallel_vcd_read_equals_one_thread(tmp_path):
+    p = _big_vcd(tmp_path / "big.vcd")
+    assert p.stat().st_size > (8 << 20)
+    one = gls.Stimulus.from_vcd(str(p), threads=1)
+    ref = _vcd_back(one, tmp_path / "one.vcd")
+    for th in (2, 7, 16, None):
+        many = gls.Stimulus.from_vcd(str(p), threads=th)
+        assert many.info() == one.info(), th
+        assert _vcd_back(many, tmp_path / "many.vcd") == ref, th
+
+
+def test_parallel_vcd_read_special_bodies(tmp_path):
+    # a $comment in the body is read by one thread (a comment could hold a
+    # line-initial '#'); the result is the same either way
+    p = _big_vcd(tmp_path / "c.vcd", comment_at=200000)
+    one = gls.Stimulus.from_vcd(str(p), threads=1)
+    many = gls.Stimulus.from_vcd(str(p), threads=8)
+    assert _vcd_back(many, tmp_path / "m.vcd") == _vcd_back(one, tmp_path / "o.vcd")
+    # timestamps going backwards late in the body: the same error as one thread
+    p = _big_vcd(tmp_path / "b.vcd", backwards_at=400000)
+    for th in (1, 8):
+        with pytest.raises(gls.GlsError) as e:
+            gls.Stimulus.from_vcd(str(p), threads=th)
+        assert e.value.status == "GLS_E_FORMAT", th
