@@ -84,15 +84,17 @@ def pool_bytes_for(d, s):
     return 4 * (ev + 12 * gates * (ev // max(1, sig) + 1) + (1 << 20))
 
 
-def batched_vs_plain(ctx, n_sets, scale):
-    """Batched form against plain runs on a scaled-down batching workload:
-    one netlist from the 200k-cell generator at `scale` of its size driven by
-    `n_sets` seeded stimulus sets (0-2 Mps windows, mean gap U[100, 5000] ps),
-    plus n_sets/4 netlists with log-uniform cell counts and depths U{10..60}.
-    Plain: every instance prepared and executed alone, one after the other.
-    Batched: all instances in one prepared run (level l of every instance in
-    one launch), autotuned, then executed.  Every instance's SAIF is compared
-    with its plain run."""
+def batched_vs_plain(ctx, n_sets, scale, budget_frac=0.6):
+    """Batched form against plain runs on the batching workload: one netlist
+    from the 200k-cell generator at `scale` of its size driven by `n_sets`
+    seeded stimulus sets (0-2 Mps windows, mean gap U[100, 5000] ps), plus
+    n_sets/4 netlists with log-uniform cell counts (5k-500k x scale) and
+    depths U{10..60}.  Plain: every instance prepared and executed alone, one
+    after the other.  Batched: the instances in as few prepared runs as fit
+    `budget_frac` of free HBM (each run: level l of all its instances in one
+    launch, the level sequence captured as one hipGraph), autotuned on the
+    first run, then executed.  Every instance's SAIF is compared with its
+    plain run."""
     import random
     import gls
     rng = random.Random(0x5EED0003)
@@ -102,49 +104,77 @@ def batched_vs_plain(ctx, n_sets, scale):
     for i in range(n_sets):
         s = gls.Stimulus.generate(main_d, 0x5EED0003 + 1 + i, 2000000, rng.uniform(100, 5000), 0.01)
         insts.append((main_d, s))
+        if i % 64 == 63:
+            log(f"bench: batch stimulus sets {i + 1}/{n_sets} generated")
     for i in range(max(1, n_sets // 4)):
         c = int(round(10 ** rng.uniform(3.7, 5.7) * scale))
         c = max(500, c)
         d = gls.Design.generate(1, 0x5EED1003 + i, c, rng.randint(10, 60), max(16, int(1024 * c / 200000)))
         s = gls.Stimulus.generate(d, 0x5EED2003 + i, 2000000, rng.uniform(100, 5000), 0.01)
         insts.append((d, s))
+        if i % 16 == 15:
+            log(f"bench: batch netlists {i + 1}/{max(1, n_sets // 4)} generated")
     # plain: one instance at a time (the first run of each pays nothing extra:
     # its netlist image is uploaded in prepare, outside the timed execute)
     log(f"bench: batch workload generated: {len(insts)} instances")
-    plain_s, plain_ev, saifs = 0.0, 0, []
+    plain_s, plain_ev, saifs, used = 0.0, 0, [], []
     for k, (d, s) in enumerate(insts):
-        r = ctx.prepare(d, s, 0, 2000001, pool_bytes=pool_bytes_for(d, s))
+        r = ctx.prepare(d, s, 0, 2000001)
         r.execute()
         t0 = time.perf_counter()
         r.execute()
         plain_s += time.perf_counter() - t0
-        plain_ev += r.stats()["n_events"]
+        st = r.stats()
+        plain_ev += st["n_events"]
+        used.append(st["pool_used_bytes"])
         saifs.append(r.saif_text())
         r.close()
         if k % 8 == 7:
             log(f"bench: plain instance {k + 1}/{len(insts)} done")
-    b = ctx.prepare_batch([i[0] for i in insts], [i[1] for i in insts], [0] * len(insts),
-                          [2000001] * len(insts),
-                          pool_bytes=sum(pool_bytes_for(d, s) for d, s in insts), graph=True)
-    log("bench: batch prepared, autotuning")
-    tune = b.autotune()
-    log(f"bench: autotune picked {tune['seg_events']} events/segment, "
-        f"{tune['level_blocks']} workgroups, prefetch {tune['prefetch']}")
-    b.execute()  # eager run with the tuned setting, captured as one graph
-    t0 = time.perf_counter()
-    b.execute()  # graph replay: every level of every instance, no host round trip
-    batch_s = time.perf_counter() - t0
-    replayed = b.stats()["eval_launches"] == 1
-    st = b.stats()
-    same = all(b.instance_saif_text(k) == saifs[k] for k in range(len(insts)))
-    b.close()
+    # groups of instances whose waveforms fit the budget together
+    free, _ = ctx.mem_info()
+    budget = int(free * budget_frac)
+    groups, cur, cur_b = [], [], 0
+    for k in range(len(insts)):
+        need = int(used[k] * 1.25) + (8 << 20)
+        if cur and cur_b + need > budget:
+            groups.append((cur, cur_b))
+            cur, cur_b = [], 0
+        cur.append(k)
+        cur_b += need
+    groups.append((cur, cur_b))
+    log(f"bench: {len(groups)} batched run(s) within {budget / 2**30:.0f} GiB")
+    batch_s, batch_ev, same, replayed, tune, knobs = 0.0, 0, True, True, None, {}
+    for gi, (ks, pool) in enumerate(groups):
+        b = ctx.prepare_batch([insts[k][0] for k in ks], [insts[k][1] for k in ks], [0] * len(ks),
+                              [2000001] * len(ks), pool_bytes=pool, pool_grow=True, graph=True,
+                              **knobs)
+        if tune is None:
+            log("bench: batch prepared, autotuning")
+            tune = b.autotune()
+            knobs = {"seg_events": tune["seg_events"], "level_blocks": tune["level_blocks"],
+                     "prefetch": bool(tune["prefetch"])}
+            log(f"bench: autotune picked {tune['seg_events']} events/segment, "
+                f"{tune['level_blocks']} workgroups, prefetch {tune['prefetch']}")
+        b.execute()  # eager run with the tuned setting, captured as one graph
+        t0 = time.perf_counter()
+        b.execute()  # graph replay: every level of every instance, no host round trip
+        batch_s += time.perf_counter() - t0
+        replayed = replayed and b.stats()["eval_launches"] == 1
+        batch_ev += b.stats()["n_events"]
+        same = same and all(b.instance_saif_text(j) == saifs[k] for j, k in enumerate(ks))
+        b.close()
+        log(f"bench: batched run {gi + 1}/{len(groups)} done ({len(ks)} instances)")
     return {"workload": f"{n_sets} stimulus sets on a {cells}-cell netlist + {len(insts) - n_sets} "
-                        f"netlists (log-uniform cells, depth 10-60), 0-2 Mps windows, gap U[100,5000] ps",
-            "instances": len(insts), "gate_events": st["n_events"],
+                        f"netlists (log-uniform {max(500, int(5000 * scale))}-{int(500000 * scale)} "
+                        f"cells, depth 10-60), 0-2 Mps windows, gap U[100,5000] ps",
+            "instances": len(insts), "gate_events": batch_ev, "batched_runs": len(groups),
             "plain_events_per_s": round(plain_ev / plain_s, 1), "plain_s": round(plain_s, 4),
-            "batched_events_per_s": round(st["n_events"] / batch_s, 1), "batched_s": round(batch_s, 4),
+            "batched_events_per_s": round(batch_ev / batch_s, 1), "batched_s": round(batch_s, 4),
             "speedup": round(plain_s / batch_s, 3), "saif_bit_exact_vs_plain": same,
-            "form": "one captured hipGraph of all levels" if replayed else "level-synchronous launches",
+            "events_same_as_plain": batch_ev == plain_ev,
+            "form": "captured hipGraph of all levels per batched run" if replayed
+                    else "level-synchronous launches",
             "autotune": {k: tune[k] for k in ("seg_events", "level_blocks", "prefetch", "best_ms")}}
 
 
@@ -214,9 +244,11 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", choices=["c1", "c2"], default="c1",
+    ap.add_argument("--workload", choices=["c1", "c2", "c3"], default="c1",
                     help="c1: the 200k-cell configuration (default); c2: the 2M-cell, 60-level "
-                         "log-normal-width configuration at gap 200 ps on a shortened window")
+                         "log-normal-width configuration at gap 200 ps on a shortened window; "
+                         "c3: the full batching mix only (256 stimulus sets on the 200k-cell "
+                         "netlist + 64 netlists of 5k-500k cells), batched vs plain")
     ap.add_argument("--cells", type=int, default=None)
     ap.add_argument("--t-end", type=int, default=None)
     ap.add_argument("--seg-events", type=int, default=0)
@@ -238,6 +270,24 @@ def main():
                     help="only run a plain device copy of this many MiB (3 times): the "
                          "reference point that scales FETCH_SIZE/WRITE_SIZE counters to bytes")
     args = ap.parse_args()
+    if args.workload == "c3":
+        if args.gpus != 1:
+            print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus})",
+                  file=sys.stderr)
+            sys.exit(2)
+        import gls
+        ctx = gls.Context(0)
+        t0 = time.time()
+        bt = batched_vs_plain(ctx, 256, 1.0)
+        log(f"bench: batched vs plain {bt['speedup']}x ({time.time() - t0:.1f}s)")
+        print(json.dumps({
+            "metric": METRIC, "value": bt["batched_events_per_s"], "unit": "gate events/s",
+            "n_gpus": 1, "steps": 1, "warmup": 1, "ms_per_step": round(bt["batched_s"] * 1e3, 3),
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u32",
+            "data": "synthetic",
+            "config": {"workload": "batching: " + bt["workload"], "parallelism": "single GPU"},
+            "batched": bt}), flush=True)
+        return
     if args.workload == "c1":
         W = dict(config=1, seed=0x5EED0001, cells=args.cells or 200000, levels=40, pis=1024, gap=500.0,
                  t_end=args.t_end or 20000000)

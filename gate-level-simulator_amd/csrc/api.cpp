@@ -308,6 +308,17 @@ int gls_stim_info(const gls_stim* s, uint64_t* n_signals, uint64_t* n_events) {
 
 void gls_stim_free(gls_stim* s) { delete s; }
 
+int gls_ctx_mem_info(gls_ctx* c, uint64_t* free_bytes, uint64_t* total_bytes) {
+  API_BEGIN
+  if (!c || !free_bytes || !total_bytes) throw GlsError(GLS_E_ARG, "null argument");
+  hip_check(hipSetDevice(c->device), "hipSetDevice");
+  size_t f = 0, t = 0;
+  hip_check(hipMemGetInfo(&f, &t), "hipMemGetInfo");
+  *free_bytes = f;
+  *total_bytes = t;
+  API_END
+}
+
 int gls_ctx_create(int device, gls_ctx** out) {
   API_BEGIN
   if (!out) throw GlsError(GLS_E_ARG, "null argument");

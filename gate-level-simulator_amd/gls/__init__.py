@@ -112,6 +112,7 @@ SIGNATURES = [
     ("gls_stim_info", _I, [_P, ctypes.POINTER(_U64), ctypes.POINTER(_U64)]),
     ("gls_stim_free", None, [_P]),
     ("gls_ctx_create", _I, [_I, _PP]),
+    ("gls_ctx_mem_info", _I, [_P, ctypes.POINTER(_U64), ctypes.POINTER(_U64)]),
     ("gls_ctx_destroy", None, [_P]),
     ("gls_prepare", _I, [_P, _P, _P, _U64, _U64, ctypes.POINTER(SimOpts), _PP]),
     ("gls_run_execute", _I, [_P]),
@@ -386,6 +387,12 @@ class Context(_Handle):
         h = ctypes.c_void_p()
         _check(load().gls_ctx_create(device, ctypes.byref(h)))
         super().__init__(h.value)
+
+    def mem_info(self):
+        """(free, total) bytes of the device's memory."""
+        f, t = _U64(), _U64()
+        _check(load().gls_ctx_mem_info(self.ptr, ctypes.byref(f), ctypes.byref(t)))
+        return f.value, t.value
 
     @staticmethod
     def _opts(pool_bytes, seg_events, keep_waves, path, level_blocks=0, prefetch=True,

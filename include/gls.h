@@ -82,6 +82,9 @@ void gls_stim_free(gls_stim* s);
 /* ------------------------------------------------------------ device --- */
 int gls_ctx_create(int device, gls_ctx** out);
 void gls_ctx_destroy(gls_ctx* c);
+/* Free and total device memory of the context's device (hipMemGetInfo), for
+ * callers that size batches to fit. */
+int gls_ctx_mem_info(gls_ctx* c, uint64_t* free_bytes, uint64_t* total_bytes);
 
 typedef struct {
   uint64_t pool_bytes;   /* event pool size, fixed (0: sized from the stimulus and grown
