@@ -43,6 +43,7 @@ struct gls_run {
   std::unique_ptr<Job> job;
   std::vector<RunInst> inst;
   bool keep_waves = false;
+  bool saif_tc = false;  // SAIF text carries (TC n) records
   bool executed = false;
   std::vector<uint64_t> saif;  // 3 per net of the union net space
   std::vector<uint64_t> woff;
@@ -89,18 +90,21 @@ RunInst make_inst(gls_design* d, const gls_stim* s, uint64_t dumpon, uint64_t du
   return r;
 }
 
-std::string inst_saif(const RunInst& in, const std::vector<uint64_t>& saif, uint32_t base) {
+std::string inst_saif(const RunInst& in, const std::vector<uint64_t>& saif, uint32_t base,
+                      bool with_tc) {
   std::vector<SaifRecord> recs;
   for (size_t n = 0; n < in.names.size(); ++n) {
     if (!in.in_saif[n]) continue;
     SaifRecord s;
+    const uint64_t* c = &saif[(size_t)kSaifWords * (base + n)];
     s.name = in.names[n];
-    s.t0 = saif[3 * (base + n) + 0];
-    s.t1 = saif[3 * (base + n) + 1];
-    s.tx = saif[3 * (base + n) + 2];
+    s.t0 = c[0];
+    s.t1 = c[1];
+    s.tx = c[2];
+    s.tc = c[3];
     recs.push_back(std::move(s));
   }
-  return saif_text(std::move(recs), in.start, in.end, in.timescale_ps);
+  return saif_text(std::move(recs), in.start, in.end, in.timescale_ps, with_tc);
 }
 }  // namespace gls
 
@@ -343,6 +347,8 @@ void gls_ctx_destroy(gls_ctx* c) {
 static void apply_opts(Job& job, const gls_sim_opts* opts) {
   if (!opts) return;
   if (opts->path > GLS_PATH_TWO_PASS) throw GlsError(GLS_E_ARG, "unknown path");
+  if (opts->saif_tc && opts->path != GLS_PATH_FUSED)
+    throw GlsError(GLS_E_UNSUPPORTED, "toggle counts need the fused path");
   if (opts->seg_events) job.seg_target = opts->seg_events;
   job.mode = (int)opts->path;
   job.level_blocks_opt = opts->level_blocks;
@@ -353,6 +359,7 @@ static void apply_opts(Job& job, const gls_sim_opts* opts) {
   job.stack_mode = opts->hbm_stacks ? 1u : 0u;
   job.pool_grow = opts->pool_grow ? 1u : 0u;
   job.ovf_words_opt = opts->ovf_words;
+  job.saif_tc = opts->saif_tc ? 1u : 0u;
 }
 
 int gls_prepare_batch(gls_ctx* c, size_t n, const gls_design* const* designs,
@@ -379,6 +386,7 @@ int gls_prepare_batch(gls_ctx* c, size_t n, const gls_design* const* designs,
     inst.push_back(std::move(I));
   }
   r->keep_waves = opts && opts->keep_waves;
+  r->saif_tc = opts && opts->saif_tc;
   r->job = std::make_unique<Job>(c->device, c->stream);
   apply_opts(*r->job, opts);
   r->job->prepare(inst, opts ? opts->pool_bytes / 4 : 0);
@@ -443,7 +451,7 @@ static const RunInst& inst_of(const gls_run* r, size_t i) {
 
 static std::string make_saif(const gls_run* r, size_t i) {
   if (!r->executed) throw GlsError(GLS_E_STATE, "run not executed");
-  return inst_saif(inst_of(r, i), r->saif, r->job->net_base(i));
+  return inst_saif(inst_of(r, i), r->saif, r->job->net_base(i), r->saif_tc);
 }
 
 int gls_run_instances(const gls_run* r, size_t* n) {
@@ -486,7 +494,19 @@ int gls_run_net_saif(const gls_run* r, const char* net, uint64_t v[3]) {
   const RunInst& in = inst_of(r, 0);
   auto it = in.by_name.find(net);
   if (it == in.by_name.end()) throw GlsError(GLS_E_ARG, std::string("unknown net ") + net);
-  for (int i = 0; i < 3; ++i) v[i] = r->saif[3 * it->second + i];
+  for (int i = 0; i < 3; ++i) v[i] = r->saif[(size_t)kSaifWords * it->second + i];
+  API_END
+}
+
+int gls_run_net_toggles(const gls_run* r, const char* net, uint64_t* tc) {
+  API_BEGIN
+  if (!r || !net || !tc) throw GlsError(GLS_E_ARG, "null argument");
+  if (!r->executed) throw GlsError(GLS_E_STATE, "run not executed");
+  if (!r->saif_tc) throw GlsError(GLS_E_STATE, "toggle counts need the saif_tc option");
+  const RunInst& in = inst_of(r, 0);
+  auto it = in.by_name.find(net);
+  if (it == in.by_name.end()) throw GlsError(GLS_E_ARG, std::string("unknown net ") + net);
+  *tc = r->saif[(size_t)kSaifWords * it->second + 3];
   API_END
 }
 

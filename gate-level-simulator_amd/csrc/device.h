@@ -78,9 +78,19 @@ struct alignas(16) TileOut {
   uint32_t ft, lt;      // times of the first and last entries
   uint32_t flags;       // first value | last value << 2 | has entries << 4
   uint32_t t0, t1, tx;  // SAIF durations between the tile's own entries (clipped)
-  uint32_t pad;
+  uint32_t tc;          // reserved (0); toggle counts inside tiles are added by the compaction
 };
 static_assert(sizeof(TileOut) == 32, "TileOut layout");
+
+// SAIF counters per net: T0, T1, TX durations and TC, the number of direct
+// 0<->1 transitions at times in (dumpon, dumpoff].
+constexpr uint32_t kSaifWords = 4;
+
+// A value change from p to v is a toggle when it is 0 -> 1 or 1 -> 0
+// (codes 0 and 3; X <-> Z, codes 1 and 2, also differ by 3 and is excluded).
+__host__ __device__ inline bool is_toggle(uint32_t p, uint32_t v) {
+  return (p ^ v) == 3u && (v == 0u || v == 3u);
+}
 
 // Arguments of the fused level (walk) kernel.
 struct LevelArgs {

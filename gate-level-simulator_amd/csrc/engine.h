@@ -112,6 +112,7 @@ class Job {
   int mode = 0;                   // 0: fused LDS-tiled level kernel, 1: two-pass kernels
   uint32_t level_blocks_opt = 0;  // workgroups per fused launch (0: occupancy limit)
   uint32_t prefetch = 1;          // stage the next tile before storing the current one
+  uint32_t saif_tc = 0;           // count 0<->1 transitions per net (SAIF TC)
   uint64_t ovf_words_opt = 0;     // overflow stack words (0: from 2^24, doubled as needed)
   // 1: after a plain execution, capture the whole level sequence (no host
   // round trips) into a hipGraph and replay it on later executions

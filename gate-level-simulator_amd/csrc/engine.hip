@@ -55,7 +55,9 @@ hipError_t launch_gate_carry(const GateRec* gates, uint32_t ng, const uint64_t* 
 hipError_t launch_compact(const TileDesc* tiles, const uint64_t* ntiles_dev, uint64_t ntiles_cap,
                           const TileOut* tout, const uint32_t* tile_off, const uint64_t* scr_base,
                           const uint32_t* scratch, const uint64_t* gate_base, uint32_t* pool,
-                          uint64_t* woff, uint32_t* wlen, const unsigned int* err, hipStream_t st);
+                          uint64_t* woff, uint32_t* wlen, const unsigned int* err,
+                          unsigned long long* saif, const unsigned long long* win, uint32_t count_tc,
+                          hipStream_t st);
 hipError_t launch_set_scal(uint64_t* scal, uint64_t bump, hipStream_t st);
 hipError_t launch_level(const LevelArgs& a, unsigned nblocks, hipStream_t st);
 int level_max_blocks(uint32_t cap, bool stacks_in_lds);
@@ -442,7 +444,7 @@ void Job::prepare(const std::vector<SimInstance>& inst, uint64_t pool_words) {
 
   d_woff_ = (uint64_t*)dmalloc(n_nets_ * sizeof(uint64_t));
   d_wlen_ = (uint32_t*)dmalloc(n_nets_ * sizeof(uint32_t));
-  d_saif_ = (unsigned long long*)dmalloc((size_t)n_nets_ * 3 * sizeof(unsigned long long));
+  d_saif_ = (unsigned long long*)dmalloc((size_t)n_nets_ * kSaifWords * sizeof(unsigned long long));
   d_saif_nets_ = (uint32_t*)dmalloc(saif_net_list_.size() * 4);
   d_saif_job_ = (uint32_t*)dmalloc(saif_net_job_.size() * 4);
   d_win_ = (unsigned long long*)dmalloc(win_.size() * 8);
@@ -653,7 +655,7 @@ void Job::run_fused(std::vector<hipEvent_t>& ev_eval, bool device_driven) {
     HC(launch_pool_guard(d_scal_, pool_limit, stream_));
     HC(launch_compact(d_tiles_, d_scal_ + S_TOT_A, ntiles, d_tout_, d_toff_, d_scrbase_, scratch,
                       d_outoff_, d_pool_, d_woff_, d_wlen_, (const unsigned int*)(d_scal_ + S_ERR),
-                      stream_));
+                      d_saif_, d_win_, saif_tc, stream_));
     HC(launch_bump(bump, d_scal_ + S_TOT_OUT, stream_));
   }
 }
@@ -743,7 +745,7 @@ void Job::run() {
   HC(hipMemcpyAsync(d_woff_, h_woff_init_.data(), n_nets_ * 8ull, hipMemcpyHostToDevice, stream_));
   HC(hipMemcpyAsync(d_wlen_, h_wlen_init_.data(), n_nets_ * 4ull, hipMemcpyHostToDevice, stream_));
   HC(hipMemcpyAsync(d_scal_, scal, sizeof scal, hipMemcpyHostToDevice, stream_));
-  HC(launch_fill_u64(d_saif_, (uint64_t)n_nets_ * 3, 0ull, stream_));
+  HC(launch_fill_u64(d_saif_, (uint64_t)n_nets_ * kSaifWords, 0ull, stream_));
   HC(hipStreamSynchronize(stream_));
 
   HC(hipEventRecord(e0, stream_));
@@ -808,7 +810,7 @@ void Job::build_graph() {
     HC(hipMemcpyAsync(d_woff_, d_woff_init_, n_nets_ * 8ull, hipMemcpyDeviceToDevice, stream_));
     HC(hipMemcpyAsync(d_wlen_, d_wlen_init_, n_nets_ * 4ull, hipMemcpyDeviceToDevice, stream_));
     HC(launch_set_scal(d_scal_, stim_words_, stream_));
-    HC(launch_fill_u64(d_saif_, (uint64_t)n_nets_ * 3, 0ull, stream_));
+    HC(launch_fill_u64(d_saif_, (uint64_t)n_nets_ * kSaifWords, 0ull, stream_));
     HC(launch_saif_nets(d_saif_nets_, d_saif_job_, (uint32_t)saif_net_list_.size(), d_woff_,
                         d_wlen_, d_pool_, d_saif_, d_win_, stream_));
     std::vector<hipEvent_t> none;
@@ -854,7 +856,7 @@ void Job::finish_stats(const uint64_t* h, double ms) {
 }
 
 void Job::fetch_saif(std::vector<uint64_t>& v) const {
-  v.resize((size_t)n_nets_ * 3);
+  v.resize((size_t)n_nets_ * kSaifWords);
   hip_check(hipMemcpyAsync(v.data(), d_saif_, v.size() * 8, hipMemcpyDeviceToHost, stream_),
             "saif D2H");
   hip_check(hipStreamSynchronize(stream_), "saif D2H");

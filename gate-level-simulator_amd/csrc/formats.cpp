@@ -901,7 +901,8 @@ std::string vcd_text(const Stimulus& st) {
 // ---------------------------------------------------------------- SAIF ----
 // Record layout of src/simulate.cpp:85-88,162-163,303-304,373-378 with the
 // header of src/simulate.cu:457-481; nets in lexicographic order.
-std::string saif_text(std::vector<SaifRecord> recs, uint64_t start, uint64_t end, int timescale_ps) {
+std::string saif_text(std::vector<SaifRecord> recs, uint64_t start, uint64_t end, int timescale_ps,
+                      bool with_tc) {
   std::sort(recs.begin(), recs.end(),
             [](const SaifRecord& a, const SaifRecord& b) { return a.name < b.name; });
   std::string o;
@@ -910,7 +911,7 @@ std::string saif_text(std::vector<SaifRecord> recs, uint64_t start, uint64_t end
   long long res = timescale_ps;
   if (timescale_ps % 1000000 == 0) { unit = "us"; res = timescale_ps / 1000000; }
   else if (timescale_ps % 1000 == 0) { unit = "ns"; res = timescale_ps / 1000; }
-  char buf[160];
+  char buf[256];
   std::snprintf(buf, sizeof buf, "(TIMESCALE %lld %s)\n(DURATION %llu)\n(INSTANCE root\n   (NET\n",
                 res, unit, (unsigned long long)(end - start));
   o += buf;
@@ -923,8 +924,13 @@ std::string saif_text(std::vector<SaifRecord> recs, uint64_t start, uint64_t end
       continue;
     }
     o += "      (" + r.name + "\n";
-    std::snprintf(buf, sizeof buf, "         (T0 %llu) (T1 %llu) (TX %llu)\n      )\n",
-                  (unsigned long long)r.t0, (unsigned long long)r.t1, (unsigned long long)r.tx);
+    if (with_tc)
+      std::snprintf(buf, sizeof buf, "         (T0 %llu) (T1 %llu) (TX %llu) (TC %llu)\n      )\n",
+                    (unsigned long long)r.t0, (unsigned long long)r.t1, (unsigned long long)r.tx,
+                    (unsigned long long)r.tc);
+    else
+      std::snprintf(buf, sizeof buf, "         (T0 %llu) (T1 %llu) (TX %llu)\n      )\n",
+                    (unsigned long long)r.t0, (unsigned long long)r.t1, (unsigned long long)r.tx);
     o += buf;
   }
   o += "   )\n\n";
@@ -933,9 +939,15 @@ std::string saif_text(std::vector<SaifRecord> recs, uint64_t start, uint64_t end
     std::string inst = r->name.substr(0, lb);
     std::string net = r->name.substr(lb + 1, rb - lb - 1);
     o += "   (INSTANCE " + inst + "\n      (NET\n         (" + net + "\n";
-    std::snprintf(buf, sizeof buf,
-                  "            (T0 %llu) (T1 %llu) (TX %llu)\n         )\n      )\n   )\n",
-                  (unsigned long long)r->t0, (unsigned long long)r->t1, (unsigned long long)r->tx);
+    if (with_tc)
+      std::snprintf(buf, sizeof buf,
+                    "            (T0 %llu) (T1 %llu) (TX %llu) (TC %llu)\n         )\n      )\n   )\n",
+                    (unsigned long long)r->t0, (unsigned long long)r->t1, (unsigned long long)r->tx,
+                    (unsigned long long)r->tc);
+    else
+      std::snprintf(buf, sizeof buf,
+                    "            (T0 %llu) (T1 %llu) (TX %llu)\n         )\n      )\n   )\n",
+                    (unsigned long long)r->t0, (unsigned long long)r->t1, (unsigned long long)r->tx);
     o += buf;
   }
   o += ")\n";

@@ -35,9 +35,12 @@ std::string vcd_text(const Stimulus& st);
 struct SaifRecord {
   std::string name;
   uint64_t t0 = 0, t1 = 0, tx = 0;
+  uint64_t tc = 0;  // 0<->1 transitions in (start, end]
 };
+// with_tc: each record also carries (TC n) after (TX ...); off by default,
+// which is the reference's three-field record (src/simulate.cu:133-135).
 std::string saif_text(std::vector<SaifRecord> recs, uint64_t start, uint64_t end,
-                      int timescale_ps = 1);
+                      int timescale_ps = 1, bool with_tc = false);
 
 std::string read_file(const std::string& path);
 void write_file(const std::string& path, const std::string& data);

@@ -286,10 +286,11 @@ __global__ void k_gate_scan(const GateDev* __restrict__ gates, uint32_t ng,
   // atomics, so plain stores initialise the counters
   unsigned long long acc[3] = {0, 0, 0};
   if (cv != 4) saif_add(acc, cv, ct, end > ct ? end : ct, start, end);
-  unsigned long long* dst = saif + 3ull * gates[g].out_net;
+  unsigned long long* dst = saif + (uint64_t)kSaifWords * gates[g].out_net;
   dst[0] = acc[0];
   dst[1] = acc[1];
   dst[2] = acc[2];
+  dst[3] = 0;
 }
 
 // ------------------------------------------------------------------ emit ----
@@ -315,21 +316,23 @@ __global__ __launch_bounds__(256) void k_emit(
   const uint32_t* src = scratch + S.scr_off;
   uint32_t* dst = pool + base + C.pos;
   uint32_t pv = C.carry_v, pt = C.carry_t;
-  unsigned long long acc[3] = {0, 0, 0};
+  unsigned long long acc[3] = {0, 0, 0}, tc = 0;
   uint32_t w = 0;
   for (uint32_t i = 0; i < S.n_raw; ++i) {
     uint32_t e = src[i];
     uint32_t v = e & 3u, t = e >> 2;
     if (pv != 4) saif_add(acc, pv, pt, t, start, end);
     else if (t > 0) saif_add(acc, 1u, 0, t, start, end);
+    if (pv != 4 && is_toggle(pv, v) && t > start && t <= end) ++tc;
     if (v != pv) dst[w++] = e;
     pv = v;
     pt = t;
   }
-  unsigned long long* a = saif + 3ull * out_net;
+  unsigned long long* a = saif + (uint64_t)kSaifWords * out_net;
   if (acc[0]) atomicAdd(a + 0, acc[0]);
   if (acc[1]) atomicAdd(a + 1, acc[1]);
   if (acc[2]) atomicAdd(a + 2, acc[2]);
+  if (tc) atomicAdd(a + 3, tc);
 }
 
 // Stimulus / constant waveforms: SAIF durations, one thread per net.
@@ -349,21 +352,24 @@ __global__ void k_saif_nets(const uint32_t* __restrict__ nets, const uint32_t* _
   const uint64_t start = win[2 * job[i]], end = win[2 * job[i] + 1];
   const uint32_t* w = pool + woff[net];
   const uint32_t n = wlen[net];
-  unsigned long long acc[3] = {0, 0, 0};
+  unsigned long long acc[4] = {0, 0, 0, 0};
   if (lane == 0) saif_add(acc, 1u, 0, n ? (w[0] >> 2) : (end > 0 ? end : 0), start, end);
   for (uint32_t q = lane; q < n; q += 64) {
     const uint32_t e = w[q];
     const uint64_t t = e >> 2;
     const uint64_t tn = q + 1 < n ? (uint64_t)(w[q + 1] >> 2) : (end > t ? end : t);
     saif_add(acc, e & 3u, t, tn, start, end);
+    // stimulus waveforms may repeat a value (scalars are not de-duplicated)
+    if (q > 0 && is_toggle(w[q - 1] & 3u, e & 3u) && t > start && t <= end) ++acc[3];
   }
-  for (int k = 0; k < 3; ++k)
+  for (int k = 0; k < 4; ++k)
     for (int o = 32; o > 0; o >>= 1) acc[k] += __shfl_down(acc[k], o, 64);
   if (lane == 0) {
-    unsigned long long* d = saif + 3ull * net;
+    unsigned long long* d = saif + (uint64_t)kSaifWords * net;
     d[0] = acc[0];
     d[1] = acc[1];
     d[2] = acc[2];
+    d[3] = acc[3];
   }
 }
 

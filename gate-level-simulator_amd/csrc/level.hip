@@ -569,7 +569,7 @@ __device__ __forceinline__ void emit_tile(const LevelArgs& A, uint64_t tk, const
     o.t0 = t0;
     o.t1 = t1;
     o.tx = tx;
-    o.pad = 0;
+    o.tc = 0;  // counted by k_compact when toggle counts are asked for
     A.tout[tk] = o;
     wave_ev += ev;
   }
@@ -703,7 +703,7 @@ __global__ void k_gate_carry(const GateRec* __restrict__ gates, uint32_t ng,
   const unsigned long long start = win[2 * G.job], end = win[2 * G.job + 1];
   const uint64_t t0 = tile_base[g], n = ntile[g];
   uint32_t cnt = 0, lv = 0, lt = 0, has = 0;
-  unsigned long long acc[3] = {0, 0, 0};
+  unsigned long long acc[3] = {0, 0, 0}, tc = 0;
   for (uint64_t t = t0; t < t0 + n; ++t) {
     const TileOut S = tout[t];
     if (!((S.flags >> 4) & 1u)) {  // no entry in this tile
@@ -716,6 +716,8 @@ __global__ void k_gate_carry(const GateRec* __restrict__ gates, uint32_t ng,
     acc[0] += S.t0;
     acc[1] += S.t1;
     acc[2] += S.tx;
+    // the tile's first entry against the previous tile's last value
+    tc += S.tc + ((has && is_toggle(lv, S.flags & 3u) && S.ft > start && S.ft <= end) ? 1u : 0u);
     cnt += S.cnt - skip;
     lv = (S.flags >> 2) & 3u;
     lt = S.lt;
@@ -724,10 +726,11 @@ __global__ void k_gate_carry(const GateRec* __restrict__ gates, uint32_t ng,
   if (has) saif_acc(acc, lv, lt, end > lt ? end : (unsigned long long)lt, start, end);
   gate_cnt[g] = cnt;
   wlen[G.out_net] = cnt;
-  unsigned long long* sa = saif + 3ull * G.out_net;
+  unsigned long long* sa = saif + (uint64_t)kSaifWords * G.out_net;
   sa[0] = acc[0];
   sa[1] = acc[1];
   sa[2] = acc[2];
+  sa[3] = tc;
 }
 
 // Copy every tile's joined output from the scratch into the gate's dense
@@ -739,7 +742,8 @@ __global__ void k_compact(const TileDesc* __restrict__ tiles, const uint64_t* __
                           const uint64_t* __restrict__ scr_base, const uint32_t* __restrict__ scratch,
                           const uint64_t* __restrict__ gate_base, uint32_t* __restrict__ pool,
                           uint64_t* __restrict__ woff, uint32_t* __restrict__ wlen,
-                          const unsigned int* __restrict__ err) {
+                          const unsigned int* __restrict__ err, unsigned long long* __restrict__ saif,
+                          const unsigned long long* __restrict__ win, uint32_t count_tc) {
   const int lane = threadIdx.x & 63;
   const uint64_t waves = (uint64_t)gridDim.x * (blockDim.x / 64);
   const bool failed = (*err & ERR_CAPACITY) != 0;
@@ -761,6 +765,20 @@ __global__ void k_compact(const TileDesc* __restrict__ tiles, const uint64_t* __
     const uint32_t* src = scratch + tile_scratch(scr_base, D) + skip;
     uint32_t* dst = pool + gate_base[D.g] + off;
     for (uint32_t q = lane; q < n; q += 64) dst[q] = src[q];
+    if (count_tc) {
+      // SAIF TC inside the tile: 0<->1 changes between its consecutive
+      // entries (the first entry against the previous tile is k_gate_carry's)
+      const uint32_t* e = src - skip;
+      const unsigned long long start = win[2 * D.job], end = win[2 * D.job + 1];
+      uint32_t c = 0;
+      for (uint32_t q = lane + 1; q < S.cnt; q += 64) {
+        const uint32_t a = e[q - 1], b = e[q];
+        const unsigned long long t = b >> 2;
+        c += (is_toggle(a & 3u, b & 3u) && t > start && t <= end) ? 1u : 0u;
+      }
+      c = wave_sum(c);
+      if (lane == 0 && c) atomicAdd(saif + (uint64_t)kSaifWords * D.out_net + 3, (unsigned long long)c);
+    }
   }
 }
 
@@ -958,11 +976,13 @@ hipError_t launch_gate_carry(const GateRec* gates, uint32_t ng, const uint64_t* 
 hipError_t launch_compact(const TileDesc* tiles, const uint64_t* ntiles_dev, uint64_t ntiles_cap,
                           const TileOut* tout, const uint32_t* tile_off, const uint64_t* scr_base,
                           const uint32_t* scratch, const uint64_t* gate_base, uint32_t* pool,
-                          uint64_t* woff, uint32_t* wlen, const unsigned int* err, hipStream_t st) {
+                          uint64_t* woff, uint32_t* wlen, const unsigned int* err,
+                          unsigned long long* saif, const unsigned long long* win, uint32_t count_tc,
+                          hipStream_t st) {
   if (ntiles_cap == 0) return hipSuccess;
   const uint64_t blocks = std::min<uint64_t>((ntiles_cap + 3) / 4, 8192);  // 4 wavefronts each
   k_compact<<<(unsigned)blocks, 256, 0, st>>>(tiles, ntiles_dev, tout, tile_off, scr_base, scratch,
-                                              gate_base, pool, woff, wlen, err);
+                                              gate_base, pool, woff, wlen, err, saif, win, count_tc);
   return hipGetLastError();
 }
 

@@ -218,7 +218,7 @@ std::string Server::saif(uint64_t id) {
   if (it == jobs_.end()) throw GlsError(GLS_E_ARG, "unknown job");
   Rec* r = it->second.get();
   if (r->info.status != 0) throw GlsError(GLS_E_STATE, "job not finished successfully");
-  return inst_saif(r->inst, r->saif, 0);
+  return inst_saif(r->inst, r->saif, 0, false);
 }
 
 void Server::drain() {

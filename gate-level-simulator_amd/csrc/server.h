@@ -36,7 +36,9 @@ struct RunInst {
   int timescale_ps = 1;
 };
 
-std::string inst_saif(const RunInst& in, const std::vector<uint64_t>& saif, uint32_t base);
+// SAIF text of one instance; with_tc adds each net's (TC n) toggle count.
+std::string inst_saif(const RunInst& in, const std::vector<uint64_t>& saif, uint32_t base,
+                      bool with_tc);
 
 struct ServerOptions {
   uint64_t device_budget = 0;  // bytes of HBM the server may use (0: 90% of free)

@@ -36,7 +36,8 @@ class SimOpts(ctypes.Structure):
                 ("keep_waves", ctypes.c_uint32), ("path", ctypes.c_uint32),
                 ("level_blocks", ctypes.c_uint32), ("no_prefetch", ctypes.c_uint32),
                 ("graph", ctypes.c_uint32), ("hbm_stacks", ctypes.c_uint32),
-                ("pool_grow", ctypes.c_uint32), ("ovf_words", ctypes.c_uint32)]
+                ("pool_grow", ctypes.c_uint32), ("ovf_words", ctypes.c_uint32),
+                ("saif_tc", ctypes.c_uint32)]
 
 
 PATH_FUSED = 0
@@ -107,6 +108,7 @@ SIGNATURES = [
     ("gls_design_free", None, [_P]),
     ("gls_stim_from_vcd", _I, [_S, _PP]),
     ("gls_stim_from_vcd_threads", _I, [_S, ctypes.c_int, _PP]),
+    ("gls_run_net_toggles", _I, [_P, _S, ctypes.POINTER(_U64)]),
     ("gls_stim_generate", _I, [_P, _U64, ctypes.c_uint32, ctypes.c_double, ctypes.c_double, _PP]),
     ("gls_stim_write_vcd", _I, [_P, _S]),
     ("gls_stim_info", _I, [_P, ctypes.POINTER(_U64), ctypes.POINTER(_U64)]),
@@ -298,6 +300,12 @@ class Run(_Handle):
         _check(load().gls_run_net_saif(self.ptr, _b(net), v))
         return tuple(v)
 
+    def net_toggles(self, net):
+        """0<->1 transitions of a net in (dumpon, dumpoff]."""
+        v = _U64()
+        _check(load().gls_run_net_toggles(self.ptr, _b(net), ctypes.byref(v)))
+        return v.value
+
     def waveform(self, net):
         n = ctypes.c_size_t()
         _check(load().gls_run_waveform(self.ptr, _b(net), None, 0, ctypes.byref(n)))
@@ -396,24 +404,24 @@ class Context(_Handle):
 
     @staticmethod
     def _opts(pool_bytes, seg_events, keep_waves, path, level_blocks=0, prefetch=True,
-              graph=False, hbm_stacks=False, pool_grow=False, ovf_words=0):
+              graph=False, hbm_stacks=False, pool_grow=False, ovf_words=0, saif_tc=False):
         return SimOpts(pool_bytes, seg_events, 1 if keep_waves else 0, path, level_blocks,
                        0 if prefetch else 1, 1 if graph else 0, 1 if hbm_stacks else 0,
-                       1 if pool_grow else 0, ovf_words)
+                       1 if pool_grow else 0, ovf_words, 1 if saif_tc else 0)
 
     def prepare(self, design, stim, dumpon, dumpoff, pool_bytes=0, seg_events=0, keep_waves=False,
                 path=PATH_FUSED, level_blocks=0, prefetch=True, graph=False,
-                hbm_stacks=False, pool_grow=False, ovf_words=0):
+                hbm_stacks=False, pool_grow=False, ovf_words=0, saif_tc=False):
         h = ctypes.c_void_p()
         o = self._opts(pool_bytes, seg_events, keep_waves, path, level_blocks, prefetch, graph,
-                       hbm_stacks, pool_grow, ovf_words)
+                       hbm_stacks, pool_grow, ovf_words, saif_tc)
         _check(load().gls_prepare(self.ptr, design.ptr, stim.ptr, dumpon, dumpoff, ctypes.byref(o),
                                   ctypes.byref(h)))
         return Run(h.value)
 
     def prepare_batch(self, designs, stims, dumpons, dumpoffs, pool_bytes=0, seg_events=0,
                       keep_waves=False, level_blocks=0, prefetch=True,
-                      graph=False, hbm_stacks=False, pool_grow=False, ovf_words=0):
+                      graph=False, hbm_stacks=False, pool_grow=False, ovf_words=0, saif_tc=False):
         n = len(designs)
         assert len(stims) == n and len(dumpons) == n and len(dumpoffs) == n
         D = (ctypes.c_void_p * n)(*[d.ptr for d in designs])
@@ -422,17 +430,17 @@ class Context(_Handle):
         off = (_U64 * n)(*dumpoffs)
         h = ctypes.c_void_p()
         o = self._opts(pool_bytes, seg_events, keep_waves, PATH_FUSED, level_blocks, prefetch,
-                       graph, hbm_stacks, pool_grow, ovf_words)
+                       graph, hbm_stacks, pool_grow, ovf_words, saif_tc)
         _check(load().gls_prepare_batch(self.ptr, n, D, S, on, off, ctypes.byref(o),
                                         ctypes.byref(h)))
         return Run(h.value)
 
     def simulate(self, design, stim, dumpon, dumpoff, pool_bytes=0, seg_events=0, keep_waves=False,
                  path=PATH_FUSED, level_blocks=0, prefetch=True, graph=False,
-                 hbm_stacks=False, pool_grow=False, ovf_words=0):
+                 hbm_stacks=False, pool_grow=False, ovf_words=0, saif_tc=False):
         h = ctypes.c_void_p()
         o = self._opts(pool_bytes, seg_events, keep_waves, path, level_blocks, prefetch, graph,
-                       hbm_stacks, pool_grow, ovf_words)
+                       hbm_stacks, pool_grow, ovf_words, saif_tc)
         _check(load().gls_simulate(self.ptr, design.ptr, stim.ptr, dumpon, dumpoff, ctypes.byref(o),
                                    ctypes.byref(h)))
         return Run(h.value)

@@ -63,6 +63,14 @@ module GEN_TIE_D1 (a,z);
   output z;
   and (z, a, 1'b1);
 endmodule
+module GEN_HA_D1 (a,b,s,co);
+  input a;
+  input b;
+  output s;
+  output co;
+  xor (s, a, b);
+  and (co, a, b);
+endmodule
 """
 
 C = {"0": 0, "x": 1, "z": 2, "1": 3}
@@ -174,6 +182,19 @@ CASES = [
          # m: (0,X),(10,1),(120,0).  y: t0 f=X (buf of X) no event; t10: f=1 rise 5 -> 15;
          # t120: f=0 fall 6 -> 126
          expect={"m": [(0, X), (10, 3), (120, 0)], "y": [(0, X), (15, 3), (126, 0)]}),
+    # A cell with two outputs: each output is its own gate with its own
+    # IOPATHs; the chain is evaluated up to the function driving that output.
+    dict(name="half_adder_two_outputs",
+         insts=["GEN_HA_D1 U0 (.a(a), .b(b), .s(y), .co(c));"],
+         sdf={"U0": ["(IOPATH a s (10:10:10) (11:11:11))", "(IOPATH b s (12:12:12) (13:13:13))",
+                     "(IOPATH a co (20:20:20) (21:21:21))", "(IOPATH b co (22:22:22) (23:23:23))"]},
+         stim={"a": [(0, 0), (100, 1), (300, 0)], "b": [(0, 0), (200, 1)]},
+         # t0: both inputs, s=0 and co=0 fall, num_in=2 -> s min(11,13)=11, co min(21,23)=21.
+         # t100: a->1: s=1 rise, one related IOPATH a->s -> 10 -> (110,1); co stays 0.
+         # t200: b->1: s=0 fall via b->s 13 -> (213,0); co=1 rise via b->co 22 -> (222,1).
+         # t300: a->0: s=1 rise via a->s 10 -> (310,1); co=0 fall via a->co 21 -> (321,0).
+         expect={"y": [(0, X), (11, 0), (110, 3), (213, 0), (310, 3)],
+                 "c": [(0, X), (21, 0), (222, 3), (321, 0)]}),
     # Gate tied to a constant through the netlist (1'b0 on an input pin).
     dict(name="const_input",
          insts=["GEN_AND2_D1 U0 (.a1(a), .a2(1'b1), .z(y));"],

@@ -52,12 +52,12 @@ def test_binding_keyword_options():
     import inspect
     fields = [n for n, _ in gls.SimOpts._fields_]
     assert fields == ["pool_bytes", "seg_events", "keep_waves", "path", "level_blocks",
-                      "no_prefetch", "graph", "hbm_stacks", "pool_grow", "ovf_words"]
-    o = gls.Context._opts(8, 2, True, gls.PATH_FUSED, 3, False, True, True, True, 77)
+                      "no_prefetch", "graph", "hbm_stacks", "pool_grow", "ovf_words", "saif_tc"]
+    o = gls.Context._opts(8, 2, True, gls.PATH_FUSED, 3, False, True, True, True, 77, True)
     assert (o.pool_bytes, o.seg_events, o.keep_waves, o.level_blocks, o.no_prefetch, o.graph,
-            o.hbm_stacks, o.pool_grow, o.ovf_words) == (8, 2, 1, 3, 1, 1, 1, 1, 77)
+            o.hbm_stacks, o.pool_grow, o.ovf_words, o.saif_tc) == (8, 2, 1, 3, 1, 1, 1, 1, 77, 1)
     for fn in (gls.Context.prepare, gls.Context.simulate, gls.Context.prepare_batch):
         params = inspect.signature(fn).parameters
         for k in ("seg_events", "level_blocks", "prefetch", "graph", "hbm_stacks", "pool_grow",
-                  "ovf_words"):
+                  "ovf_words", "saif_tc"):
             assert k in params, (fn.__name__, k)

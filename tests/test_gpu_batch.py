@@ -181,3 +181,14 @@ def test_graph_batch_replays(ctx):
         for k in range(len(insts)):
             assert b.instance_waves_hash(k) == ref.instance_waves_hash(k), (i, k)
             assert b.instance_saif_text(k) == ref.instance_saif_text(k), (i, k)
+
+
+def test_batch_toggle_counts_match_plain(ctx):
+    insts = mixed_instances()[:5]
+    args = ([i[0] for i in insts], [i[1] for i in insts], [i[2] for i in insts],
+            [i[3] for i in insts])
+    b = ctx.prepare_batch(*args, saif_tc=True).execute()
+    for k, (d, s, on, off, _) in enumerate(insts):
+        plain = ctx.simulate(d, s, on, off, saif_tc=True).saif_text()
+        assert "(TC " in plain
+        assert b.instance_saif_text(k) == plain, k

@@ -7,6 +7,7 @@ identical to the oracle's on the same generated inputs.
 import json
 import os
 import random
+import re
 import subprocess
 import sys
 
@@ -227,3 +228,46 @@ def test_pool_exhaustion_is_reported(ctx):
     # the context stays usable after the failure
     r = ctx.simulate(d, s, 0, 200001, keep_waves=True)
     assert "%016x" % r.waves_hash() == GOLDEN["c0_small"]["waves_hash"]
+
+
+def _toggles(wave, start, end):
+    """0<->1 transitions of a waveform [(time, value code)] in (start, end]."""
+    n = 0
+    for (ta, va), (tb, vb) in zip(wave, wave[1:]):
+        if {va, vb} == {0, 3} and start < tb <= end:
+            n += 1
+    return n
+
+
+def test_toggle_counts(ctx, tmp_path):
+    # closed form on a known-answer case: y = (0,X)(10,1)(120,0)(210,1),
+    # a = (0,0)(100,1)(200,0): two toggles each in (0, 1000]
+    case = next(c for c in kat_cases.CASES if c["name"] == "inv_basic")
+    paths = kat_cases.write_case(case, str(tmp_path))
+    d = gls.Design.from_files(*paths[:3])
+    s = gls.Stimulus.from_vcd(paths[3])
+    r = ctx.simulate(d, s, 0, 1000, saif_tc=True)
+    assert r.net_toggles("y") == 2 and r.net_toggles("a") == 2
+    assert "(T0 " in r.saif_text() and "(TC 2)" in r.saif_text()
+    # every net of a multi-level workload, many tiles per gate and lanes per
+    # tile (segments of 2 events): the counts follow from the waveforms,
+    # which are pinned to the oracle; the other records are unchanged
+    d, s = golden_cases.make("c1_small")
+    on, off = golden_cases.CASES["c1_small"]["window"]
+    g = GOLDEN["c1_small"]
+    for seg in (2, 0):
+        say("toggles", seg)
+        r = ctx.simulate(d, s, on, off, seg_events=seg, keep_waves=True, saif_tc=True)
+        assert "%016x" % r.waves_hash() == g["waves_hash"]
+        plain = ctx.simulate(d, s, on, off, seg_events=seg).saif_text()
+        with_tc = r.saif_text()
+        assert re.sub(r" \(TC \d+\)", "", with_tc) == plain
+        # top-level net records (unconnected instance outputs sit in nested
+        # INSTANCE blocks and have no net name to query)
+        names = re.findall(r"^      \((\S+)\n         \(T0", with_tc, re.M)
+        assert len(names) > 50
+        for net in names:
+            assert r.net_toggles(net) == _toggles(r.waveform(net), on, off), (seg, net)
+    with pytest.raises(gls.GlsError) as e:
+        ctx.simulate(d, s, on, off, saif_tc=True, path=gls.PATH_TWO_PASS)
+    assert e.value.status == "GLS_E_UNSUPPORTED"
