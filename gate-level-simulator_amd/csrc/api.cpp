@@ -347,8 +347,6 @@ void gls_ctx_destroy(gls_ctx* c) {
 static void apply_opts(Job& job, const gls_sim_opts* opts) {
   if (!opts) return;
   if (opts->path > GLS_PATH_TWO_PASS) throw GlsError(GLS_E_ARG, "unknown path");
-  if (opts->saif_tc && opts->path != GLS_PATH_FUSED)
-    throw GlsError(GLS_E_UNSUPPORTED, "toggle counts need the fused path");
   if (opts->seg_events) job.seg_target = opts->seg_events;
   job.mode = (int)opts->path;
   job.level_blocks_opt = opts->level_blocks;

@@ -106,8 +106,8 @@ typedef struct {
                             level is then run again); a fixed size that a level outgrows fails
                             the run with GLS_E_UNSUPPORTED */
   uint32_t saif_tc;      /* 1: SAIF records also carry (TC n), the number of 0<->1 transitions
-                            in (dumpon, dumpoff]; fused path only.  0 (default): the reference's
-                            (T0)(T1)(TX) records */
+                            in (dumpon, dumpoff].  0 (default): the reference's (T0)(T1)(TX)
+                            records */
 } gls_sim_opts;
 
 /* Level-kernel variants: one fused LDS-tiled kernel per level, or the
@@ -151,7 +151,7 @@ int gls_run_saif_text(const gls_run* r, char* buf, size_t cap, size_t* len);
 /* T0/T1/TX durations of one net. */
 int gls_run_net_saif(const gls_run* r, const char* net, uint64_t t0t1tx[3]);
 /* Toggle count of one net: 0<->1 transitions in (dumpon, dumpoff] (no counterpart in the
- * reference, whose SAIF has no TC field; needs the saif_tc option, fused path only). */
+ * reference, whose SAIF has no TC field; needs the saif_tc option). */
 int gls_run_net_toggles(const gls_run* r, const char* net, uint64_t* tc);
 /* Waveform of one net as (time << 2 | value) words (needs keep_waves). */
 int gls_run_waveform(const gls_run* r, const char* net, uint32_t* words, size_t cap, size_t* n);

@@ -268,6 +268,6 @@ def test_toggle_counts(ctx, tmp_path):
         assert len(names) > 50
         for net in names:
             assert r.net_toggles(net) == _toggles(r.waveform(net), on, off), (seg, net)
-    with pytest.raises(gls.GlsError) as e:
-        ctx.simulate(d, s, on, off, saif_tc=True, path=gls.PATH_TWO_PASS)
-    assert e.value.status == "GLS_E_UNSUPPORTED"
+    # the two-pass path counts the same toggles
+    two = ctx.simulate(d, s, on, off, saif_tc=True, path=gls.PATH_TWO_PASS).saif_text()
+    assert two == with_tc
