@@ -105,6 +105,7 @@ struct LevelArgs {
   uint32_t* ovf;              // stacks of tiles that do not fit LDS
   unsigned long long ovf_cap;
   unsigned long long* ovf_used;
+  unsigned long long* ticket;   // next tile to hand out, minus the grid size (0 per level)
   unsigned long long* ev_count;
   unsigned int* err;
   const unsigned long long* win;  // SAIF window (start, end) per instance

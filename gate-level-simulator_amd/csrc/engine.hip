@@ -629,6 +629,7 @@ void Job::run_fused(std::vector<hipEvent_t>& ev_eval, bool device_driven) {
     a.win = d_win_;
     a.cap = cap_;
     a.prefetch = prefetch;
+    a.ticket = (unsigned long long*)(d_scal_ + S_TICKET);
     a.stk = nullptr;
     const unsigned nb = (unsigned)std::min<uint64_t>(ntiles, (uint64_t)std::max(1, level_blocks_));
     if (stack_mode == 1) {
