@@ -5,11 +5,13 @@
 // [c*stride, (c+1)*stride) of the gate's longest input (the "leader"), so
 // segment boundaries are exact event times.
 //
-// k_level (walk), one tile at a time per wavefront, tiles independent:
+// k_level (walk), one tile at a time per wavefront, tiles independent and
+// handed out by ticket:
 //   1. the tile's input ranges (one event before the tile for the starting
 //      values, a D_max look-ahead halo after it), its delay table and its
 //      truth table are copied into LDS with direct-to-LDS loads, each range
-//      followed by an INF sentinel;
+//      followed by an INF sentinel (tiles too long for LDS read HBM instead
+//      and keep their stacks in the overflow buffer);
 //   2. every lane merges its segment's input events in time order (branch
 //      free), evaluates the 4-state truth table (packed in registers for
 //      fan-in <= 3), classifies the output edge, picks the SDF delay with one
@@ -26,7 +28,8 @@
 //   repeats the value before it drops that entry), places them, and finishes
 //   the gate's SAIF record.
 // k_compact: copies each tile's joined entries into the gate's dense
-//   waveform in the pool (coalesced), so the pool holds exactly the outputs.
+//   waveform in the pool (coalesced), so the pool holds exactly the outputs,
+//   and counts SAIF toggles inside tiles when asked.
 //
 // Event rules: src/simulate.cpp:919-1201 (Calculate, AddDelay, CheckDelay),
 // src/simulate.cpp:1365-1506 (Process, GetValueAt).
