@@ -395,6 +395,8 @@ def main():
                      "ms_per_launch": round(eval_ms / max(1, launches), 3),
                      "alg_bytes_per_step": eval_bytes, "ms_per_step": round(eval_ms, 3)},
         "saif_durations_consistent": ok,
+        "tiles_per_step": st["tiles"],
+        "hbm_path_tile_frac": round(st["tiles_hbm"] / max(1, st["tiles"]), 4),
         "cpu_baseline": cpu,
     }
     # the same workload as one captured graph (no host round trip per level)

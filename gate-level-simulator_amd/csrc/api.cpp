@@ -438,6 +438,8 @@ int gls_run_stats_get(const gls_run* r, gls_run_stats* st) {
   st->eval_launches = j.eval_launches;
   st->level_blocks = j.level_blocks;
   st->pool_bytes = j.pool_words * 4;
+  st->tiles = j.tiles;
+  st->tiles_hbm = j.tiles_hbm;
   st->pool_used_bytes = j.pool_words_used * 4;
   API_END
 }

@@ -106,6 +106,7 @@ struct LevelArgs {
   unsigned long long ovf_cap;
   unsigned long long* ovf_used;
   unsigned long long* ticket;   // next tile to hand out, minus the grid size (0 per level)
+  unsigned long long* tiles_walked;  // [0] tiles walked, [1] of them on the HBM path (per run)
   unsigned long long* ev_count;
   unsigned int* err;
   const unsigned long long* win;  // SAIF window (start, end) per instance

@@ -30,6 +30,7 @@ struct JobStats {
   uint64_t level_blocks = 0;    // workgroups per fused level launch
   uint64_t pool_words_used = 0;
   uint64_t pool_words = 0;      // event pool capacity after the run
+  uint64_t tiles = 0, tiles_hbm = 0;  // fused tiles walked, and of them on the HBM path
 };
 
 // Host and device image of one compiled program for one delay time scale:

@@ -88,7 +88,7 @@ static constexpr uint64_t kPoolPad = 16;
 
 // scalar slots in d_scal_
 enum { S_BUMP = 0, S_TOT_A = 1, S_TOT_B = 2, S_TOT_OUT = 3, S_EVC = 4, S_TICKET = 5,
-       S_OVF_USED = 6, S_ERR = 7 };
+       S_OVF_USED = 6, S_ERR = 7, S_TILES = 8, S_TILES_HBM = 9 };
 
 static uint32_t sel_delay(uint32_t r, uint32_t f, int s) {
   return s == 1 ? r : (s == 2 ? f : (s == 3 ? std::max(r, f) : std::min(r, f)));
@@ -630,6 +630,7 @@ void Job::run_fused(std::vector<hipEvent_t>& ev_eval, bool device_driven) {
     a.cap = cap_;
     a.prefetch = prefetch;
     a.ticket = (unsigned long long*)(d_scal_ + S_TICKET);
+    a.tiles_walked = (unsigned long long*)(d_scal_ + S_TILES);
     a.stk = nullptr;
     const unsigned nb = (unsigned)std::min<uint64_t>(ntiles, (uint64_t)std::max(1, level_blocks_));
     if (stack_mode == 1) {
@@ -839,6 +840,8 @@ void Job::finish_stats(const uint64_t* h, double ms) {
   stats_.kernel_ms = ms;
   stats_.n_events = h[S_EVC];
   stats_.pool_words_used = h[S_BUMP];
+  stats_.tiles = h[S_TILES];
+  stats_.tiles_hbm = h[S_TILES_HBM];
   stats_.pool_words = pool_words_;
   // algorithmic bytes: every input waveform read once per reading gate, every
   // output written once, 3 u64 SAIF counters per net, CSR metadata

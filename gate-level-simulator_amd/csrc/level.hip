@@ -625,6 +625,7 @@ __global__ __launch_bounds__(kLanes) void k_level(LevelArgs A) {
   const uint16_t* dt16 = (const uint16_t*)s_dt;
   const int lane = threadIdx.x;
   unsigned long long wave_ev = 0;  // gate events of this wavefront (lane 0)
+  unsigned long long wave_tiles = 0, wave_hbm = 0;
   const uint64_t ntiles = *A.ntiles;  // uniform: one scalar load
   uint64_t tk = blockIdx.x;
   bool fits = false;
@@ -683,11 +684,17 @@ __global__ __launch_bounds__(kLanes) void k_level(LevelArgs A) {
       stage_tile(A, Dn, fits_n, s_in, s_lut, s_dt);
     }
     __syncthreads();
+    ++wave_tiles;
+    wave_hbm += fits ? 0u : 1u;
     tk = tn;
     fits = fits_n;
     dw = dw_n;
   }
   if (lane == 0 && wave_ev) atomicAdd(A.ev_count, wave_ev);
+  if (lane == 0 && wave_tiles) {
+    atomicAdd(A.tiles_walked, wave_tiles);
+    if (wave_hbm) atomicAdd(A.tiles_walked + 1, wave_hbm);
+  }
 }
 
 // Per gate, in tile order: join the tiles' outputs (a tile whose first

@@ -51,7 +51,8 @@ class RunStats(ctypes.Structure):
                 ("n_pins", ctypes.c_uint64), ("kernel_ms", ctypes.c_double),
                 ("wall_ms", ctypes.c_double), ("eval_ms", ctypes.c_double),
                 ("eval_launches", ctypes.c_uint64), ("level_blocks", ctypes.c_uint64),
-                ("pool_bytes", ctypes.c_uint64), ("pool_used_bytes", ctypes.c_uint64)]
+                ("pool_bytes", ctypes.c_uint64), ("pool_used_bytes", ctypes.c_uint64),
+                ("tiles", ctypes.c_uint64), ("tiles_hbm", ctypes.c_uint64)]
 
 
 TUNE_MAX = 16

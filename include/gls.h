@@ -141,6 +141,8 @@ typedef struct {
   uint64_t level_blocks; /* workgroups per fused level launch */
   uint64_t pool_bytes;      /* event pool capacity after the run */
   uint64_t pool_used_bytes; /* pool bytes holding waveforms at the end of the run */
+  uint64_t tiles;           /* fused path: tiles walked (a re-run level counts twice) */
+  uint64_t tiles_hbm;       /* of them too long for LDS (inputs read from HBM) */
 } gls_run_stats;
 int gls_run_stats_get(const gls_run* r, gls_run_stats* st);
 /* SAIF writer (DumpSaif, src/simulate.cu:113-481 / src/simulate.cpp:37-380). */
