@@ -645,6 +645,12 @@ int gls_server_create(gls_ctx* c, const gls_server_opts* o, gls_server** out) {
 
 int gls_server_submit(gls_server* s, const gls_design* d, const gls_stim* st, uint64_t dumpon,
                       uint64_t dumpoff, uint64_t* job) {
+  return gls_server_submit_deadline(s, d, st, dumpon, dumpoff, 0.0, job);
+}
+
+int gls_server_submit_deadline(gls_server* s, const gls_design* d, const gls_stim* st,
+                               uint64_t dumpon, uint64_t dumpoff, double deadline_s,
+                               uint64_t* job) {
   API_BEGIN
   if (!s || !d || !st || !job) throw GlsError(GLS_E_ARG, "null argument");
   if (dumpoff <= dumpon) throw GlsError(GLS_E_ARG, "dumpoff time must be above dumpon time");
@@ -652,7 +658,15 @@ int gls_server_submit(gls_server* s, const gls_design* d, const gls_stim* st, ui
   auto keep = std::make_shared<const Stimulus>(st->st);
   RunInst in = make_inst(const_cast<gls_design*>(d), st, dumpon, dumpoff);
   in.st = keep.get();
-  *job = s->srv->submit(std::move(in), keep);
+  *job = s->srv->submit(std::move(in), keep, deadline_s > 0 ? deadline_s : 0.0);
+  API_END
+}
+
+int gls_server_cancel(gls_server* s, uint64_t job, int* status) {
+  API_BEGIN
+  if (!s) throw GlsError(GLS_E_ARG, "null argument");
+  const int st = s->srv->cancel(job);
+  if (status) *status = st;
   API_END
 }
 
@@ -709,6 +723,7 @@ int gls_server_stats_get(gls_server* s, gls_server_stats* st) {
   st->peak_running = x.peak_running;
   st->resident_programs = x.resident_programs;
   st->peak_bytes = x.peak_bytes;
+  st->cancelled = x.cancelled;
   API_END
 }
 

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <functional>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -120,6 +121,9 @@ class Job {
   uint32_t graph = 0;
   uint32_t stack_mode = 0;  // 0: preemption stacks in LDS, 1: per-workgroup scratch in HBM
   uint32_t pool_grow = 0;   // 1: given pool and overflow-buffer sizes are starting sizes that grow
+  // Asked before each level of a plain execution; true stops the run with
+  // GLS_E_CANCELLED after the stream has drained (job cancel or deadline).
+  std::function<bool()> should_stop;
   bool graph_replayed() const { return last_replay_; }
   static constexpr uint32_t kDefaultSeg = 32;  // input events per time segment
 
@@ -129,6 +133,7 @@ class Job {
   void build_graph();
   void finish_stats(const uint64_t* scal, double ms);
   void run_two_pass(std::vector<hipEvent_t>& ev_eval);
+  void stop_point();
   void* dmalloc(size_t bytes);
   bool grow_pool(uint64_t need, uint64_t used);
   int device_;

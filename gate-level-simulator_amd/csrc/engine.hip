@@ -519,6 +519,10 @@ void Job::prepare(const std::vector<SimInstance>& inst, uint64_t pool_words) {
 // the buffer doubles and that level runs again from its starting state
 // (bump pointer and event count; its outputs and SAIF counters are
 // rewritten).
+void Job::stop_point() {
+  if (should_stop && should_stop()) throw GlsError(GLS_E_CANCELLED, "job cancelled or past its deadline");
+}
+
 void Job::run_fused(std::vector<hipEvent_t>& ev_eval, bool device_driven) {
   uint64_t* bump = d_scal_ + S_BUMP;
   unsigned long long* evc = (unsigned long long*)(d_scal_ + S_EVC);
@@ -549,6 +553,7 @@ void Job::run_fused(std::vector<hipEvent_t>& ev_eval, bool device_driven) {
   size_t next = 0;  // level to run next
   while (next <= nlev) {
     const size_t l = next++;
+    if (!device_driven) stop_point();
     if (l == nlev) {  // after the last level: did it overflow?
       if (!retry || prev == SIZE_MAX) break;
       uint64_t h[8];
@@ -667,6 +672,7 @@ void Job::run_two_pass(std::vector<hipEvent_t>& ev_eval) {
   unsigned long long* evc = (unsigned long long*)(d_scal_ + S_EVC);
   const uint32_t seg = seg_target ? seg_target : 256;
   for (size_t l = 0; l + 1 < level_start_.size(); ++l) {
+    stop_point();
     uint32_t g0 = level_start_[l], ng = level_start_[l + 1] - g0;
     if (ng == 0) continue;
     const GateDev* gates = prog_->d_gdev + g0;

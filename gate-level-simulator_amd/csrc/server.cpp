@@ -59,11 +59,16 @@ uint64_t Server::estimate(const Rec& r, bool* resident) const {
   return bytes;
 }
 
-uint64_t Server::submit(RunInst inst, std::shared_ptr<const Stimulus> keep) {
+uint64_t Server::submit(RunInst inst, std::shared_ptr<const Stimulus> keep, double deadline_s) {
   auto r = std::make_unique<Rec>();
   r->inst = std::move(inst);
   r->keep = std::move(keep);
   r->t_submit = Clock::now();
+  if (deadline_s > 0) {
+    r->has_deadline = true;
+    r->deadline = r->t_submit + std::chrono::duration_cast<Clock::duration>(
+                                    std::chrono::duration<double>(deadline_s));
+  }
   uint64_t id;
   {
     std::lock_guard<std::mutex> lk(mu_);
@@ -92,6 +97,26 @@ void Server::worker(int slot) {
       std::unique_lock<std::mutex> lk(mu_);
       for (;;) {
         if (stop_ && queue_.empty()) break;
+        // queued jobs cancelled or past their deadline leave the queue first
+        const Clock::time_point now = Clock::now();
+        bool wake = false, any_deadline = false;
+        Clock::time_point next_deadline = Clock::time_point::max();
+        for (auto it = queue_.begin(); it != queue_.end();) {
+          Rec* c = jobs_[*it].get();
+          if (expired(*c, now)) {
+            fail_queued(c);
+            it = queue_.erase(it);
+            wake = true;
+            continue;
+          }
+          if (c->has_deadline) {
+            any_deadline = true;
+            next_deadline = std::min(next_deadline, c->deadline);
+          }
+          ++it;
+        }
+        if (wake) cv_done_.notify_all();
+        if (stop_ && queue_.empty()) break;
         // admission: the oldest queued job whose footprint fits what is free;
         // a job may pass the head of the queue only while the head does not fit
         for (auto it = queue_.begin(); it != queue_.end(); ++it) {
@@ -116,7 +141,8 @@ void Server::worker(int slot) {
           }
         }
         if (r) break;
-        cv_work_.wait(lk);
+        if (any_deadline) cv_work_.wait_until(lk, next_deadline);
+        else cv_work_.wait(lk);
       }
       if (!r) break;
       in_use_ += r->reserve;
@@ -150,6 +176,8 @@ void Server::worker(int slot) {
         try {
           Job job(device_, stream);
           job.seg_target = opt_.seg_events;
+          job.should_stop = [this, r] { return expired(*r, Clock::now()); };
+          if (job.should_stop()) throw GlsError(GLS_E_CANCELLED, "job cancelled or past its deadline");
           job.prepare({I}, pool);
           job.run();
           job.fetch_saif(saif);
@@ -192,6 +220,34 @@ void Server::worker(int slot) {
     cv_work_.notify_all();
   }
   (void)hipStreamDestroy(stream);
+}
+
+void Server::fail_queued(Rec* r) {
+  r->info.status = GLS_E_CANCELLED;
+  r->t_end = Clock::now();
+  r->t_start = r->t_end;
+  r->info.queue_ms = ms_between(r->t_submit, r->t_end);
+  r->info.latency_ms = r->info.queue_ms;
+}
+
+int Server::cancel(uint64_t id) {
+  int st;
+  {
+    std::lock_guard<std::mutex> lk(mu_);
+    auto it = jobs_.find(id);
+    if (it == jobs_.end()) throw GlsError(GLS_E_ARG, "unknown job");
+    Rec* r = it->second.get();
+    if (r->info.status <= 0) return r->info.status;  // finished: keeps its result
+    r->stop.store(true, std::memory_order_relaxed);
+    if (r->info.status == 1) {
+      queue_.erase(std::find(queue_.begin(), queue_.end(), id));
+      fail_queued(r);
+    }
+    st = r->info.status;
+  }
+  cv_done_.notify_all();
+  cv_work_.notify_all();
+  return st;
 }
 
 int Server::wait(uint64_t id, double timeout_s) {
@@ -244,6 +300,7 @@ ServerStats Server::stats() {
       last = std::max(last, r.t_end);
     } else if (r.info.status < 0) {
       ++s.failed;
+      if (r.info.status == GLS_E_CANCELLED) ++s.cancelled;
     }
   }
   if (!lat.empty()) {

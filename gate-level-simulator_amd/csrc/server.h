@@ -55,7 +55,7 @@ struct ServerJobInfo {
 };
 
 struct ServerStats {
-  uint64_t submitted = 0, done = 0, failed = 0, events = 0;
+  uint64_t submitted = 0, done = 0, failed = 0, events = 0, cancelled = 0;
   double p50_ms = 0, p99_ms = 0, mean_ms = 0, wall_s = 0, events_per_s = 0;
   uint32_t peak_running = 0, resident_programs = 0;
   uint64_t peak_bytes = 0;
@@ -65,7 +65,14 @@ class Server {
  public:
   Server(int device, const ServerOptions& o, ProgramCache* cache);
   ~Server();  // finishes every submitted job first
-  uint64_t submit(RunInst inst, std::shared_ptr<const Stimulus> keep);
+  // deadline_s > 0: the job fails with GLS_E_CANCELLED when it has not
+  // finished that many seconds after submission (checked while queued and
+  // between levels while running).
+  uint64_t submit(RunInst inst, std::shared_ptr<const Stimulus> keep, double deadline_s = 0);
+  // A queued job is removed at once; a running one stops before its next
+  // level.  Returns the job's status after the call (a finished job keeps
+  // its result).
+  int cancel(uint64_t id);
   // Wait for a job (timeout_s < 0: forever).  Returns its status (1/2 when
   // still queued/running at the timeout).
   int wait(uint64_t id, double timeout_s);
@@ -83,8 +90,14 @@ class Server {
     ServerJobInfo info;
     uint64_t pool_words = 0, reserve = 0;
     std::vector<uint64_t> saif;
-    std::chrono::steady_clock::time_point t_submit, t_start, t_end;
+    std::chrono::steady_clock::time_point t_submit, t_start, t_end, deadline;
+    bool has_deadline = false;
+    std::atomic<bool> stop{false};  // cancel asked for a running job
   };
+  bool expired(const Rec& r, std::chrono::steady_clock::time_point now) const {
+    return r.stop.load(std::memory_order_relaxed) || (r.has_deadline && now >= r.deadline);
+  }
+  void fail_queued(Rec* r);  // caller holds mu_
   void worker(int slot);
   uint64_t estimate(const Rec& r, bool* resident) const;
   int device_;

@@ -87,7 +87,8 @@ class ServerStats(ctypes.Structure):
                 ("p50_ms", ctypes.c_double), ("p99_ms", ctypes.c_double),
                 ("mean_ms", ctypes.c_double), ("wall_s", ctypes.c_double),
                 ("events_per_s", ctypes.c_double), ("peak_running", ctypes.c_uint32),
-                ("resident_programs", ctypes.c_uint32), ("peak_bytes", ctypes.c_uint64)]
+                ("resident_programs", ctypes.c_uint32), ("peak_bytes", ctypes.c_uint64),
+                ("cancelled", ctypes.c_uint64)]
 
 
 # Every symbol include/gls.h declares: (name, restype, argtypes)
@@ -137,6 +138,9 @@ SIGNATURES = [
     ("gls_run_autotune", _I, [_P, ctypes.POINTER(TuneReport)]),
     ("gls_server_create", _I, [_P, ctypes.POINTER(ServerOpts), _PP]),
     ("gls_server_submit", _I, [_P, _P, _P, _U64, _U64, ctypes.POINTER(_U64)]),
+    ("gls_server_submit_deadline", _I, [_P, _P, _P, _U64, _U64, ctypes.c_double,
+                                        ctypes.POINTER(_U64)]),
+    ("gls_server_cancel", _I, [_P, _U64, ctypes.POINTER(ctypes.c_int)]),
     ("gls_server_wait", _I, [_P, _U64, ctypes.c_double, ctypes.POINTER(ctypes.c_int)]),
     ("gls_server_job_info", _I, [_P, _U64, ctypes.POINTER(JobInfo)]),
     ("gls_server_job_saif_text", _I, [_P, _U64, ctypes.c_char_p, ctypes.c_size_t,
@@ -356,12 +360,27 @@ class Server(_Handle):
         self._keep = [ctx]
         super().__init__(h.value)
 
-    def submit(self, design, stim, dumpon, dumpoff):
+    def submit(self, design, stim, dumpon, dumpoff, deadline_s=None):
+        """Queue a job; with deadline_s it fails with GLS_E_CANCELLED when not
+        finished that many seconds after submission."""
         j = _U64()
-        _check(load().gls_server_submit(self.ptr, design.ptr, stim.ptr, dumpon, dumpoff,
-                                        ctypes.byref(j)))
+        if deadline_s is None:
+            _check(load().gls_server_submit(self.ptr, design.ptr, stim.ptr, dumpon, dumpoff,
+                                            ctypes.byref(j)))
+        else:
+            _check(load().gls_server_submit_deadline(self.ptr, design.ptr, stim.ptr, dumpon,
+                                                     dumpoff, float(deadline_s),
+                                                     ctypes.byref(j)))
         self._keep.append(design)
         return j.value
+
+    def cancel(self, job):
+        """Cancel a job; returns its status afterwards (a finished job keeps
+        its result, a queued one is GLS_E_CANCELLED at once, a running one
+        stops before its next level)."""
+        st = ctypes.c_int()
+        _check(load().gls_server_cancel(self.ptr, job, ctypes.byref(st)))
+        return st.value
 
     def wait(self, job, timeout_s=-1.0):
         st = ctypes.c_int()

@@ -216,11 +216,23 @@ typedef struct {
   double p50_ms, p99_ms, mean_ms, wall_s, events_per_s;
   uint32_t peak_running, resident_programs;
   uint64_t peak_bytes;
+  uint64_t cancelled;     /* failed jobs that were cancelled or missed their deadline */
 } gls_server_stats;
 int gls_server_create(gls_ctx* c, const gls_server_opts* o, gls_server** out);
 /* The server copies the stimulus; the design must outlive the server. */
 int gls_server_submit(gls_server* s, const gls_design* d, const gls_stim* st, uint64_t dumpon,
                       uint64_t dumpoff, uint64_t* job);
+/* As gls_server_submit, with a deadline: a job not finished deadline_s
+ * seconds after submission ends with GLS_E_CANCELLED (checked while it is
+ * queued and before each level while it runs; deadline_s <= 0: none). */
+int gls_server_submit_deadline(gls_server* s, const gls_design* d, const gls_stim* st,
+                               uint64_t dumpon, uint64_t dumpoff, double deadline_s,
+                               uint64_t* job);
+/* Cancel a job: a queued job leaves the queue at once, a running job stops
+ * before its next level (its device memory is freed when it has stopped),
+ * a finished job keeps its result.  *status (may be NULL) is the job's
+ * status after the call. */
+int gls_server_cancel(gls_server* s, uint64_t job, int* status);
 /* timeout_s < 0 waits forever; *status as in gls_job_info. */
 int gls_server_wait(gls_server* s, uint64_t job, double timeout_s, int* status);
 int gls_server_job_info(gls_server* s, uint64_t job, gls_job_info* info);

@@ -8,6 +8,7 @@ import json
 import os
 import random
 import sys
+import time
 
 import pytest
 
@@ -146,6 +147,42 @@ def test_server_rejects_jobs_over_budget(ctx):
     srv = gls.Server(ctx, device_budget=1 << 20, max_running=1)
     j = srv.submit(d, s, 0, 200001)
     assert srv.wait(j) == -3  # GLS_E_OOM: the estimate can never fit
+    srv.close()
+
+
+def test_server_cancel_and_deadline(ctx):
+    """A queued job cancelled, a running job cancelled, a deadline already
+    past and one far away, and a finished job cancelled: the cancelled and
+    late jobs end with GLS_E_CANCELLED, the others keep their exact SAIF."""
+    d, s = golden_cases.make("c0_small")
+    on, off = golden_cases.CASES["c0_small"]["window"]
+    ref = ctx.simulate(d, s, on, off).saif_text()
+    # long enough to be caught while running: 50k cells, 40 levels, 5 Mps
+    d_big = gls.Design.generate(1, 0x5EED0001, 50000, 40, 1024)
+    s_big = gls.Stimulus.generate(d_big, 3, 5_000_000, 500.0, 0.01)
+    srv = gls.Server(ctx, max_running=1)
+    big = srv.submit(d_big, s_big, 0, 5_000_001)
+    queued = srv.submit(d, s, on, off)
+    late = srv.submit(d, s, on, off, deadline_s=1e-6)
+    timely = srv.submit(d, s, on, off, deadline_s=600.0)
+    # one worker: these wait behind the big job
+    assert srv.cancel(queued) == -6
+    assert srv.info(queued)["status"] == -6
+    t0 = time.time()
+    while srv.info(big)["status"] == 1 and time.time() - t0 < 60:
+        time.sleep(0.001)
+    assert srv.cancel(big) == 2  # running: stops before its next level
+    assert srv.wait(big) == -6
+    assert srv.wait(late) == -6
+    assert srv.wait(timely) == 0
+    assert srv.saif_text(timely) == ref
+    assert srv.cancel(timely) == 0  # finished: keeps its result
+    assert srv.saif_text(timely) == ref
+    again = srv.submit(d, s, on, off)  # the cancelled job's memory is back
+    assert srv.wait(again) == 0 and srv.saif_text(again) == ref
+    st = srv.stats()
+    say("server with cancels", st)
+    assert st["cancelled"] == 3 and st["failed"] == 3 and st["done"] == 2
     srv.close()
 
 
