@@ -131,17 +131,30 @@ extern "C" {
 const char* gls_last_error(void) { return g_err.c_str(); }
 const char* gls_version(void) { return "gls-mi355x 0.1"; }
 
-int gls_design_from_texts(const char* vlib, const char* gv, const char* sdf, gls_design** out) {
+int gls_design_from_texts_threads(const char* vlib, const char* gv, const char* sdf, int threads,
+                                  gls_design** out) {
   API_BEGIN
   if (!vlib || !gv || !out) throw GlsError(GLS_E_ARG, "null argument");
+  if (threads < 0) throw GlsError(GLS_E_ARG, "negative thread count");
   auto d = std::make_unique<gls_design>();
   d->vlib_text = vlib;
   d->gv_text = gv;
   d->sdf_text = sdf ? sdf : "";
   d->lib = parse_vlib_text(d->vlib_text);
-  d->nl = parse_gv_text(d->gv_text, d->lib);
-  if (sdf) parse_sdf_text(d->sdf_text, d->nl);
+  d->nl = parse_gv_text(d->gv_text, d->lib, threads);
+  if (sdf) parse_sdf_text(d->sdf_text, d->nl, threads);
   *out = d.release();
+  API_END
+}
+
+int gls_design_from_texts(const char* vlib, const char* gv, const char* sdf, gls_design** out) {
+  return gls_design_from_texts_threads(vlib, gv, sdf, 0, out);
+}
+
+int gls_design_digest(const gls_design* d, uint64_t* digest) {
+  API_BEGIN
+  if (!d || !digest) throw GlsError(GLS_E_ARG, "null argument");
+  *digest = netlist_digest(d->nl);
   API_END
 }
 

@@ -1,6 +1,7 @@
 #include "formats.h"
 
 #include <algorithm>
+#include <array>
 #include <exception>
 #include <thread>
 #include <cctype>
@@ -8,6 +9,7 @@
 #include <cstring>
 #include <fstream>
 #include <sstream>
+#include <unordered_map>
 
 #include "../../include/gls_status.h"
 
@@ -179,13 +181,225 @@ static void parse_range(const std::string& tok, int& l, int& r) {
   r = std::stoi(tok.substr(c + 1, b - c - 1));
 }
 
-Netlist parse_gv_text(const std::string& text, const Library& lib) {
+// The text is read in three steps: a serial scan cuts it into statements
+// (cheap: it looks only at each line's first word and last character); the
+// statements are then tokenised in parallel into net names and pins; a
+// serial pass finally numbers the nets in text order and applies every
+// check that depends on earlier statements (undeclared nets).  Net numbers,
+// instance order and the first error reported are those of a one-thread
+// read.
+namespace {
+
+struct GvStmt {
+  uint8_t kind = 0;                // 0 nothing, 1 declaration, 2 assign, 3 instance
+  bool input = false;              // declaration: `input`
+  std::vector<std::string> names;  // declared nets; assign: left, right; instance: net per pin bit
+  std::vector<std::string> pin_bits;
+  std::vector<uint8_t> pin_in;     // instance: 1 input pin, 0 output pin
+  Instance inst;                   // instance: name, type, pins, unconnected outputs
+  size_t n_ok = 0;                 // instance: pin bits read before `err`
+  // instance pin bits whose net was not numbered before this round (new
+  // nets, constants): (pin bit, 1 input / 0 output, index in in_pins/out_pins)
+  std::vector<std::array<uint32_t, 3>> todo;
+  std::exception_ptr err;
+};
+
+inline bool gv_delim(char c) { return c == ' ' || c == '\t' || c == '\r' || c == '\n'; }
+
+// First word of a line, lower-cased, compared with a keyword.
+bool first_word_is(const std::string& text, size_t a, size_t b, const char* kw) {
+  while (a < b && gv_delim(text[a])) ++a;
+  size_t e = a;
+  while (e < b && !gv_delim(text[e])) ++e;
+  const size_t n = std::strlen(kw);
+  if (e - a != n) return false;
+  for (size_t k = 0; k < n; ++k)
+    if ((char)std::tolower((unsigned char)text[a + k]) != kw[k]) return false;
+  return true;
+}
+
+// Pins of an instance, numbered where the net already has a number (nets
+// are never renumbered, so a number found now is the one a one-thread read
+// finds); the others are left to the serial pass.
+void place_pins(const Netlist& nl, GvStmt& R) {
+  for (size_t b = 0; b < R.n_ok; ++b) {
+    const std::string& nn = R.names[b];
+    const int id = (nn == "1'b1" || nn == "1'b0") ? -1 : nl.net(nn);
+    auto& pins = R.pin_in[b] ? R.inst.in_pins : R.inst.out_pins;
+    if (id < 0) R.todo.push_back({(uint32_t)b, (uint32_t)R.pin_in[b], (uint32_t)pins.size()});
+    pins.push_back({std::move(R.pin_bits[b]), id});
+  }
+}
+
+void parse_gv_stmt(const std::string& stmt, const Library& lib, GvStmt& R) {
+  auto t = split(stmt, " \t\r\n");
+  if (t.empty()) return;
+  std::string k0 = lower(t[0]);
+  if (k0 == "input" || k0 == "output" || k0 == "wire" || k0 == "inout") {
+    R.kind = 1;
+    R.input = k0 == "input";
+    if (t.size() < 2) throw GlsError(GLS_E_FORMAT, "gv: empty declaration");
+    if (t[1].find('[') != std::string::npos) {
+      if (t.size() < 3) throw GlsError(GLS_E_FORMAT, "gv: bus declaration without a name");
+      int l, r;
+      parse_range(t[1], l, r);
+      // a bus declaration may list several names: input [3:0] a, b;
+      for (size_t ni = 2; ni < t.size(); ++ni) {
+        std::string nm = strip_tail(t[ni]);
+        if (nm.empty()) continue;
+        int step = l > r ? -1 : 1;
+        for (int i = l;; i += step) {
+          R.names.push_back(nm + "[" + std::to_string(i) + "]");
+          if (i == r) break;
+        }
+      }
+    } else {
+      for (size_t i = 1; i < t.size(); ++i) {
+        std::string nm = strip_tail(t[i]);
+        if (!nm.empty()) R.names.push_back(std::move(nm));
+      }
+    }
+    return;
+  }
+  if (k0 == "assign") {
+    // assign left = right;  (any spacing)
+    R.kind = 2;
+    std::string body;
+    for (size_t i = 1; i < t.size(); ++i) body += t[i];
+    if (!body.empty() && body.back() == ';') body.pop_back();
+    size_t eq = body.find('=');
+    if (eq == std::string::npos) throw GlsError(GLS_E_FORMAT, "gv: assign without '='");
+    R.names.push_back(body.substr(0, eq));
+    R.names.push_back(body.substr(eq + 1));
+    return;
+  }
+  // instance statement
+  int ty = lib.find(t[0]);
+  if (ty < 0) return;  // the reference skips unknown modules with a warning
+  R.kind = 3;
+  const CellType& ct = lib.types[ty];
+  std::string body;
+  for (size_t i = 1; i < t.size(); ++i) body += t[i];
+  size_t lp = body.find('(');
+  if (lp == std::string::npos) throw GlsError(GLS_E_FORMAT, "gv: instance without connections");
+  Instance& inst = R.inst;
+  inst.name = body.substr(0, lp);
+  inst.type = ty;
+  std::vector<std::string> used_ports;
+  // connections: .pin(net) or .pin({a,b[3:0],1'b0})
+  size_t p = lp + 1;
+  while ((p = body.find('.', p)) != std::string::npos) {
+    size_t po = body.find('(', p);
+    if (po == std::string::npos) throw GlsError(GLS_E_FORMAT, "gv: bad connection in " + inst.name);
+    std::string pin = body.substr(p + 1, po - p - 1);
+    int depth = 1;
+    size_t q = po + 1;
+    while (q < body.size() && depth > 0) {
+      if (body[q] == '(') depth++;
+      else if (body[q] == ')') depth--;
+      if (depth > 0) ++q;
+    }
+    std::string conn = body.substr(po + 1, q - po - 1);
+    p = q;
+    auto it = std::find(ct.ports.begin(), ct.ports.end(), pin);
+    if (it == ct.ports.end()) throw GlsError(GLS_E_FORMAT, "gv: " + ct.name + " has no pin " + pin);
+    bool is_in = std::find(ct.inputs.begin(), ct.inputs.end(), pin) != ct.inputs.end();
+    bool is_out = std::find(ct.outputs.begin(), ct.outputs.end(), pin) != ct.outputs.end();
+    if (!is_in && !is_out) throw GlsError(GLS_E_FORMAT, "gv: pin " + pin + " must be input or output");
+    used_ports.push_back(pin);
+    std::vector<std::string> nets;
+    if (!conn.empty() && conn[0] == '{') {
+      std::string inner = conn.substr(1, conn.size() - 2);
+      for (auto& part : split(inner, ",")) {
+        if (part.find('[') != std::string::npos && part.find(':') != std::string::npos) {
+          int l, r;
+          parse_range(part, l, r);
+          std::string base = part.substr(0, part.find('['));
+          int step = l > r ? -1 : 1;
+          for (int i = l;; i += step) {
+            nets.push_back(base + "[" + std::to_string(i) + "]");
+            if (i == r) break;
+          }
+        } else {
+          nets.push_back(part);
+        }
+      }
+    } else {
+      nets.push_back(conn);
+    }
+    for (size_t bi = 0; bi < nets.size(); ++bi) {
+      R.pin_bits.push_back(nets.size() > 1 ? pin + "[" + std::to_string(bi) + "]" : pin);
+      R.names.push_back(std::move(nets[bi]));
+      R.pin_in.push_back(is_in ? 1 : 0);
+      R.n_ok = R.names.size();
+    }
+  }
+  for (auto& op : ct.outputs)
+    if (std::find(used_ports.begin(), used_ports.end(), op) == used_ports.end())
+      inst.unknown_out.push_back(op);
+}
+
+}  // namespace
+
+Netlist parse_gv_text(const std::string& text, const Library& lib, int threads) {
   Netlist nl;
-  auto lines = lines_of(text);
-  bool in_module = false;
-  std::string stmt;
+  // 1. statements: spans of trimmed lines; a statement is " " + line for
+  // each of its lines, ending with the line that ends in ';'
+  std::vector<std::pair<size_t, size_t>> spans;  // trimmed lines of all statements
+  std::vector<size_t> first;                     // statement k: spans[first[k], first[k+1])
+  {
+    bool in_module = false, header = false;
+    size_t open = 0;  // spans of the statement being assembled
+    const size_t n = text.size();
+    size_t p = 0;
+    while (p < n) {
+      size_t q = text.find('\n', p);
+      if (q == std::string::npos) q = n;
+      size_t a = p, b = q;
+      p = q + 1;
+      while (a < b && text[a] == ' ') ++a;
+      while (b > a && text[b - 1] == ' ') --b;
+      if (a == b) continue;
+      size_t w = a;
+      while (w < b && gv_delim(text[w])) ++w;
+      if (w == b) continue;  // no word
+      const bool semi = text[b - 1] == ';';
+      if (first_word_is(text, a, b, "module")) {
+        in_module = true;
+        auto tmp = split(text.substr(a, b - a), " \t\r\n");
+        if (tmp.size() > 1) {
+          const std::string& nm = tmp[1];
+          size_t k = nm.find('(');
+          nl.top = k == std::string::npos ? nm : nm.substr(0, k);
+        }
+        // the header's port list may span lines; it declares nothing
+        if (!semi) {
+          header = true;
+          spans.resize(spans.size() - open);
+          open = 0;
+        }
+        continue;
+      }
+      if (first_word_is(text, a, b, "endmodule")) break;
+      if (!in_module) continue;
+      if (header) {
+        if (semi) header = false;
+        continue;
+      }
+      spans.push_back({a, b - a});
+      ++open;
+      if (!semi) continue;
+      first.push_back(spans.size() - open);
+      open = 0;
+    }
+    spans.resize(spans.size() - open);  // an unterminated statement is dropped
+    first.push_back(spans.size());
+  }
+  const size_t n_stmt = first.size() - 1;
+  // 2 + 3, in rounds that bound the memory held by tokenised statements
+  if (threads <= 0) threads = (int)std::min<unsigned>(16, std::max(1u, std::thread::hardware_concurrency()));
+  if (text.size() < (8u << 20)) threads = 1;
   int hidden_const = 0;
-  auto declared = [&](const std::string& n) { return nl.net(n) >= 0; };
   auto const_net = [&](const std::string& v) {
     std::string nm = "gv_net_const" + std::to_string(hidden_const++);
     int id = nl.add_net(nm);
@@ -193,202 +407,141 @@ Netlist parse_gv_text(const std::string& text, const Library& lib) {
     nl.const_hidden[id] = 1;
     return id;
   };
-  for (auto& raw : lines) {
-    std::string line = trim_spaces(raw);
-    if (line.empty()) continue;
-    auto tmp = split(line, " \t\r\n");
-    if (tmp.empty()) continue;
-    std::string kw = lower(tmp[0]);
-    if (kw == "module") {
-      in_module = true;
-      if (tmp.size() > 1) {
-        std::string nm = tmp[1];
-        size_t p = nm.find('(');
-        nl.top = p == std::string::npos ? nm : nm.substr(0, p);
-      }
-      // the header's port list may span lines; it declares nothing
-      if (line.back() != ';') stmt = "\x01";
-      continue;
-    }
-    if (kw == "endmodule") break;
-    if (!in_module) continue;
-    if (stmt == "\x01") {  // inside a multi-line module header
-      if (line.back() == ';') stmt.clear();
-      continue;
-    }
-    stmt += " " + line;
-    if (line.back() != ';') continue;
-    auto t = split(stmt, " \t\r\n");
-    stmt.clear();
-    if (t.empty()) continue;
-    std::string k0 = lower(t[0]);
-    if (k0 == "input" || k0 == "output" || k0 == "wire" || k0 == "inout") {
-      if (t.size() < 2) throw GlsError(GLS_E_FORMAT, "gv: empty declaration");
-      if (t[1].find('[') != std::string::npos) {
-        if (t.size() < 3) throw GlsError(GLS_E_FORMAT, "gv: bus declaration without a name");
-        int l, r;
-        parse_range(t[1], l, r);
-        // a bus declaration may list several names: input [3:0] a, b;
-        for (size_t ni = 2; ni < t.size(); ++ni) {
-          std::string nm = strip_tail(t[ni]);
-          if (nm.empty()) continue;
-          int step = l > r ? -1 : 1;
-          for (int i = l;; i += step) {
-            int id = nl.add_net(nm + "[" + std::to_string(i) + "]");
-            if (k0 == "input") nl.net_is_input[id] = 1;
-            if (i == r) break;
-          }
+  const size_t round = 1u << 17;
+  nl.net_ids.reserve(text.size() / 64 + 16);  // about one net per 64 bytes of text
+  nl.insts.reserve(n_stmt);
+  std::vector<GvStmt> R;
+  for (size_t r0 = 0; r0 < n_stmt; r0 += round) {
+    const size_t nr = std::min(round, n_stmt - r0);
+    R.clear();
+    R.resize(nr);
+    auto work = [&](size_t lo, size_t hi) {
+      std::string stmt;
+      for (size_t k = lo; k < hi; ++k) {
+        stmt.clear();
+        for (size_t x = first[r0 + k]; x < first[r0 + k + 1]; ++x) {
+          stmt += ' ';
+          stmt.append(text, spans[x].first, spans[x].second);
         }
-      } else {
-        for (size_t i = 1; i < t.size(); ++i) {
-          std::string nm = strip_tail(t[i]);
-          if (nm.empty()) continue;
+        try {
+          parse_gv_stmt(stmt, lib, R[k]);
+        } catch (...) {
+          R[k].err = std::current_exception();
+        }
+        if (R[k].kind == 3) place_pins(nl, R[k]);
+      }
+    };
+    const size_t nt = std::min<size_t>((size_t)threads, (nr + 1023) / 1024);
+    if (nt <= 1) {
+      work(0, nr);
+    } else {
+      std::vector<std::thread> th;
+      for (size_t w = 0; w < nt; ++w) th.emplace_back(work, nr * w / nt, nr * (w + 1) / nt);
+      for (auto& t : th) t.join();
+    }
+    for (size_t k = 0; k < nr; ++k) {
+      GvStmt& S = R[k];
+      if (S.kind == 1 && !S.err) {
+        for (auto& nm : S.names) {
           int id = nl.add_net(nm);
-          if (k0 == "input") nl.net_is_input[id] = 1;
+          if (S.input) nl.net_is_input[id] = 1;
         }
-      }
-      continue;
-    }
-    if (k0 == "assign") {
-      // assign left = right;  (any spacing)
-      std::string body;
-      for (size_t i = 1; i < t.size(); ++i) body += t[i];
-      if (!body.empty() && body.back() == ';') body.pop_back();
-      size_t eq = body.find('=');
-      if (eq == std::string::npos) throw GlsError(GLS_E_FORMAT, "gv: assign without '='");
-      std::string left = body.substr(0, eq), right = body.substr(eq + 1);
-      int lid = nl.add_net(left);
-      if (right == "1'b1" || right == "1'b0") {
-        nl.const_val[lid] = right == "1'b1" ? V1 : V0;
-      } else {
-        int rid = nl.add_net(right);
-        nl.alias_of[lid] = rid;
-      }
-      continue;
-    }
-    // instance statement
-    int ty = lib.find(t[0]);
-    if (ty < 0) continue;  // the reference skips unknown modules with a warning
-    const CellType& ct = lib.types[ty];
-    std::string body;
-    for (size_t i = 1; i < t.size(); ++i) body += t[i];
-    size_t lp = body.find('(');
-    if (lp == std::string::npos) throw GlsError(GLS_E_FORMAT, "gv: instance without connections");
-    Instance inst;
-    inst.name = body.substr(0, lp);
-    inst.type = ty;
-    std::vector<std::string> used_ports;
-    // connections: .pin(net) or .pin({a,b[3:0],1'b0})
-    size_t p = lp + 1;
-    while ((p = body.find('.', p)) != std::string::npos) {
-      size_t po = body.find('(', p);
-      if (po == std::string::npos) throw GlsError(GLS_E_FORMAT, "gv: bad connection in " + inst.name);
-      std::string pin = body.substr(p + 1, po - p - 1);
-      int depth = 1;
-      size_t q = po + 1;
-      while (q < body.size() && depth > 0) {
-        if (body[q] == '(') depth++;
-        else if (body[q] == ')') depth--;
-        if (depth > 0) ++q;
-      }
-      std::string conn = body.substr(po + 1, q - po - 1);
-      p = q;
-      auto it = std::find(ct.ports.begin(), ct.ports.end(), pin);
-      if (it == ct.ports.end()) throw GlsError(GLS_E_FORMAT, "gv: " + ct.name + " has no pin " + pin);
-      bool is_in = std::find(ct.inputs.begin(), ct.inputs.end(), pin) != ct.inputs.end();
-      bool is_out = std::find(ct.outputs.begin(), ct.outputs.end(), pin) != ct.outputs.end();
-      if (!is_in && !is_out) throw GlsError(GLS_E_FORMAT, "gv: pin " + pin + " must be input or output");
-      used_ports.push_back(pin);
-      std::vector<std::string> nets;
-      if (!conn.empty() && conn[0] == '{') {
-        std::string inner = conn.substr(1, conn.size() - 2);
-        for (auto& part : split(inner, ",")) {
-          if (part.find('[') != std::string::npos && part.find(':') != std::string::npos) {
-            int l, r;
-            parse_range(part, l, r);
-            std::string base = part.substr(0, part.find('['));
-            int step = l > r ? -1 : 1;
-            for (int i = l;; i += step) {
-              nets.push_back(base + "[" + std::to_string(i) + "]");
-              if (i == r) break;
-            }
+      } else if (S.kind == 2 && !S.err) {
+        int lid = nl.add_net(S.names[0]);
+        const std::string& right = S.names[1];
+        if (right == "1'b1" || right == "1'b0") {
+          nl.const_val[lid] = right == "1'b1" ? V1 : V0;
+        } else {
+          int rid = nl.add_net(right);
+          nl.alias_of[lid] = rid;
+        }
+      } else if (S.kind == 3) {
+        for (auto& td : S.todo) {
+          const std::string& nn = S.names[td[0]];
+          int id;
+          if (nn == "1'b1" || nn == "1'b0") {
+            id = const_net(nn);
           } else {
-            nets.push_back(part);
+            id = nl.net(nn);
+            if (id < 0) {
+              if (nn.find('[') == std::string::npos)
+                throw GlsError(GLS_E_FORMAT, "gv: undeclared net " + nn + " on " + S.inst.name);
+              id = nl.add_net(nn);
+            }
           }
+          (td[1] ? S.inst.in_pins : S.inst.out_pins)[td[2]].second = id;
         }
-      } else {
-        nets.push_back(conn);
+        if (!S.err) nl.insts.push_back(std::move(S.inst));
       }
-      for (size_t bi = 0; bi < nets.size(); ++bi) {
-        const std::string& nn = nets[bi];
-        std::string pin_bit = nets.size() > 1 ? pin + "[" + std::to_string(bi) + "]" : pin;
-        int id;
-        if (nn == "1'b1" || nn == "1'b0") id = const_net(nn);
-        else {
-          if (!declared(nn) && nn.find('[') == std::string::npos)
-            throw GlsError(GLS_E_FORMAT, "gv: undeclared net " + nn + " on " + inst.name);
-          id = nl.add_net(nn);
-        }
-        if (is_in) inst.in_pins.push_back({pin_bit, id});
-        else inst.out_pins.push_back({pin_bit, id});
-      }
+      if (S.err) std::rethrow_exception(S.err);
     }
-    for (auto& op : ct.outputs)
-      if (std::find(used_ports.begin(), used_ports.end(), op) == used_ports.end())
-        inst.unknown_out.push_back(op);
-    nl.insts.push_back(std::move(inst));
   }
   return nl;
 }
 
-Netlist parse_gv_file(const std::string& path, const Library& lib) {
-  return parse_gv_text(read_file(path), lib);
+Netlist parse_gv_file(const std::string& path, const Library& lib, int threads) {
+  return parse_gv_text(read_file(path), lib, threads);
 }
 
 // ----------------------------------------------------------------- SDF ----
 // src/PreProcess.cpp:1276-1464: TIMESCALE factor (default 1ns = 1e3),
 // INSTANCE blocks closed by the first line that trims to ")", typ value of
 // each triple times the factor.
-void parse_sdf_text(const std::string& text, Netlist& nl) {
-  std::unordered_map<std::string, int> by_name;
-  for (size_t i = 0; i < nl.insts.size(); ++i) by_name[nl.insts[i].name] = (int)i;
-  auto lines = lines_of(text);
-  double factor = 1e3;
+//
+// A large file is read in parallel stretches.  A stretch starts right after
+// a line that trims to ")": such a line ends any open INSTANCE block, so
+// every stretch starts outside a block, as the one-thread read would be
+// there.  What a stretch cannot know is the TIMESCALE factor in force at its
+// start; its delays are kept unscaled up to its own first TIMESCALE line and
+// multiplied by the inherited factor when the stretches are joined in order
+// (the same single multiplication the one-thread read makes).
+namespace {
+
+struct SdfPart {
+  std::vector<std::pair<int, IoPath>> paths;  // (instance, path) in text order
+  std::vector<uint8_t> scaled;                // path's values already carry a factor
+  bool has_factor = false;                    // a TIMESCALE line of this stretch set one
+  double factor = 1e3;                        // factor in force at the end, when has_factor
+  std::exception_ptr err;
+};
+
+// Iterate the lines of text[a, b) as lines_of() cuts them.
+template <class F>
+void for_lines(const std::string& text, size_t a, size_t b, F&& f) {
+  size_t p = a;
+  while (p < b) {
+    size_t q = text.find('\n', p);
+    if (q == std::string::npos || q > b) q = b;
+    if (!f(text.substr(p, q - p))) return;
+    p = q + 1;
+  }
+}
+
+void parse_sdf_part(const std::string& text, size_t a, size_t b, const Netlist& nl,
+                    const std::unordered_map<std::string, int>& by_name, SdfPart& P) {
   const char* D = " \t\r\n\"'()";
-  auto typ = [&](const std::string& tok) -> double {
+  bool in_block = false;  // inside the INSTANCE block of a known instance
+  int cur = -1;
+  auto typ = [&](const std::string& tok, uint8_t& sc) -> double {
     auto v = split(tok, ":");
     const std::string& s = v.size() == 3 ? v[1] : (v.empty() ? tok : v[0]);
     char* end = nullptr;
     double x = std::strtod(s.c_str(), &end);
     if (end == s.c_str()) throw GlsError(GLS_E_FORMAT, "sdf: bad delay value " + tok);
-    return x * factor;
+    sc = P.has_factor ? 1 : 0;
+    return P.has_factor ? x * P.factor : x;
   };
-  for (size_t li = 0; li < lines.size(); ++li) {
-    std::string line = trim_spaces(lines[li]);
-    if (line.empty()) continue;
-    auto tmp = split(line, D);
-    if (tmp.empty()) continue;
-    std::string k0 = upper(tmp[0]);
-    if (k0 == "TIMESCALE" && tmp.size() > 1) {
-      std::string ts = tmp[1];
-      if (ts == "1ns") factor = 1e3;
-      else if (ts == "1ps") factor = 1.0;
-      else if (ts == "1us") factor = 1e6;
-      else if (ts == "1ms") factor = 1e9;
-      else if (ts == "1s") factor = 1e12;
-      // other spellings keep the current factor, as the reference does
-      continue;
-    }
-    if (k0 != "INSTANCE" || tmp.size() < 2) continue;
-    auto it = by_name.find(tmp[1]);
-    if (it == by_name.end()) continue;
-    Instance& inst = nl.insts[it->second];
-    for (++li; li < lines.size(); ++li) {
-      std::string l2 = trim_spaces(lines[li]);
-      if (l2.empty()) continue;
-      if (l2 == ")") break;
-      auto t = split(l2, D);
-      if (t.empty() || upper(t[0]) != "IOPATH") continue;
+  for_lines(text, a, b, [&](const std::string& raw) {
+    std::string line = trim_spaces(raw);
+    if (line.empty()) return true;
+    if (in_block) {
+      if (line == ")") {
+        in_block = false;
+        return true;
+      }
+      auto t = split(line, D);
+      if (t.empty() || upper(t[0]) != "IOPATH") return true;
+      const Instance& inst = nl.insts[cur];
       IoPath d;
       size_t ip, op, vp;
       std::string e = upper(t.size() > 1 ? t[1] : "");
@@ -413,14 +566,166 @@ void parse_sdf_text(const std::string& text, Netlist& nl) {
       }
       if (d.out_net == -1000000)
         throw GlsError(GLS_E_FORMAT, "sdf: instance " + inst.name + " has no output pin " + t[op]);
-      d.rise = typ(t[vp]);
-      d.fall = t.size() > vp + 1 ? typ(t[vp + 1]) : d.rise;
-      inst.delays.push_back(d);
+      uint8_t sc = 0;
+      d.rise = typ(t[vp], sc);
+      d.fall = t.size() > vp + 1 ? typ(t[vp + 1], sc) : d.rise;
+      P.paths.push_back({cur, d});
+      P.scaled.push_back(sc);
+      return true;
     }
+    auto tmp = split(line, D);
+    if (tmp.empty()) return true;
+    std::string k0 = upper(tmp[0]);
+    if (k0 == "TIMESCALE" && tmp.size() > 1) {
+      const std::string& ts = tmp[1];
+      double f = 0;
+      if (ts == "1ns") f = 1e3;
+      else if (ts == "1ps") f = 1.0;
+      else if (ts == "1us") f = 1e6;
+      else if (ts == "1ms") f = 1e9;
+      else if (ts == "1s") f = 1e12;
+      // other spellings keep the current factor, as the reference does
+      if (f != 0) {
+        P.factor = f;
+        P.has_factor = true;
+      }
+      return true;
+    }
+    if (k0 != "INSTANCE" || tmp.size() < 2) return true;
+    auto it = by_name.find(tmp[1]);
+    if (it == by_name.end()) return true;
+    cur = it->second;
+    in_block = true;
+    return true;
+  });
+}
+
+}  // namespace
+
+void parse_sdf_text(const std::string& text, Netlist& nl, int threads) {
+  std::unordered_map<std::string, int> by_name;
+  by_name.reserve(nl.insts.size());
+  for (size_t i = 0; i < nl.insts.size(); ++i) by_name[nl.insts[i].name] = (int)i;
+  const size_t n = text.size();
+  if (threads <= 0) threads = (int)std::min<unsigned>(16, std::max(1u, std::thread::hardware_concurrency()));
+  if (n < (8u << 20)) threads = 1;
+  // stretch starts: the line after a line that trims to ")"
+  std::vector<size_t> cut{0};
+  for (int i = 1; i < threads; ++i) {
+    // first line start at or after the target
+    size_t p = std::max(n / (size_t)threads * (size_t)i, cut.back());
+    if (p > 0 && text[p - 1] != '\n') {
+      p = text.find('\n', p);
+      if (p == std::string::npos) break;
+      ++p;
+    }
+    size_t found = std::string::npos;
+    while (p < n) {
+      size_t q = text.find('\n', p);
+      if (q == std::string::npos) break;
+      if (trim_spaces(text.substr(p, q - p)) == ")") {
+        found = q + 1;
+        break;
+      }
+      p = q + 1;
+    }
+    if (found == std::string::npos || found >= n) break;
+    if (found > cut.back()) cut.push_back(found);
+  }
+  cut.push_back(n);
+  const size_t np = cut.size() - 1;
+  std::vector<SdfPart> parts(np);
+  auto run_part = [&](size_t i) {
+    try {
+      parse_sdf_part(text, cut[i], cut[i + 1], nl, by_name, parts[i]);
+    } catch (...) {
+      parts[i].err = std::current_exception();
+    }
+  };
+  if (np == 1) {
+    run_part(0);
+  } else {
+    std::vector<std::thread> th;
+    for (size_t i = 0; i < np; ++i) th.emplace_back(run_part, i);
+    for (auto& t : th) t.join();
+  }
+  // the first error in text order is the one a one-thread read reports
+  for (auto& P : parts)
+    if (P.err) std::rethrow_exception(P.err);
+  double factor = 1e3;
+  for (auto& P : parts) {
+    for (size_t k = 0; k < P.paths.size(); ++k) {
+      IoPath& d = P.paths[k].second;
+      if (!P.scaled[k]) {
+        d.rise = d.rise * factor;
+        d.fall = d.fall * factor;
+      }
+      nl.insts[P.paths[k].first].delays.push_back(d);
+    }
+    if (P.has_factor) factor = P.factor;
   }
 }
 
-void parse_sdf_file(const std::string& path, Netlist& nl) { parse_sdf_text(read_file(path), nl); }
+void parse_sdf_file(const std::string& path, Netlist& nl, int threads) {
+  parse_sdf_text(read_file(path), nl, threads);
+}
+
+// -------------------------------------------------------------- digest ----
+namespace {
+struct Fnv {
+  uint64_t h = 1469598103934665603ull;
+  void bytes(const void* p, size_t n) {
+    const unsigned char* b = (const unsigned char*)p;
+    for (size_t i = 0; i < n; ++i) h = (h ^ b[i]) * 1099511628211ull;
+  }
+  void u64(uint64_t x) { bytes(&x, 8); }
+  void str(const std::string& s) {
+    u64(s.size());
+    bytes(s.data(), s.size());
+  }
+  void f64(double x) {
+    uint64_t b;
+    std::memcpy(&b, &x, 8);
+    u64(b);
+  }
+};
+}  // namespace
+
+uint64_t netlist_digest(const Netlist& nl) {
+  Fnv f;
+  f.str(nl.top);
+  f.u64(nl.net_names.size());
+  for (size_t i = 0; i < nl.net_names.size(); ++i) {
+    f.str(nl.net_names[i]);
+    f.u64(nl.net_is_input[i]);
+    f.u64((uint64_t)(int64_t)nl.const_val[i]);
+    f.u64(nl.const_hidden[i]);
+    f.u64((uint64_t)(int64_t)nl.alias_of[i]);
+  }
+  f.u64(nl.insts.size());
+  for (const Instance& in : nl.insts) {
+    f.str(in.name);
+    f.u64((uint64_t)(int64_t)in.type);
+    for (auto* pins : {&in.in_pins, &in.out_pins}) {
+      f.u64(pins->size());
+      for (auto& p : *pins) {
+        f.str(p.first);
+        f.u64((uint64_t)(int64_t)p.second);
+      }
+    }
+    f.u64(in.unknown_out.size());
+    for (auto& u : in.unknown_out) f.str(u);
+    f.u64(in.delays.size());
+    for (const IoPath& d : in.delays) {
+      f.u64((uint64_t)(int64_t)d.edge);
+      f.u64((uint64_t)(int64_t)d.in_net);
+      f.u64((uint64_t)(int64_t)d.out_net);
+      f.f64(d.rise);
+      f.f64(d.fall);
+    }
+  }
+  return f.h;
+}
 
 // ----------------------------------------------------------------- VCD ----
 static int vcd_code(char c) {

@@ -14,15 +14,21 @@ namespace gls {
 // throw GlsError(GLS_E_FORMAT) where the reference would assert or exit.
 Library parse_vlib_text(const std::string& text);
 Library parse_vlib_file(const std::string& path);
-Netlist parse_gv_file(const std::string& path, const Library& lib);
-Netlist parse_gv_text(const std::string& text, const Library& lib);
-void parse_sdf_file(const std::string& path, Netlist& nl);
-void parse_sdf_text(const std::string& text, Netlist& nl);
+// threads as for the VCD reader below; the result does not depend on it.
+Netlist parse_gv_file(const std::string& path, const Library& lib, int threads = 0);
+Netlist parse_gv_text(const std::string& text, const Library& lib, int threads = 0);
+void parse_sdf_file(const std::string& path, Netlist& nl, int threads = 0);
+void parse_sdf_text(const std::string& text, Netlist& nl, int threads = 0);
 // threads: 0 = up to 16 (the body is read in parallel stretches when it is
 // large), 1 = one thread; the result does not depend on it.
 Stimulus parse_vcd_file(const std::string& path, int threads = 0);
 Stimulus parse_vcd_text(const std::string& text);
 Stimulus parse_vcd_text(const std::string& text, int threads);
+
+// 64-bit FNV-1a digest of everything a parse produces (nets with their
+// flags, instances with pins, unconnected outputs and delays bit for bit);
+// two netlists with the same digest simulate identically.
+uint64_t netlist_digest(const Netlist& nl);
 
 // Writers (used by the generator and tests).
 std::string gv_text(const Netlist& nl, const Library& lib);

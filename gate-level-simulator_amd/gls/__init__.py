@@ -102,6 +102,8 @@ SIGNATURES = [
     ("gls_version", _S, []),
     ("gls_design_from_files", _I, [_S, _S, _S, _PP]),
     ("gls_design_from_texts", _I, [_S, _S, _S, _PP]),
+    ("gls_design_from_texts_threads", _I, [_S, _S, _S, ctypes.c_int, _PP]),
+    ("gls_design_digest", _I, [_P, ctypes.POINTER(_U64)]),
     ("gls_design_generate", _I, [_I, _U64, _I, _I, _I, _PP]),
     ("gls_design_save", _I, [_P, _S]),
     ("gls_design_load", _I, [_S, _PP]),
@@ -220,10 +222,22 @@ class Design(_Handle):
         return cls(h.value)
 
     @classmethod
-    def from_texts(cls, vlib, gv, sdf):
+    def from_texts(cls, vlib, gv, sdf, threads=None):
+        """threads: None = the library's choice (parallel for large texts),
+        1 = one thread, n = up to n threads; the design does not depend on it."""
         h = ctypes.c_void_p()
-        _check(load().gls_design_from_texts(_b(vlib), _b(gv), _b(sdf), ctypes.byref(h)))
+        if threads is None:
+            _check(load().gls_design_from_texts(_b(vlib), _b(gv), _b(sdf), ctypes.byref(h)))
+        else:
+            _check(load().gls_design_from_texts_threads(_b(vlib), _b(gv), _b(sdf), int(threads),
+                                                        ctypes.byref(h)))
         return cls(h.value)
+
+    def digest(self):
+        """64-bit digest of the parsed netlist (nets, pins, delays bit for bit)."""
+        v = _U64()
+        _check(load().gls_design_digest(self.ptr, ctypes.byref(v)))
+        return v.value
 
     @classmethod
     def generate(cls, config, seed=0, n_cells=0, n_levels=0, n_pis=0):

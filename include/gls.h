@@ -42,6 +42,14 @@ int gls_design_from_files(const char* vlib_path, const char* gv_path, const char
 /* Same, from in-memory texts. */
 int gls_design_from_texts(const char* vlib_text, const char* gv_text, const char* sdf_text,
                           gls_design** out);
+/* Same with an explicit thread count for the gv and SDF readers (0: up to
+ * 16; 1: one thread; texts under 8 MiB are read on one thread).  The design
+ * does not depend on it. */
+int gls_design_from_texts_threads(const char* vlib_text, const char* gv_text,
+                                  const char* sdf_text, int threads, gls_design** out);
+/* 64-bit digest of the parsed design: nets and their flags, instances with
+ * their pins and every delay bit for bit.  Equal digests simulate alike. */
+int gls_design_digest(const gls_design* d, uint64_t* digest);
 /* Seeded synthetic netlist of one benchmark configuration (0, 1, 2) with the
  * generic cell library; n_cells/n_levels/n_pis override when > 0. */
 int gls_design_generate(int config, uint64_t seed, int n_cells, int n_levels, int n_pis,
