@@ -266,6 +266,7 @@ int gls_design_info_get(const gls_design* d, gls_design_info* info) {
   info->n_gates = pg.gates.size();
   info->n_levels = (uint64_t)pg.n_levels();
   info->n_pins = pg.n_pins;
+  info->program_digest = program_digest(pg);
   API_END
 }
 

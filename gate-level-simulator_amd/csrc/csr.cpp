@@ -2,7 +2,10 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstring>
+#include <exception>
 #include <map>
+#include <thread>
 #include <unordered_map>
 #include <unordered_set>
 
@@ -120,44 +123,36 @@ struct ChainEval {
 
 static int delay_kind(int edge) { return edge == 2 ? DK_BOTH : (edge == 0 ? DK_POS : DK_NEG); }
 
-Program compile_program(const Netlist& nl, const Library& lib,
-                        const std::vector<std::string>& stim_names) {
-  Program pg;
-  const int N = (int)nl.net_names.size();
-  pg.net_names = nl.net_names;
-  pg.n_netlist_nets = N;
-  pg.role.assign(N, NR_UNDRIVEN);
-  pg.const_val.assign(N, -1);
-  pg.in_saif.assign(N, 0);
+namespace {
 
-  std::unordered_set<std::string> stim(stim_names.begin(), stim_names.end());
-  // gate outputs
-  std::vector<int> driver_gate(N, -1);  // temporary: instance index
-  for (size_t i = 0; i < nl.insts.size(); ++i)
-    for (auto& op : nl.insts[i].out_pins) driver_gate[op.second] = (int)i;
-  for (int n = 0; n < N; ++n) {
-    if (stim.count(nl.net_names[n])) pg.role[n] = NR_STIM;
-    else if (driver_gate[n] >= 0) pg.role[n] = NR_GATE;
-    else if (nl.const_val[n] >= 0) { pg.role[n] = NR_CONST; pg.const_val[n] = nl.const_val[n]; }
-    else if (nl.alias_of[n] >= 0) pg.role[n] = NR_ALIAS;
-  }
+// Gates of a contiguous run of instances, compiled by one thread exactly as
+// a serial pass would compile them, with numbers local to the run: truth
+// tables index `luts` (first appearance in the run), and the pseudo nets of
+// unconnected outputs are numbered N + their index in `pseudo`.
+struct CompiledRun {
+  std::vector<CompiledGate> gates;
+  std::vector<std::vector<uint8_t>> luts;
+  std::vector<int> lut_k;
+  std::vector<std::string> pseudo;
+  std::exception_ptr err;  // the run stopped at its first error
+};
+
+void compile_run(const Netlist& nl, const Library& lib, const std::vector<uint8_t>& role,
+                 size_t i0, size_t i1, CompiledRun& R) {
+  const int N = (int)nl.net_names.size();
   // resolve a net read by an instance input to the net whose waveform is used
   auto resolve = [&](int n) -> int {
-    if (pg.role[n] == NR_ALIAS) {
+    if (role[n] == NR_ALIAS) {
       int t = nl.alias_of[n];  // one level, as the reference does
-      if (pg.role[t] == NR_STIM || pg.role[t] == NR_GATE || pg.role[t] == NR_CONST) return t;
+      if (role[t] == NR_STIM || role[t] == NR_GATE || role[t] == NR_CONST) return t;
       return -1;
     }
-    if (pg.role[n] == NR_UNDRIVEN) return -1;
+    if (role[n] == NR_UNDRIVEN) return -1;
     return n;
   };
-
-  // Compile gates (one per connected output and per unknown_out).
-  struct Pending { CompiledGate g; };
-  std::vector<CompiledGate> gates;
-  gates.reserve(nl.insts.size());
-  std::map<std::string, int> lut_cache;  // signature -> lut id
-  for (size_t ii = 0; ii < nl.insts.size(); ++ii) {
+  std::map<std::string, int> lut_cache;  // signature -> local lut id
+  R.gates.reserve(i1 - i0);
+  for (size_t ii = i0; ii < i1; ++ii) {
     const Instance& inst = nl.insts[ii];
     const CellType& ct = lib.types[inst.type];
     std::vector<int> dnets = inst.in_nets();
@@ -204,18 +199,13 @@ Program compile_program(const Netlist& nl, const Library& lib,
     std::vector<OutSpec> outs;
     for (auto& op : inst.out_pins) outs.push_back({nl.net_names[op.second], op.second, op.first});
     for (size_t k = 0; k < inst.unknown_out.size(); ++k) {
-      std::string pseudo = inst.name + "[" + inst.unknown_out[k] + "]";
-      int id = (int)pg.net_names.size();
-      pg.net_names.push_back(pseudo);
-      pg.role.push_back(NR_GATE);
-      pg.const_val.push_back(-1);
-      pg.in_saif.push_back(1);
+      const int id = N + (int)R.pseudo.size();
+      R.pseudo.push_back(inst.name + "[" + inst.unknown_out[k] + "]");
       outs.push_back({inst.unknown_out[k], id, inst.unknown_out[k]});
-      (void)k;
     }
     for (size_t oi = 0; oi < outs.size(); ++oi) {
       const OutSpec& os = outs[oi];
-      if (os.net < N && pg.role[os.net] != NR_GATE) continue;  // a VCD signal overrides this output
+      if (os.net < N && role[os.net] != NR_GATE) continue;  // a VCD signal overrides this output
       CompiledGate g;
       g.inst = (int)ii;
       g.out_net = os.net;
@@ -238,12 +228,12 @@ Program compile_program(const Netlist& nl, const Library& lib,
           t[e] = r;
         }
         int id = -1;
-        for (size_t q = 0; q < pg.luts.size(); ++q)
-          if (pg.luts[q] == t && pg.lut_k[q] == g.k) { id = (int)q; break; }
+        for (size_t q = 0; q < R.luts.size(); ++q)
+          if (R.luts[q] == t && R.lut_k[q] == g.k) { id = (int)q; break; }
         if (id < 0) {
-          id = (int)pg.luts.size();
-          pg.luts.push_back(t);
-          pg.lut_k.push_back(g.k);
+          id = (int)R.luts.size();
+          R.luts.push_back(t);
+          R.lut_k.push_back(g.k);
         }
         if (!collide) lut_cache[key] = id;
         g.lut = id;
@@ -264,8 +254,90 @@ Program compile_program(const Netlist& nl, const Library& lib,
             throw GlsError(GLS_E_UNSUPPORTED, "negative delay on " + inst.name);
         }
       }
+      R.gates.push_back(std::move(g));
+    }
+  }
+}
+
+}  // namespace
+
+Program compile_program(const Netlist& nl, const Library& lib,
+                        const std::vector<std::string>& stim_names, int threads) {
+  Program pg;
+  const int N = (int)nl.net_names.size();
+  pg.net_names = nl.net_names;
+  pg.n_netlist_nets = N;
+  pg.role.assign(N, NR_UNDRIVEN);
+  pg.const_val.assign(N, -1);
+  pg.in_saif.assign(N, 0);
+
+  std::unordered_set<std::string> stim(stim_names.begin(), stim_names.end());
+  // gate outputs
+  std::vector<int> driver_gate(N, -1);  // temporary: instance index
+  for (size_t i = 0; i < nl.insts.size(); ++i)
+    for (auto& op : nl.insts[i].out_pins) driver_gate[op.second] = (int)i;
+  for (int n = 0; n < N; ++n) {
+    if (stim.count(nl.net_names[n])) pg.role[n] = NR_STIM;
+    else if (driver_gate[n] >= 0) pg.role[n] = NR_GATE;
+    else if (nl.const_val[n] >= 0) { pg.role[n] = NR_CONST; pg.const_val[n] = nl.const_val[n]; }
+    else if (nl.alias_of[n] >= 0) pg.role[n] = NR_ALIAS;
+  }
+
+  // Compile gates (one per connected output and per unknown_out) in runs of
+  // instances on parallel threads, then number truth tables and pseudo nets
+  // in instance order: the first appearance of a table in the netlist fixes
+  // its number, exactly as one serial pass numbers it.
+  const size_t NI = nl.insts.size();
+  threads = host_threads(threads);
+  const size_t nrun = std::max<size_t>(1, std::min<size_t>((size_t)threads, NI / 16384));
+  std::vector<CompiledRun> runs(nrun);
+  auto work = [&](size_t r) {
+    try {
+      compile_run(nl, lib, pg.role, NI * r / nrun, NI * (r + 1) / nrun, runs[r]);
+    } catch (...) {
+      runs[r].err = std::current_exception();
+    }
+  };
+  if (nrun == 1) {
+    work(0);
+  } else {
+    std::vector<std::thread> th;
+    for (size_t r = 0; r < nrun; ++r) th.emplace_back(work, r);
+    for (auto& t : th) t.join();
+  }
+  // the first error in instance order is the one a serial pass reports
+  for (auto& R : runs)
+    if (R.err) std::rethrow_exception(R.err);
+  std::vector<CompiledGate> gates;
+  size_t total = 0;
+  for (auto& R : runs) total += R.gates.size();
+  gates.reserve(total);
+  for (auto& R : runs) {
+    std::vector<int> lut_id(R.luts.size());
+    for (size_t q = 0; q < R.luts.size(); ++q) {
+      int id = -1;
+      for (size_t p = 0; p < pg.luts.size(); ++p)
+        if (pg.luts[p] == R.luts[q] && pg.lut_k[p] == R.lut_k[q]) { id = (int)p; break; }
+      if (id < 0) {
+        id = (int)pg.luts.size();
+        pg.luts.push_back(R.luts[q]);
+        pg.lut_k.push_back(R.lut_k[q]);
+      }
+      lut_id[q] = id;
+    }
+    const int pseudo0 = (int)pg.net_names.size();
+    for (auto& nm : R.pseudo) {
+      pg.net_names.push_back(std::move(nm));
+      pg.role.push_back(NR_GATE);
+      pg.const_val.push_back(-1);
+      pg.in_saif.push_back(1);
+    }
+    for (auto& g : R.gates) {
+      g.lut = lut_id[g.lut];
+      if (g.out_net >= N) g.out_net = pseudo0 + (g.out_net - N);
       gates.push_back(std::move(g));
     }
+    R = CompiledRun();
   }
 
   // SAIF membership: gate outputs, VCD signals, visible constants.
@@ -274,52 +346,108 @@ Program compile_program(const Netlist& nl, const Library& lib,
     if (pg.role[n] == NR_CONST && !nl.const_hidden[n]) pg.in_saif[n] = 1;
   }
 
-  // Levels: longest path from stimulus / constants (Kahn).
+  // Levels: longest path from stimulus / constants (Kahn), readers in CSR.
   const int G = (int)gates.size();
   std::vector<int> gate_of_net(pg.net_names.size(), -1);
   for (int g = 0; g < G; ++g) gate_of_net[gates[g].out_net] = g;
   std::vector<int> indeg(G, 0);
-  std::vector<std::vector<int>> readers(G);
-  for (int g = 0; g < G; ++g) {
+  std::vector<uint32_t> rd_off(G + 1, 0);
+  auto src_of = [&](int g, int j) {
+    const int n = gates[g].in_net[j];
+    const int s = gate_of_net[n];
+    return (s >= 0 && pg.role[n] == NR_GATE) ? s : -1;
+  };
+  for (int g = 0; g < G; ++g)
     for (int j = 0; j < gates[g].k; ++j) {
-      int src = gate_of_net[gates[g].in_net[j]];
-      if (src >= 0 && pg.role[gates[g].in_net[j]] == NR_GATE) {
-        readers[src].push_back(g);
+      const int s = src_of(g, j);
+      if (s >= 0) {
+        rd_off[s + 1]++;
         indeg[g]++;
       }
     }
+  for (int g = 0; g < G; ++g) rd_off[g + 1] += rd_off[g];
+  std::vector<int> readers(rd_off[G]);
+  {
+    std::vector<uint32_t> fill(rd_off.begin(), rd_off.end() - 1);
+    for (int g = 0; g < G; ++g)
+      for (int j = 0; j < gates[g].k; ++j) {
+        const int s = src_of(g, j);
+        if (s >= 0) readers[fill[s]++] = g;
+      }
   }
-  std::vector<int> level(G, 0), order;
-  order.reserve(G);
   std::vector<int> frontier;
   for (int g = 0; g < G; ++g)
     if (indeg[g] == 0) frontier.push_back(g);
-  int lv = 0;
   std::vector<std::vector<int>> by_level;
   while (!frontier.empty()) {
     std::sort(frontier.begin(), frontier.end());
-    by_level.push_back(frontier);
     std::vector<int> next;
-    for (int g : frontier) {
-      level[g] = lv;
-      for (int r : readers[g])
-        if (--indeg[r] == 0) next.push_back(r);
-    }
+    for (int g : frontier)
+      for (uint32_t q = rd_off[g]; q < rd_off[g + 1]; ++q)
+        if (--indeg[readers[q]] == 0) next.push_back(readers[q]);
+    by_level.push_back(std::move(frontier));
     frontier.swap(next);
-    ++lv;
   }
   size_t placed = 0;
   for (auto& l : by_level) placed += l.size();
   if ((int)placed != G) throw GlsError(GLS_E_CYCLE, "combinational loop in the netlist");
+  pg.gates.reserve(G);
   pg.level_start.push_back(0);
   for (auto& l : by_level) {
     for (int g : l) {
-      pg.gates.push_back(gates[g]);
-      for (int j = 0; j < gates[g].k; ++j) pg.n_pins++;
+      pg.n_pins += (uint64_t)gates[g].k;
+      pg.gates.push_back(std::move(gates[g]));
     }
     pg.level_start.push_back((uint32_t)pg.gates.size());
   }
   return pg;
+}
+
+uint64_t program_digest(const Program& pg) {
+  uint64_t h = 1469598103934665603ull;
+  auto bytes = [&](const void* p, size_t n) {
+    const unsigned char* b = (const unsigned char*)p;
+    for (size_t i = 0; i < n; ++i) h = (h ^ b[i]) * 1099511628211ull;
+  };
+  auto u64 = [&](uint64_t x) { bytes(&x, 8); };
+  auto f64 = [&](double x) {
+    uint64_t b;
+    std::memcpy(&b, &x, 8);
+    u64(b);
+  };
+  u64(pg.net_names.size());
+  for (size_t n = 0; n < pg.net_names.size(); ++n) {
+    u64(pg.net_names[n].size());
+    bytes(pg.net_names[n].data(), pg.net_names[n].size());
+    u64(pg.role[n]);
+    u64((uint64_t)(int64_t)pg.const_val[n]);
+    u64(pg.in_saif[n]);
+  }
+  u64(pg.gates.size());
+  for (const CompiledGate& g : pg.gates) {
+    u64((uint64_t)(int64_t)g.inst);
+    u64((uint64_t)(int64_t)g.out_net);
+    u64((uint64_t)(int64_t)g.k);
+    for (int j = 0; j < kMaxK; ++j) u64((uint64_t)(int64_t)g.in_net[j]);
+    u64((uint64_t)(int64_t)g.lut);
+    for (int j = 0; j < kMaxK; ++j) {
+      u64(g.recs[j].size());
+      for (auto& r : g.recs[j]) {
+        u64((uint64_t)(int64_t)r.kind);
+        f64(r.rise);
+        f64(r.fall);
+      }
+    }
+  }
+  for (uint32_t l : pg.level_start) u64(l);
+  for (size_t q = 0; q < pg.luts.size(); ++q) {
+    u64((uint64_t)(int64_t)pg.lut_k[q]);
+    u64(pg.luts[q].size());
+    bytes(pg.luts[q].data(), pg.luts[q].size());
+  }
+  u64(pg.n_pins);
+  u64((uint64_t)(int64_t)pg.n_netlist_nets);
+  return h;
 }
 
 GateDelays gate_delays(const CompiledGate& g, int timescale_ps) {

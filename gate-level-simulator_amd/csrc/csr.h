@@ -69,7 +69,12 @@ uint8_t eval_prim(int op, const uint8_t* in, int n);
 // Throws GlsError(GLS_E_CYCLE) for combinational loops / undriven nets and
 // GlsError(GLS_E_UNSUPPORTED) for gates outside the device domain.
 Program compile_program(const Netlist& nl, const Library& lib,
-                        const std::vector<std::string>& stim_names);
+                        const std::vector<std::string>& stim_names, int threads = 0);
+
+// 64-bit FNV-1a digest of a compiled program: net names and roles, every
+// gate (nets, truth-table number, IOPATH records bit for bit), the level
+// boundaries and the truth tables.
+uint64_t program_digest(const Program& pg);
 
 // Integer delays for the given simulation time unit (ps per VCD tick):
 // trunc(value_ps * factor) exactly as src/simulate.cpp:1165-1175.

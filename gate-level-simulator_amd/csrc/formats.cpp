@@ -397,7 +397,7 @@ Netlist parse_gv_text(const std::string& text, const Library& lib, int threads) 
   }
   const size_t n_stmt = first.size() - 1;
   // 2 + 3, in rounds that bound the memory held by tokenised statements
-  if (threads <= 0) threads = (int)std::min<unsigned>(16, std::max(1u, std::thread::hardware_concurrency()));
+  threads = host_threads(threads);
   if (text.size() < (8u << 20)) threads = 1;
   int hidden_const = 0;
   auto const_net = [&](const std::string& v) {
@@ -607,7 +607,7 @@ void parse_sdf_text(const std::string& text, Netlist& nl, int threads) {
   by_name.reserve(nl.insts.size());
   for (size_t i = 0; i < nl.insts.size(); ++i) by_name[nl.insts[i].name] = (int)i;
   const size_t n = text.size();
-  if (threads <= 0) threads = (int)std::min<unsigned>(16, std::max(1u, std::thread::hardware_concurrency()));
+  threads = host_threads(threads);
   if (n < (8u << 20)) threads = 1;
   // stretch starts: the line after a line that trims to ")"
   std::vector<size_t> cut{0};
@@ -1027,7 +1027,7 @@ Stimulus parse_vcd_text(const std::string& s, int threads) {
   Stimulus st;
   VcdHeader H = parse_vcd_header(s, st);
   const size_t a = H.body, n = s.size();
-  if (threads <= 0) threads = (int)std::min<unsigned>(16, std::max(1u, std::thread::hardware_concurrency()));
+  threads = host_threads(threads);
   if (n - a < (8u << 20) || s.find("$comment", a) != std::string::npos) threads = 1;
   std::vector<size_t> cut{a};
   for (int i = 1; i < threads; ++i) {

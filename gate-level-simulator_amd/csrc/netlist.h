@@ -4,14 +4,29 @@
 // The 4-state value codes are the reference's VCDBit layout
 // (src/VCDTypes.h:49-54): 0 -> 0, X -> 1, Z -> 2, 1 -> 3.
 #pragma once
+#include <algorithm>
 #include <cstdint>
+#include <cstdlib>
 #include <map>
 #include <stdexcept>
 #include <string>
+#include <thread>
 #include <unordered_map>
 #include <vector>
 
 namespace gls {
+
+// Host threads of the parallel readers and the netlist compiler: `requested`
+// when positive, else the GLS_HOST_THREADS environment variable when set to
+// a positive count, else one per hardware thread up to 16.
+inline int host_threads(int requested) {
+  if (requested > 0) return requested;
+  if (const char* e = std::getenv("GLS_HOST_THREADS")) {
+    const int v = std::atoi(e);
+    if (v > 0) return v;
+  }
+  return (int)std::min<unsigned>(16, std::max(1u, std::thread::hardware_concurrency()));
+}
 
 enum : uint8_t { V0 = 0, VX = 1, VZ = 2, V1 = 3 };
 

@@ -28,7 +28,8 @@ class GlsError(RuntimeError):
 
 class DesignInfo(ctypes.Structure):
     _fields_ = [(n, ctypes.c_uint64) for n in
-                ("n_instances", "n_nets", "n_inputs", "n_gates", "n_levels", "n_pins")]
+                ("n_instances", "n_nets", "n_inputs", "n_gates", "n_levels", "n_pins",
+                 "program_digest")]
 
 
 class SimOpts(ctypes.Structure):

@@ -64,8 +64,12 @@ int gls_design_write(const gls_design* d, const char* vlib_path, const char* gv_
 typedef struct {
   uint64_t n_instances, n_nets, n_inputs; /* netlist */
   uint64_t n_gates, n_levels, n_pins;     /* compiled program (after levelization) */
+  uint64_t program_digest;                /* digest of the compiled program (gates, levels,
+                                             truth tables, delays) */
 } gls_design_info;
-/* Counts; levelizes against the design's declared inputs. */
+/* Counts; levelizes against the design's declared inputs.  The compiler
+ * runs on up to 16 host threads (GLS_HOST_THREADS caps them, as it caps the
+ * readers); its program does not depend on the count. */
 int gls_design_info_get(const gls_design* d, gls_design_info* info);
 void gls_design_free(gls_design* d);
 

@@ -1,4 +1,5 @@
-"""The gv and SDF readers give the same design on one thread and on many.
+"""The gv and SDF readers and the netlist compiler give the same result on
+one thread and on many.
 
 Large texts (over 8 MiB) are read in parallel: SDF in stretches that start
 after a line closing a block, gv as a serial statement scan, parallel
@@ -11,8 +12,11 @@ blocks), blocks for unknown instances, repeated blocks, statements over
 several lines, constants, bus bits first seen on a pin, wires declared late,
 `assign`, unknown modules, concatenations, and planted errors.
 """
+import os
 import random
 import re
+import subprocess
+import sys
 
 import pytest
 
@@ -178,3 +182,50 @@ def test_gv_first_error_wins(texts, errors, expect):
     vlib, gv, sdf = texts
     r = same_for_all_threads(vlib, edit_gv(gv, 5, errors), sdf)
     assert r[0] == "error" and r[1] == -2 and expect in r[2], r
+
+
+def edit_for_compile(gv, seed):
+    """Unconnected outputs (pseudo nets) throughout, and late in the file
+    instances with one net on two input pins (truth tables first seen there)."""
+    rng = random.Random(seed)
+    lines = gv.split("\n")
+    count = {}
+    for l in lines:
+        if l.startswith("  GEN_"):
+            for n in re.findall(r"\((\w+)\)", l):
+                count[n] = count.get(n, 0) + 1
+    insts = [i for i, l in enumerate(lines) if l.startswith("  GEN_")]
+    for pos, i in enumerate(insts):
+        l = lines[i]
+        m = re.search(r", \.(zn|z)\((\w+)\)\);", l)
+        if m and count.get(m.group(2), 0) == 1 and rng.random() < 0.3:
+            l = l.replace(m.group(0), ");")
+        if pos > len(insts) * 0.95 and rng.random() < 0.05:
+            ins = [p for p in re.findall(r"\.(\w+)\((\w+)\)", l) if p[0] not in ("z", "zn")]
+            if len(ins) >= 2:
+                l = l.replace(".%s(%s)" % ins[1], ".%s(%s)" % (ins[1][0], ins[0][1]), 1)
+        lines[i] = l
+    return "\n".join(lines)
+
+
+def program_digest_one_thread(paths):
+    code = ("import sys, gls; d = gls.Design.from_files(*sys.argv[1:4]); "
+            "print(d.info()['program_digest'])")
+    env = dict(os.environ, GLS_HOST_THREADS="1",
+               PYTHONPATH=os.pathsep.join(p for p in sys.path if p))
+    out = subprocess.run([sys.executable, "-c", code] + list(paths), env=env, check=True,
+                         capture_output=True, text=True, timeout=600)
+    return int(out.stdout.strip().split()[-1])
+
+
+def test_compile_same_on_one_thread(texts, tmp_path):
+    vlib, gv, sdf = texts
+    cases = [(gv, sdf), (edit_for_compile(gv, 1), "")]
+    for k, (g, s) in enumerate(cases):
+        paths = [str(tmp_path / ("%d.%s" % (k, e))) for e in ("vlib", "gv", "sdf")]
+        for p, t in zip(paths, (vlib, g, s)):
+            with open(p, "w") as f:
+                f.write(t)
+        info = gls.Design.from_files(*paths).info()  # compiled on several threads
+        assert info["n_gates"] > 0
+        assert program_digest_one_thread(paths) == info["program_digest"], k
