@@ -240,7 +240,8 @@ int Server::cancel(uint64_t id) {
     if (r->info.status <= 0) return r->info.status;  // finished: keeps its result
     r->stop.store(true, std::memory_order_relaxed);
     if (r->info.status == 1) {
-      queue_.erase(std::find(queue_.begin(), queue_.end(), id));
+      auto q = std::find(queue_.begin(), queue_.end(), id);
+      if (q != queue_.end()) queue_.erase(q);
       fail_queued(r);
     }
     st = r->info.status;
