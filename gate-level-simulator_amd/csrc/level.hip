@@ -918,7 +918,21 @@ __global__ void k_tile_desc(const GateRec* __restrict__ gates, uint32_t ng,
     if (clast + 1 < P.nseg) {
       const uint32_t tau_nx = lw[(clast + 1) * P.stride] >> 2;
       Ln = (uint32_t)j == P.lead ? (clast + 1) * P.stride : lb_words(iw, n, tau_nx << 2);
-      E = lb_words(iw, n, (tau_nx + G.dmax) << 2);
+      // halo end: the first word at or after tau_nx + dmax lies at or after
+      // Ln, usually at it; step forward with doubling strides, then search
+      // only inside the last stride (every word before a is below the key)
+      const uint32_t key = (tau_nx + G.dmax) << 2;
+      E = Ln;
+      if (E < n && iw[E] < key) {
+        uint32_t a = E, d = 1, b = a + 1;
+        while (b < n && iw[b] < key) {
+          a = b;
+          d <<= 1;
+          b = a + d;
+        }
+        if (b > n) b = n;
+        E = a + 1 + lb_words(iw + a + 1, b - a - 1, key);
+      }
     }
     D.L[j] = L;
     D.E[j] = E;
