@@ -418,17 +418,22 @@ Netlist parse_gv_text(const std::string& text, const Library& lib, int threads) 
     auto work = [&](size_t lo, size_t hi) {
       std::string stmt;
       for (size_t k = lo; k < hi; ++k) {
-        stmt.clear();
-        for (size_t x = first[r0 + k]; x < first[r0 + k + 1]; ++x) {
-          stmt += ' ';
-          stmt.append(text, spans[x].first, spans[x].second);
-        }
         try {
-          parse_gv_stmt(stmt, lib, R[k]);
-        } catch (...) {
+          stmt.clear();
+          for (size_t x = first[r0 + k]; x < first[r0 + k + 1]; ++x) {
+            stmt += ' ';
+            stmt.append(text, spans[x].first, spans[x].second);
+          }
+          try {
+            parse_gv_stmt(stmt, lib, R[k]);
+          } catch (...) {
+            R[k].err = std::current_exception();
+          }
+          if (R[k].kind == 3) place_pins(nl, R[k]);
+        } catch (...) {  // out of memory: reported as soon as the serial pass reaches it
+          R[k].kind = 0;
           R[k].err = std::current_exception();
         }
-        if (R[k].kind == 3) place_pins(nl, R[k]);
       }
     };
     const size_t nt = std::min<size_t>((size_t)threads, (nr + 1023) / 1024);
@@ -1066,8 +1071,13 @@ Stimulus parse_vcd_text(const std::string& s, int threads) {
   }
   const size_t nsig = H.is_vec_bit.size();
   const size_t nthr = std::min<size_t>(np, nsig ? nsig : 1);
+  std::vector<std::exception_ptr> ferr(std::max<size_t>(nthr, 1));
   auto finish = [&](size_t w) {
-    for (size_t g = w; g < nsig; g += nthr) finish_vcd_wave(parts, (int)g, H.is_vec_bit[g] != 0, st.waves[g]);
+    try {
+      for (size_t g = w; g < nsig; g += nthr) finish_vcd_wave(parts, (int)g, H.is_vec_bit[g] != 0, st.waves[g]);
+    } catch (...) {
+      ferr[w] = std::current_exception();
+    }
   };
   if (nthr <= 1) {
     finish(0);
@@ -1076,6 +1086,8 @@ Stimulus parse_vcd_text(const std::string& s, int threads) {
     for (size_t w = 0; w < nthr; ++w) th.emplace_back(finish, w);
     for (auto& t : th) t.join();
   }
+  for (auto& e : ferr)
+    if (e) std::rethrow_exception(e);
   return st;
 }
 
