@@ -701,7 +701,17 @@ __global__ __launch_bounds__(kLanes) void k_level(LevelArgs A) {
 // entry repeats the last value before it drops that entry), place each
 // tile's output (offset inside the gate, bit 31: first entry dropped), and
 // add up the SAIF durations: inside tiles, between tiles, and from the last
-// entry to the window end.  One thread per gate.
+// entry to the window end.  One wavefront per gate, 64 tiles per step: each
+// lane takes the last value and time of the nearest non-empty tile before
+// it (a ballot of the non-empty lanes, then a shuffle; the step's first
+// non-empty tile takes them from the previous step), the offsets are an
+// exclusive scan of the joined counts, and the sums are reduced at the end.
+__device__ __forceinline__ unsigned long long wave_sum_u64(unsigned long long x) {
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) x += __shfl_xor(x, d);
+  return x;
+}
+
 __global__ void k_gate_carry(const GateRec* __restrict__ gates, uint32_t ng,
                              const uint64_t* __restrict__ tile_base,
                              const uint64_t* __restrict__ ntile, const TileOut* __restrict__ tout,
@@ -709,45 +719,75 @@ __global__ void k_gate_carry(const GateRec* __restrict__ gates, uint32_t ng,
                              uint32_t* __restrict__ wlen, unsigned long long* __restrict__ saif,
                              const unsigned long long* __restrict__ win,
                              const unsigned int* __restrict__ err) {
-  const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
-  if (g >= ng) return;
+  const int lane = threadIdx.x & 63;
+  const uint32_t g = blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64;
+  if (g >= ng) return;  // uniform over the wavefront
   const GateRec& G = gates[g];
   if (*err & ERR_CAPACITY) {  // the level was skipped: its tiles were never walked
-    gate_cnt[g] = 0;
-    wlen[G.out_net] = 0;
+    if (lane == 0) {
+      gate_cnt[g] = 0;
+      wlen[G.out_net] = 0;
+    }
     return;
   }
   const unsigned long long start = win[2 * G.job], end = win[2 * G.job + 1];
   const uint64_t t0 = tile_base[g], n = ntile[g];
+  // carried from the previous step: count so far, last value and time
   uint32_t cnt = 0, lv = 0, lt = 0, has = 0;
   unsigned long long acc[3] = {0, 0, 0}, tc = 0;
-  for (uint64_t t = t0; t < t0 + n; ++t) {
-    const TileOut S = tout[t];
-    if (!((S.flags >> 4) & 1u)) {  // no entry in this tile
-      tile_off[t] = cnt;
-      continue;
+  for (uint64_t c = 0; c < n; c += 64) {
+    const bool in = c + (uint64_t)lane < n;
+    const uint64_t t = t0 + c + (uint64_t)lane;
+    TileOut S = {};
+    if (in) S = tout[t];
+    const bool ne = in && ((S.flags >> 4) & 1u);
+    const uint64_t mask = __ballot(ne);
+    const uint32_t my_lv = (S.flags >> 2) & 3u, my_lt = S.lt, my_fv = S.flags & 3u;
+    // nearest non-empty tile below this lane in the step, if any
+    const uint64_t below = mask & ((1ull << lane) - 1ull);
+    const int pl = below ? 63 - __clzll((long long)below) : 0;
+    const uint32_t s_lv = __shfl(my_lv, pl), s_lt = __shfl(my_lt, pl);
+    const uint32_t p_has = below ? 1u : has;
+    const uint32_t p_lv = below ? s_lv : lv, p_lt = below ? s_lt : lt;
+    uint32_t mine = 0, skip = 0;
+    if (ne) {
+      skip = (p_has && p_lv == my_fv) ? 1u : 0u;
+      mine = S.cnt - skip;
+      if (p_has) saif_acc(acc, p_lv, p_lt, S.ft, start, end);
+      acc[0] += S.t0;
+      acc[1] += S.t1;
+      acc[2] += S.tx;
+      // the tile's first entry against the previous tile's last value
+      tc += S.tc + ((p_has && is_toggle(p_lv, my_fv) && S.ft > start && S.ft <= end) ? 1u : 0u);
     }
-    const uint32_t skip = (has && lv == (S.flags & 3u)) ? 1u : 0u;
-    tile_off[t] = cnt | (skip << 31);
-    if (has) saif_acc(acc, lv, lt, S.ft, start, end);
-    acc[0] += S.t0;
-    acc[1] += S.t1;
-    acc[2] += S.tx;
-    // the tile's first entry against the previous tile's last value
-    tc += S.tc + ((has && is_toggle(lv, S.flags & 3u) && S.ft > start && S.ft <= end) ? 1u : 0u);
-    cnt += S.cnt - skip;
-    lv = (S.flags >> 2) & 3u;
-    lt = S.lt;
-    has = 1;
+    // inclusive scan of the joined counts over the step
+    uint32_t incl = mine;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+      const uint32_t v = __shfl_up(incl, d);
+      if (lane >= d) incl += v;
+    }
+    if (in) tile_off[t] = (cnt + incl - mine) | (skip << 31);
+    cnt += __shfl(incl, 63);
+    if (mask) {  // the step's last non-empty tile carries on
+      const int ll = 63 - __clzll((long long)mask);
+      lv = __shfl(my_lv, ll);
+      lt = __shfl(my_lt, ll);
+      has = 1;
+    }
   }
-  if (has) saif_acc(acc, lv, lt, end > lt ? end : (unsigned long long)lt, start, end);
-  gate_cnt[g] = cnt;
-  wlen[G.out_net] = cnt;
-  unsigned long long* sa = saif + (uint64_t)kSaifWords * G.out_net;
-  sa[0] = acc[0];
-  sa[1] = acc[1];
-  sa[2] = acc[2];
-  sa[3] = tc;
+  if (has && lane == 0) saif_acc(acc, lv, lt, end > lt ? end : (unsigned long long)lt, start, end);
+  const unsigned long long s0 = wave_sum_u64(acc[0]), s1 = wave_sum_u64(acc[1]);
+  const unsigned long long s2 = wave_sum_u64(acc[2]), s3 = wave_sum_u64(tc);
+  if (lane == 0) {
+    gate_cnt[g] = cnt;
+    wlen[G.out_net] = cnt;
+    unsigned long long* sa = saif + (uint64_t)kSaifWords * G.out_net;
+    sa[0] = s0;
+    sa[1] = s1;
+    sa[2] = s2;
+    sa[3] = s3;
+  }
 }
 
 // Copy every tile's joined output from the scratch into the gate's dense
@@ -1009,8 +1049,8 @@ hipError_t launch_gate_carry(const GateRec* gates, uint32_t ng, const uint64_t* 
                              const unsigned long long* win, const unsigned int* err,
                              hipStream_t st) {
   if (ng == 0) return hipSuccess;
-  k_gate_carry<<<(ng + 255) / 256, 256, 0, st>>>(gates, ng, tile_base, ntile, tout, tile_off,
-                                                 gate_cnt, wlen, saif, win, err);
+  k_gate_carry<<<(ng + 3) / 4, 256, 0, st>>>(gates, ng, tile_base, ntile, tout, tile_off, gate_cnt,
+                                             wlen, saif, win, err);  // one wavefront per gate
   return hipGetLastError();
 }
 
