@@ -61,3 +61,26 @@ def test_binding_keyword_options():
         for k in ("seg_events", "level_blocks", "prefetch", "graph", "hbm_stacks", "pool_grow",
                   "ovf_words", "saif_tc"):
             assert k in params, (fn.__name__, k)
+
+
+def test_null_handles_and_bad_arguments_are_status_codes():
+    import ctypes
+    lib = gls.load()
+    st = ctypes.c_int()
+    h = ctypes.c_uint64()
+    assert lib.gls_server_cancel(None, 1, ctypes.byref(st)) == -1  # GLS_E_ARG
+    assert lib.gls_server_submit_deadline(None, None, None, 0, 1, 1.0, ctypes.byref(h)) == -1
+    assert lib.gls_design_digest(None, ctypes.byref(h)) == -1
+    d = ctypes.c_void_p()
+    assert lib.gls_design_from_texts_threads(b"", b"", None, -1, ctypes.byref(d)) == -1
+    assert lib.gls_design_from_texts_threads(None, b"", None, 0, ctypes.byref(d)) == -1
+
+
+def test_digests_follow_the_design():
+    import golden_cases
+    a, _ = golden_cases.make("c0_small")
+    b, _ = golden_cases.make("c0_small")
+    c = gls.Design.generate(0, 12345, 400, 20, 64)
+    assert a.digest() == b.digest() != c.digest()
+    ia, ib, ic = a.info(), b.info(), c.info()
+    assert ia["program_digest"] == ib["program_digest"] != ic["program_digest"]
