@@ -45,9 +45,7 @@ struct gls_run {
   bool keep_waves = false;
   bool saif_tc = false;  // SAIF text carries (TC n) records
   bool executed = false;
-  std::vector<uint64_t> saif;  // 3 per net of the union net space
-  std::vector<uint64_t> woff;
-  std::vector<uint32_t> wlen, pool;
+  std::vector<uint64_t> saif;  // kSaifWords per net of the union net space
 };
 
 namespace gls {
@@ -372,6 +370,10 @@ static void apply_opts(Job& job, const gls_sim_opts* opts) {
   job.pool_grow = opts->pool_grow ? 1u : 0u;
   job.ovf_words_opt = opts->ovf_words;
   job.saif_tc = opts->saif_tc ? 1u : 0u;
+  job.keep_waves = opts->keep_waves ? 1u : 0u;
+  if (opts->path == GLS_PATH_TWO_PASS && opts->chunk_ps != 0 && opts->chunk_ps != GLS_NO_CHUNKS)
+    throw GlsError(GLS_E_UNSUPPORTED, "time chunks need the fused path");
+  job.chunk_ps = opts->chunk_ps;
 }
 
 int gls_prepare_batch(gls_ctx* c, size_t n, const gls_design* const* designs,
@@ -416,7 +418,6 @@ int gls_run_execute(gls_run* r) {
   if (!r) throw GlsError(GLS_E_ARG, "null argument");
   r->job->run();
   r->job->fetch_saif(r->saif);
-  if (r->keep_waves) r->job->fetch_waves(r->woff, r->wlen, r->pool);
   r->executed = true;
   API_END
 }
@@ -455,6 +456,8 @@ int gls_run_stats_get(const gls_run* r, gls_run_stats* st) {
   st->tiles = j.tiles;
   st->tiles_hbm = j.tiles_hbm;
   st->pool_used_bytes = j.pool_words_used * 4;
+  st->chunks = j.chunks;
+  st->chunk_retries = j.chunk_retries;
   API_END
 }
 
@@ -524,38 +527,87 @@ int gls_run_net_toggles(const gls_run* r, const char* net, uint64_t* tc) {
   API_END
 }
 
-int gls_run_waveform(const gls_run* r, const char* net, uint32_t* words, size_t cap, size_t* n) {
-  API_BEGIN
-  if (!r || !net || !n) throw GlsError(GLS_E_ARG, "null argument");
+// Waveform of net `id` of instance i as absolute (time << 2 | value) words:
+// gate outputs from the run, stimulus signals and constants from their
+// inputs.
+static void net_words(const gls_run* r, size_t i, int id, std::vector<uint64_t>& out) {
+  const RunInst& in = inst_of(r, i);
+  out.clear();
+  if (in.is_gate[id]) {
+    const auto& W = r->job->waves();
+    const size_t n = r->job->net_base(i) + (size_t)id;
+    if (n < W.size()) out = W[n];
+    return;
+  }
+  for (size_t s = 0; s < in.stim_nets.size(); ++s)
+    if (in.stim_nets[s] == id) {
+      const Wave& w = in.st->waves[s];
+      for (size_t q = 0; q < w.size(); ++q) out.push_back(((uint64_t)w.t[q] << 2) | (w.v[q] & 3u));
+      return;
+    }
+  if (id < (int)in.pg->net_names.size() && in.pg->role[id] == NR_CONST)
+    out.push_back((uint64_t)(in.pg->const_val[id] & 3));
+}
+
+static int net_id(const gls_run* r, const char* net) {
   if (!r->executed || !r->keep_waves) throw GlsError(GLS_E_STATE, "waveforms not kept");
   const RunInst& in = inst_of(r, 0);
   auto it = in.by_name.find(net);
   if (it == in.by_name.end()) throw GlsError(GLS_E_ARG, std::string("unknown net ") + net);
-  int id = it->second;
-  *n = r->wlen[id];
+  return it->second;
+}
+
+int gls_run_waveform(const gls_run* r, const char* net, uint32_t* words, size_t cap, size_t* n) {
+  API_BEGIN
+  if (!r || !net || !n) throw GlsError(GLS_E_ARG, "null argument");
+  std::vector<uint64_t> w;
+  net_words(r, 0, net_id(r, net), w);
+  *n = w.size();
   if (words) {
-    if (cap < r->wlen[id]) throw GlsError(GLS_E_ARG, "buffer too small");
-    std::memcpy(words, r->pool.data() + r->woff[id], r->wlen[id] * 4ull);
+    if (cap < w.size()) throw GlsError(GLS_E_ARG, "buffer too small");
+    for (size_t q = 0; q < w.size(); ++q) {
+      if ((w[q] >> 2) >= (1ull << 30))
+        throw GlsError(GLS_E_RANGE, "time of 2^30 or more: use gls_run_waveform64");
+      words[q] = (uint32_t)w[q];
+    }
   }
   API_END
 }
 
-// Per-net FNV-1a over the name bytes then the little-endian event words,
-// finalised with SplitMix64 and summed over gate-output nets.
+int gls_run_waveform64(const gls_run* r, const char* net, uint64_t* words, size_t cap, size_t* n) {
+  API_BEGIN
+  if (!r || !net || !n) throw GlsError(GLS_E_ARG, "null argument");
+  std::vector<uint64_t> w;
+  net_words(r, 0, net_id(r, net), w);
+  *n = w.size();
+  if (words) {
+    if (cap < w.size()) throw GlsError(GLS_E_ARG, "buffer too small");
+    std::memcpy(words, w.data(), w.size() * 8);
+  }
+  API_END
+}
+
+// Per-net FNV-1a over the name bytes then the little-endian 32-bit event
+// words ((time << 2 | value) truncated to 32 bits), finalised with
+// SplitMix64 and summed over gate-output nets.
 int gls_run_instance_waves_hash(const gls_run* r, size_t i, uint64_t* hash) {
   API_BEGIN
   if (!r || !hash) throw GlsError(GLS_E_ARG, "null argument");
   if (!r->executed || !r->keep_waves) throw GlsError(GLS_E_STATE, "waveforms not kept");
   const RunInst& in = inst_of(r, i);
+  const auto& W = r->job->waves();
   const uint32_t base = r->job->net_base(i);
   uint64_t total = 0;
+  static const std::vector<uint64_t> empty;
   for (size_t n = 0; n < in.names.size(); ++n) {
     if (!in.is_gate[n]) continue;
     uint64_t h = 1469598103934665603ull;
     for (unsigned char ch : in.names[n]) h = (h ^ ch) * 1099511628211ull;
-    const uint32_t* w = r->pool.data() + r->woff[base + n];
-    for (uint32_t q = 0; q < r->wlen[base + n]; ++q)
-      for (int b = 0; b < 4; ++b) h = (h ^ ((w[q] >> (8 * b)) & 0xFF)) * 1099511628211ull;
+    const std::vector<uint64_t>& w = base + n < W.size() ? W[base + n] : empty;
+    for (uint64_t e : w) {
+      const uint32_t x = (uint32_t)e;
+      for (int b = 0; b < 4; ++b) h = (h ^ ((x >> (8 * b)) & 0xFF)) * 1099511628211ull;
+    }
     uint64_t z = h + 0x9E3779B97F4A7C15ull;
     z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
     z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
@@ -621,7 +673,6 @@ int gls_run_autotune(gls_run* r, gls_tune_report* rep) {
   j.prefetch = best_pf;
   j.run();
   r->job->fetch_saif(r->saif);
-  if (r->keep_waves) r->job->fetch_waves(r->woff, r->wlen, r->pool);
   r->executed = true;
   R.seg_events = best_seg;
   R.level_blocks = (uint32_t)j.stats().level_blocks;

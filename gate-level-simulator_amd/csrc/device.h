@@ -6,9 +6,14 @@
 
 namespace gls {
 
-// Event word: (time << 2) | value, time < 2^30 time units.
+// Event word: (time << 2) | value, time < 2^30 time units of the chunk's
+// relative time (see ChunkWin).  Resident stimulus words are 64-bit:
+// (absolute time << 2) | value.
 constexpr uint32_t kTimeInf = 0x3FFFFFFFu;
 constexpr uint32_t kWordInf = 0xFFFFFFFFu;
+// Largest span of one chunk in relative time: every event and candidate
+// time of a chunk stays below it (room is kept for one more delay).
+constexpr uint64_t kRelSpan = (1ull << 30) - (1ull << 18);
 
 // One simulated output ("gate"), 80 bytes.  Delays are integer time units;
 // 0xFFFF marks a missing IOPATH of that kind.
@@ -92,6 +97,34 @@ __host__ __device__ inline bool is_toggle(uint32_t p, uint32_t v) {
   return (p ^ v) == 3u && (v == 0u || v == 3u);
 }
 
+// Time chunks.  A simulation runs as a sequence of time chunks
+// [F_k, F_k+1); every chunk runs all levels.  Inside chunk k, times are
+// relative: rel = abs - (lb_k - 1) with lb_k = F_k - W (W = the largest gate
+// delay; lb_0 = 0), so every net's chunk waveform starts with a state entry
+// at rel 0 (its value before lb_k), followed by its entries at abs times in
+// [lb_k, F_k+1).  Entries at abs times in [lb_k, F_k) are the look-back
+// window: they were final after chunk k-1 and are walked again only so that
+// the candidates they make with output times >= F_k are known.
+//
+// Per instance and chunk: SAIF bounds in the chunk's relative time.
+// Durations are summed over [lo, hi) (the chunk's part of the SAIF window);
+// toggles are counted at entry times in (tc_lo, tc_hi].
+struct ChunkWin {
+  unsigned long long lo, hi, tc_lo, tc_hi;
+};
+
+// Tails of the gate outputs, the first words of their chunk waveforms: per
+// net, the state entry and the look-back entries, kept from the previous
+// chunk (in its relative time; `shift` = this chunk's time base minus the
+// previous one).  In chunk 0 every tail is one X state entry.
+struct TailArgs {
+  const uint32_t* tail_len;  // per net
+  const uint64_t* tail_off;  // per net: first word in `tails`
+  const uint32_t* tails;
+  uint32_t shift;
+  uint32_t first;            // 1 in chunk 0
+};
+
 // Arguments of the fused level (walk) kernel.
 struct LevelArgs {
   const TileDesc* tiles;      // per tile
@@ -109,10 +142,14 @@ struct LevelArgs {
   unsigned long long* tiles_walked;  // [0] tiles walked, [1] of them on the HBM path (per run)
   unsigned long long* ev_count;
   unsigned int* err;
-  const unsigned long long* win;  // SAIF window (start, end) per instance
+  const ChunkWin* cwin;        // SAIF bounds of this chunk per instance
   uint32_t cap;                // LDS input capacity per tile (words)
   uint32_t prefetch;           // 1: stage the next tile before storing this one
   uint32_t* stk;               // null: stacks in LDS; else (cap + 64) words per workgroup
+  uint32_t lo_rel;             // F_k in chunk time: candidates below it only pre-empt
+  uint32_t hi_rel;             // F_k+1 in chunk time (0xFFFFFFFF: last chunk): candidates
+                               // at or above it belong to the next chunk and are dropped
+  uint32_t first;              // 1 in chunk 0: outputs start with (0, X), f before t = 0 is X
 };
 
 // LDS bytes of one fused-level workgroup for an input capacity of `cap`

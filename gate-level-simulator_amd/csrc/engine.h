@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <atomic>
 #include <functional>
 #include <map>
 #include <memory>
@@ -32,6 +33,8 @@ struct JobStats {
   uint64_t pool_words_used = 0;
   uint64_t pool_words = 0;      // event pool capacity after the run
   uint64_t tiles = 0, tiles_hbm = 0;  // fused tiles walked, and of them on the HBM path
+  uint64_t chunks = 0;          // time chunks of the run (fused path)
+  uint64_t chunk_retries = 0;   // chunks run again shorter (pool too small)
 };
 
 // Host and device image of one compiled program for one delay time scale:
@@ -46,6 +49,7 @@ struct DevProgram {
   std::vector<uint16_t> h_dtab;
   std::vector<uint8_t> h_luts;   // 256 bytes per table
   std::vector<uint32_t> h_lutp;  // 16 packed words per table
+  // device tables (valid once is_resident(); one allocation, d_block)
   GateRec* d_grec = nullptr;
   GateDev* d_gdev = nullptr;
   uint16_t* d_dtab = nullptr;
@@ -57,7 +61,14 @@ struct DevProgram {
   DevProgram& operator=(const DevProgram&) = delete;
   ~DevProgram();
   size_t device_bytes() const;
-  void upload(int device, hipStream_t st);  // no-op when already resident
+  bool is_resident() const;
+  // No-op when already resident.  All or nothing: on failure nothing is
+  // published and a later call uploads again.
+  void upload(int device, hipStream_t st);
+
+ private:
+  void* d_block = nullptr;
+  std::atomic<bool> resident{false};
 };
 
 std::shared_ptr<DevProgram> build_program_image(const Program& pg, uint32_t timescale_ps);
@@ -91,6 +102,15 @@ class ProgramCache {
 
 // A prepared simulation of one or more instances, run level by level: level
 // l of every instance is evaluated by the same launches.
+//
+// The fused path runs the window as a sequence of time chunks (device.h,
+// ChunkWin): the stimulus stays resident; each chunk cuts its part of the
+// stimulus into the event pool, runs every level, adds its SAIF share and
+// keeps each gate output's tail for the next chunk.  One chunk covers the
+// whole window when it fits the pool; otherwise the chunk length adapts to
+// the pool (a chunk that would overflow it is run again shorter), and times
+// of any size up to 2^32 ps are simulated by moving each chunk to its own
+// time base.
 class Job {
  public:
   Job(int device, hipStream_t stream);
@@ -103,12 +123,18 @@ class Job {
   void run();
   // Results (union net space; instance i's nets start at net_base(i))
   void fetch_saif(std::vector<uint64_t>& t0t1tx) const;
-  void fetch_waves(std::vector<uint64_t>& off, std::vector<uint32_t>& len,
-                   std::vector<uint32_t>& pool_words) const;
+  // Waveforms of the gate outputs (absolute (time << 2 | value) words; other
+  // nets empty); needs keep_waves before run().
+  const std::vector<std::vector<uint64_t>>& waves() const { return waves_; }
   const JobStats& stats() const { return stats_; }
   size_t n_instances() const { return net_base_.size(); }
   uint32_t net_base(size_t i) const { return net_base_[i]; }
   uint64_t device_bytes() const { return dev_bytes_; }
+  // chunk boundaries of the last run (F_1 .. F_K; the last is kChunkEnd)
+  const std::vector<uint64_t>& chunks() const { return sched_; }
+
+  static constexpr uint64_t kChunkEnd = ~0ull;
+  static constexpr uint64_t kNoChunks = ~0ull;  // chunk_ps: one chunk, never split
 
   uint32_t seg_target = 0;        // input events per time segment (0: mode default)
   int mode = 0;                   // 0: fused LDS-tiled level kernel, 1: two-pass kernels
@@ -121,6 +147,10 @@ class Job {
   uint32_t graph = 0;
   uint32_t stack_mode = 0;  // 0: preemption stacks in LDS, 1: per-workgroup scratch in HBM
   uint32_t pool_grow = 0;   // 1: given pool and overflow-buffer sizes are starting sizes that grow
+  uint32_t keep_waves = 0;  // 1: collect every gate output's waveform on the host
+  // time-chunk length in time units (fused path): 0 automatic, kNoChunks one
+  // chunk (a pool too small fails with GLS_E_OOM), else a fixed length
+  uint64_t chunk_ps = 0;
   // Asked before each level of a plain execution; true stops the run with
   // GLS_E_CANCELLED after the stream has drained (job cancel or deadline).
   std::function<bool()> should_stop;
@@ -128,14 +158,30 @@ class Job {
   static constexpr uint32_t kDefaultSeg = 32;  // input events per time segment
 
  private:
+  struct Chunk {
+    uint64_t F, Fn;       // absolute bounds [F, Fn) (Fn = kChunkEnd: to the end)
+    uint64_t lb;          // look-back start (abs); relative time = abs - lb + 1
+    uint32_t lo_rel, hi_rel, shift, first;
+    size_t slot;          // ChunkWin slot of the chunk's windows
+  };
+  enum ChunkResult { CHUNK_OK = 0, CHUNK_SHORTER = 1 };
   void ensure(void** p, size_t& cap, size_t need);
-  void run_fused(std::vector<hipEvent_t>& ev_eval, bool device_driven = false);
+  bool try_ensure(void** p, size_t& cap, size_t need);
+  Chunk make_chunk(size_t k, uint64_t F, uint64_t Fn, uint64_t prev_lb) const;
+  void put_chunk_windows(const Chunk& c, bool capture);
+  int run_chunk(const Chunk& c, std::vector<hipEvent_t>& ev_eval, bool device_driven,
+                double* grow_hint);
+  void end_chunk(const Chunk& c, bool device_driven);
+  void run_chunks(std::vector<hipEvent_t>& ev_eval);
   void build_graph();
   void finish_stats(const uint64_t* scal, double ms);
   void run_two_pass(std::vector<hipEvent_t>& ev_eval);
   void stop_point();
   void* dmalloc(size_t bytes);
   bool grow_pool(uint64_t need, uint64_t used);
+  bool refresh_pool(uint64_t words);
+  uint64_t pool_max_words() const;
+  void fetch_chunk_waves(const Chunk& c);
   int device_;
   hipStream_t stream_;
   std::shared_ptr<DevProgram> prog_;
@@ -175,19 +221,42 @@ class Job {
   uint64_t ovf_words_ = 0;
   size_t cap_ovf_ = 0;
   int level_blocks_ = 0;
-  uint32_t cap_ = 0;  // LDS input words per fused tile
+  uint32_t cap_ = 0;  // LDS input capacity per tile (words)
   std::vector<void*> owned_;
-  uint64_t* d_woff_init_ = nullptr;
-  uint32_t* d_wlen_init_ = nullptr;
+  // two-pass path: initial waveform tables
+  std::vector<uint64_t> h_woff_init_;
+  std::vector<uint32_t> h_wlen_init_;
+  // fused path: resident stimulus (signals and constants), 64-bit words
+  uint32_t n_sig_ = 0;
+  uint64_t* d_store_ = nullptr;
+  uint64_t* d_soff_ = nullptr;
+  uint32_t *d_slen_ = nullptr, *d_snet_ = nullptr, *d_sjob_ = nullptr;
+  uint64_t *d_scnt_ = nullptr, *d_sbase_ = nullptr, *d_spart_ = nullptr;
+  uint32_t* d_sfirst_ = nullptr;
+  std::vector<uint32_t> h_slen_, h_snet_;
+  uint64_t t_last_ = 0;   // last stimulus time
+  uint32_t wmax_ = 1;     // largest gate delay (the look-back length)
+  uint64_t horizon_ = 0;  // no output time reaches it
+  // tails of the gate outputs
+  uint32_t* d_tail_len_ = nullptr;
+  uint64_t* d_tail_off_ = nullptr;
+  uint32_t* d_tails_ = nullptr;
+  size_t cap_tails_ = 0;
+  uint64_t *d_tlen_g_ = nullptr, *d_tsrc_g_ = nullptr, *d_toff_g_ = nullptr, *d_gpart_ = nullptr;
+  unsigned long long* d_saif_chunk_ = nullptr;
+  unsigned long long* d_wtot_ = nullptr;
+  ChunkWin* d_cwin_ = nullptr;
+  size_t cap_cwin_ = 0;
+  std::vector<uint64_t> sched_;  // F_1 .. F_K of the last run
   uint64_t max_tiles_ = 0, tiles_cap_ = 0, max_scratch_ = 0, scratch_cap_ = 0;
+  uint64_t max_tails_ = 0, max_bump_ = 0;
   bool graph_scratch_in_pool_ = false;  // captured levels keep their scratch at the pool top
   uint32_t* d_stk_ = nullptr;
   size_t cap_stk_ = 0;
   hipGraphExec_t gexec_ = nullptr;
   std::vector<uint64_t> graph_sig_;
   bool last_replay_ = false;
-  std::vector<uint64_t> h_woff_init_;
-  std::vector<uint32_t> h_wlen_init_;
+  std::vector<std::vector<uint64_t>> waves_;
   JobStats stats_;
 };
 

@@ -39,7 +39,7 @@ hipError_t launch_saif_nets(const uint32_t* nets, const uint32_t* job, uint32_t 
 // level.hip: fused path
 hipError_t launch_plan2(const GateRec* gates, uint32_t ng, const uint32_t* wlen,
                         uint32_t seg_target, GatePlan* plan, uint64_t* ntile, uint64_t* bound,
-                        unsigned int* err, hipStream_t st);
+                        unsigned int* err, const uint32_t* tail_len, hipStream_t st);
 hipError_t launch_tile_desc(const GateRec* gates, uint32_t ng, const GatePlan* plan,
                             const uint64_t* tile_base, const uint64_t* woff, const uint32_t* wlen,
                             const uint32_t* pool, const uint32_t* lutp, TileDesc* tiles,
@@ -50,16 +50,40 @@ hipError_t launch_pool_guard(uint64_t* scal, uint64_t pool_words, hipStream_t st
 hipError_t launch_gate_carry(const GateRec* gates, uint32_t ng, const uint64_t* tile_base,
                              const uint64_t* ntile, const TileOut* tout, uint32_t* tile_off,
                              uint64_t* gate_cnt, uint32_t* wlen, unsigned long long* saif,
-                             const unsigned long long* win, const unsigned int* err,
+                             const ChunkWin* cwin, const unsigned int* err, const TailArgs& T,
                              hipStream_t st);
 hipError_t launch_compact(const TileDesc* tiles, const uint64_t* ntiles_dev, uint64_t ntiles_cap,
                           const TileOut* tout, const uint32_t* tile_off, const uint64_t* scr_base,
                           const uint32_t* scratch, const uint64_t* gate_base, uint32_t* pool,
                           uint64_t* woff, uint32_t* wlen, const unsigned int* err,
-                          unsigned long long* saif, const unsigned long long* win, uint32_t count_tc,
-                          hipStream_t st);
+                          unsigned long long* saif, const ChunkWin* cwin, uint32_t count_tc,
+                          const TailArgs& T, hipStream_t st);
 hipError_t launch_set_scal(uint64_t* scal, uint64_t bump, hipStream_t st);
 hipError_t launch_level(const LevelArgs& a, unsigned nblocks, hipStream_t st);
+// chunk.hip: time chunks
+hipError_t launch_stim_count(const uint64_t* store, const uint64_t* soff, const uint32_t* slen,
+                             uint32_t ns, uint64_t lb, uint64_t ub, uint64_t* cnt, uint32_t* first,
+                             hipStream_t st);
+hipError_t launch_stim_copy(const uint64_t* store, const uint64_t* soff, const uint32_t* snet,
+                            uint32_t ns, uint64_t lb, const uint64_t* cnt, const uint32_t* first,
+                            const uint64_t* base, uint32_t* pool, uint64_t pool_words,
+                            uint64_t* woff, uint32_t* wlen, unsigned int* err, hipStream_t st);
+hipError_t launch_tails_init(const GateRec* gates, uint32_t ng, uint32_t* tail_len,
+                             uint64_t* tail_off, uint32_t* tails, hipStream_t st);
+hipError_t launch_saif_accum(const GateRec* gates, uint32_t ng, const unsigned long long* chunk,
+                             unsigned long long* saif, const uint32_t* wlen,
+                             const uint32_t* tail_len, unsigned long long* wtot, hipStream_t st);
+hipError_t launch_tail_len(const GateRec* gates, uint32_t ng, const uint64_t* woff,
+                           const uint32_t* wlen, const uint32_t* pool, uint32_t key, uint64_t* len,
+                           uint64_t* src, hipStream_t st);
+hipError_t launch_tail_copy(const GateRec* gates, uint32_t ng, const uint64_t* len,
+                            const uint64_t* src, const uint64_t* off, const uint32_t* pool,
+                            uint32_t* tails, uint64_t cap, uint32_t* tail_len, uint64_t* tail_off,
+                            unsigned int* err, hipStream_t st);
+hipError_t launch_saif_stim(const uint64_t* store, const uint64_t* soff, const uint32_t* slen,
+                            const uint32_t* snet, const uint32_t* sjob, uint32_t ns,
+                            unsigned long long* saif, const unsigned long long* win,
+                            hipStream_t st);
 int level_max_blocks(uint32_t cap, bool stacks_in_lds);
 
 // LDS input capacity of a fused tile: 64 segments of `seg` input events
@@ -88,7 +112,7 @@ static constexpr uint64_t kPoolPad = 16;
 
 // scalar slots in d_scal_
 enum { S_BUMP = 0, S_TOT_A = 1, S_TOT_B = 2, S_TOT_OUT = 3, S_EVC = 4, S_TICKET = 5,
-       S_OVF_USED = 6, S_ERR = 7, S_TILES = 8, S_TILES_HBM = 9 };
+       S_OVF_USED = 6, S_ERR = 7, S_TILES = 8, S_TILES_HBM = 9, S_TAILS = 10 };
 
 static uint32_t sel_delay(uint32_t r, uint32_t f, int s) {
   return s == 1 ? r : (s == 2 ? f : (s == 3 ? std::max(r, f) : std::min(r, f)));
@@ -146,9 +170,7 @@ static void fill_delay_table(const GateDelays& D, int k, uint16_t* tab) {
 // ------------------------------------------------------------- programs ---
 
 DevProgram::~DevProgram() {
-  void* ptrs[] = {d_grec, d_gdev, d_dtab, d_luts, d_lutp};
-  for (void* p : ptrs)
-    if (p) (void)hipFree(p);
+  if (d_block) (void)hipFree(d_block);
 }
 
 size_t DevProgram::device_bytes() const {
@@ -156,23 +178,44 @@ size_t DevProgram::device_bytes() const {
          h_dtab.size() * 2 + h_luts.size() + h_lutp.size() * 4;
 }
 
+bool DevProgram::is_resident() const { return resident.load(std::memory_order_acquire); }
+
+// The five tables share one device allocation and are published only after
+// it and every copy succeeded: a failed upload leaves the image
+// non-resident (all pointers null) and the next upload starts over.
 void DevProgram::upload(int dev, hipStream_t st) {
   std::lock_guard<std::mutex> lk(upload_mu);
-  if (d_grec) {
+  if (resident.load(std::memory_order_relaxed)) {
     if (dev != device) throw GlsError(GLS_E_ARG, "program image belongs to another device");
     return;
   }
+  const size_t sizes[5] = {h_grec.size() * sizeof(GateRec), h_gdev.size() * sizeof(GateDev),
+                           h_dtab.size() * 2, h_luts.size(), h_lutp.size() * 4};
+  const void* srcs[5] = {h_grec.data(), h_gdev.data(), h_dtab.data(), h_luts.data(), h_lutp.data()};
+  size_t off[5], total = 0;
+  for (int i = 0; i < 5; ++i) {
+    off[i] = total;
+    total += (std::max<size_t>(sizes[i], 1) + 255) / 256 * 256;
+  }
+  char* blk = nullptr;
+  HC(hipMalloc((void**)&blk, total));
+  try {
+    for (int i = 0; i < 5; ++i)
+      if (sizes[i]) HC(hipMemcpyAsync(blk + off[i], srcs[i], sizes[i], hipMemcpyHostToDevice, st));
+    HC(hipStreamSynchronize(st));
+  } catch (...) {
+    (void)hipStreamSynchronize(st);
+    (void)hipFree(blk);
+    throw;
+  }
   device = dev;
-  auto put = [&](void** d, const void* h, size_t bytes) {
-    HC(hipMalloc(d, std::max<size_t>(bytes, 256)));
-    if (bytes) HC(hipMemcpyAsync(*d, h, bytes, hipMemcpyHostToDevice, st));
-  };
-  put((void**)&d_grec, h_grec.data(), h_grec.size() * sizeof(GateRec));
-  put((void**)&d_gdev, h_gdev.data(), h_gdev.size() * sizeof(GateDev));
-  put((void**)&d_dtab, h_dtab.data(), h_dtab.size() * 2);
-  put((void**)&d_luts, h_luts.data(), h_luts.size());
-  put((void**)&d_lutp, h_lutp.data(), h_lutp.size() * 4);
-  HC(hipStreamSynchronize(st));
+  d_block = blk;
+  d_grec = (GateRec*)(blk + off[0]);
+  d_gdev = (GateDev*)(blk + off[1]);
+  d_dtab = (uint16_t*)(blk + off[2]);
+  d_luts = (uint8_t*)(blk + off[3]);
+  d_lutp = (uint32_t*)(blk + off[4]);
+  resident.store(true, std::memory_order_release);
 }
 
 std::shared_ptr<DevProgram> build_program_image(const Program& pg, uint32_t timescale_ps) {
@@ -238,14 +281,14 @@ std::shared_ptr<DevProgram> ProgramCache::get(const std::shared_ptr<const Progra
 bool ProgramCache::resident(const Program* pg, uint32_t timescale_ps) const {
   std::lock_guard<std::mutex> lk(mu_);
   auto it = map_.find(std::make_pair(pg, timescale_ps));
-  return it != map_.end() && it->second.second->d_grec != nullptr;
+  return it != map_.end() && it->second.second->is_resident();
 }
 
 size_t ProgramCache::device_bytes() const {
   std::lock_guard<std::mutex> lk(mu_);
   size_t b = 0;
   for (auto& kv : map_)
-    if (kv.second.second->d_grec) b += kv.second.second->device_bytes();
+    if (kv.second.second->is_resident()) b += kv.second.second->device_bytes();
   return b;
 }
 
@@ -319,7 +362,7 @@ Job::~Job() {
   for (void* p : owned_)
     if (p) (void)hipFree(p);
   void* grown[] = {d_segmap_, d_sums_, d_carry_, d_scratch_, d_tiles_, d_tout_, d_toff_, d_stk_,
-                   d_pool_, d_ovf_};
+                   d_pool_,   d_ovf_,  d_tails_, d_cwin_};
   for (void* p : grown)
     if (p) (void)hipFree(p);
 }
@@ -334,14 +377,32 @@ void* Job::dmalloc(size_t bytes) {
 }
 
 void Job::ensure(void** p, size_t& cap, size_t need) {
-  if (need <= cap && *p) return;
+  if (!try_ensure(p, cap, need)) HC(hipErrorOutOfMemory);
+}
+
+// As ensure(), but an allocation that fails leaves the buffer released
+// (cap 0) and returns false instead of throwing.
+bool Job::try_ensure(void** p, size_t& cap, size_t need) {
+  if (need <= cap && *p) return true;
   size_t n = std::max<size_t>(need, cap + cap / 2);
   n = std::max<size_t>(n, 4096);
   if (*p) HC(hipFree(*p));
   *p = nullptr;
-  HC(hipMalloc(p, n));
-  dev_bytes_ += n - cap;
+  dev_bytes_ -= cap;
+  cap = 0;
+  if (hipMalloc(p, n) != hipSuccess) {
+    (void)hipGetLastError();
+    *p = nullptr;
+    if (n == need || hipMalloc(p, need) != hipSuccess) {
+      (void)hipGetLastError();
+      *p = nullptr;
+      return false;
+    }
+    n = need;
+  }
+  dev_bytes_ += n;
   cap = n;
+  return true;
 }
 
 // Grows an automatically sized pool to at least `need` words, keeping its
@@ -377,6 +438,46 @@ bool Job::grow_pool(uint64_t need, uint64_t used) {
   return true;
 }
 
+// Largest pool an automatically sized pool may take between chunks (when
+// nothing in it needs keeping): 80% of the free HBM plus the current pool.
+uint64_t Job::pool_max_words() const {
+  size_t free_b = 0, total_b = 0;
+  hip_check(hipMemGetInfo(&free_b, &total_b), "hipMemGetInfo");
+  return (uint64_t)((free_b + pool_words_ * 4.0) * 0.8) / 4;
+}
+
+// Between chunks (the pool holds nothing to keep): replaces an automatically
+// sized pool by one of `words` words when that is larger (or the largest
+// that can be allocated above the current size).  Returns whether the pool
+// grew.
+bool Job::refresh_pool(uint64_t words) {
+  if (!pool_auto_ || words <= pool_words_) return false;
+  HC(hipStreamSynchronize(stream_));
+  HC(hipFree(d_pool_));
+  d_pool_ = nullptr;
+  dev_bytes_ -= (pool_words_ + kPoolPad) * sizeof(uint32_t);
+  uint64_t n = words;
+  while (n > pool_words_) {
+    if (hipMalloc((void**)&d_pool_, (n + kPoolPad) * sizeof(uint32_t)) == hipSuccess) break;
+    (void)hipGetLastError();
+    d_pool_ = nullptr;
+    n = n / 10 * 9;
+  }
+  if (!d_pool_) {
+    n = pool_words_;
+    HC(hipMalloc((void**)&d_pool_, (n + kPoolPad) * sizeof(uint32_t)));
+  }
+  dev_bytes_ += (n + kPoolPad) * sizeof(uint32_t);
+  const bool grew = n > pool_words_;
+  pool_words_ = n;
+  if (gexec_) {
+    HC(hipGraphExecDestroy(gexec_));
+    gexec_ = nullptr;
+    graph_sig_.clear();
+  }
+  return grew;
+}
+
 void Job::prepare(const std::vector<SimInstance>& inst, uint64_t pool_words) {
   HC(hipSetDevice(device_));
   if (inst.empty()) throw GlsError(GLS_E_ARG, "no instance to simulate");
@@ -407,46 +508,69 @@ void Job::prepare(const std::vector<SimInstance>& inst, uint64_t pool_words) {
   n_gates_ = prog_->n_gates;
   n_nets_ = (uint32_t)nets;
   level_start_ = prog_->level_start;
+  wmax_ = 1;
+  for (const GateRec& g : prog_->h_grec) wmax_ = std::max<uint32_t>(wmax_, g.dmax);
 
-  // initial waveforms: stimulus signals, then constants, per instance
-  std::vector<uint32_t> words;
-  h_woff_init_.assign(n_nets_, 0);
-  h_wlen_init_.assign(n_nets_, 0);
+  // stimulus signals, then constants, per instance
   saif_net_list_.clear();
   saif_net_job_.clear();
+  std::vector<uint32_t> words;    // two-pass: initial waveforms in the pool
+  std::vector<uint64_t> store;    // fused: resident 64-bit words
+  std::vector<uint64_t> soff;
+  h_slen_.clear();
+  h_snet_.clear();
+  std::vector<uint32_t> sjob;
+  h_woff_init_.clear();
+  h_wlen_init_.clear();
+  if (mode == 1) {
+    h_woff_init_.assign(n_nets_, 0);
+    h_wlen_init_.assign(n_nets_, 0);
+  }
+  t_last_ = 0;
+  auto add_signal = [&](uint32_t net, uint32_t job, const uint32_t* t, const uint8_t* v, size_t n) {
+    if (mode == 1) {
+      h_woff_init_[net] = words.size();
+      h_wlen_init_[net] = (uint32_t)n;
+      for (size_t q = 0; q < n; ++q) {
+        if (t[q] >= (1u << 30) - (1u << 17))
+          throw GlsError(GLS_E_RANGE, "the two-pass path keeps times below 2^30 - 2^17");
+        words.push_back((t[q] << 2) | (v[q] & 3u));
+      }
+    } else {
+      soff.push_back(store.size());
+      h_slen_.push_back((uint32_t)n);
+      h_snet_.push_back(net);
+      sjob.push_back(job);
+      for (size_t q = 0; q < n; ++q) store.push_back(((uint64_t)t[q] << 2) | (v[q] & 3u));
+      if (n) t_last_ = std::max<uint64_t>(t_last_, t[n - 1]);
+    }
+    saif_net_list_.push_back(net);
+    saif_net_job_.push_back(job);
+  };
   for (size_t i = 0; i < inst.size(); ++i) {
     const SimInstance& in = inst[i];
     const uint32_t nb = net_base_[i];
     for (size_t s = 0; s < in.st->waves.size(); ++s) {
-      const uint32_t net = nb + (uint32_t)in.stim_nets[s];
       const Wave& w = in.st->waves[s];
-      h_woff_init_[net] = words.size();
-      h_wlen_init_[net] = (uint32_t)w.size();
-      for (size_t q = 0; q < w.size(); ++q) {
-        if (w.t[q] >= (1u << 30) - (1u << 17))
-          throw GlsError(GLS_E_RANGE, "stimulus time beyond the 30-bit event word");
-        words.push_back((w.t[q] << 2) | (w.v[q] & 3u));
-      }
-      saif_net_list_.push_back(net);
-      saif_net_job_.push_back((uint32_t)i);
+      for (size_t q = 1; q < w.size(); ++q)
+        if (w.t[q] <= w.t[q - 1]) throw GlsError(GLS_E_FORMAT, "stimulus times must increase");
+      add_signal(nb + (uint32_t)in.stim_nets[s], (uint32_t)i, w.t.data(), w.v.data(), w.size());
     }
     const Program& pg = *in.pg;
     for (uint32_t n = 0; n < (uint32_t)pg.net_names.size(); ++n) {
       if (pg.role[n] != NR_CONST) continue;
-      h_woff_init_[nb + n] = words.size();
-      h_wlen_init_[nb + n] = 1;
-      words.push_back((0u << 2) | (uint32_t)(pg.const_val[n] & 3));
-      saif_net_list_.push_back(nb + n);
-      saif_net_job_.push_back((uint32_t)i);
+      const uint32_t t0 = 0;
+      const uint8_t v0 = (uint8_t)(pg.const_val[n] & 3);
+      add_signal(nb + n, (uint32_t)i, &t0, &v0, 1);
     }
   }
-  stim_words_ = words.size();
+  n_sig_ = (uint32_t)h_snet_.size();
+  size_t nlev = level_start_.size() - 1;
+  horizon_ = t_last_ + (uint64_t)(nlev + 1) * wmax_ + 2;
 
   d_woff_ = (uint64_t*)dmalloc(n_nets_ * sizeof(uint64_t));
   d_wlen_ = (uint32_t*)dmalloc(n_nets_ * sizeof(uint32_t));
   d_saif_ = (unsigned long long*)dmalloc((size_t)n_nets_ * kSaifWords * sizeof(unsigned long long));
-  d_saif_nets_ = (uint32_t*)dmalloc(saif_net_list_.size() * 4);
-  d_saif_job_ = (uint32_t*)dmalloc(saif_net_job_.size() * 4);
   d_win_ = (unsigned long long*)dmalloc(win_.size() * 8);
   d_scal_ = (uint64_t*)dmalloc(16 * sizeof(uint64_t));
   uint32_t maxw = 1;
@@ -454,9 +578,12 @@ void Job::prepare(const std::vector<SimInstance>& inst, uint64_t pool_words) {
     maxw = std::max(maxw, level_start_[l + 1] - level_start_[l]);
   for (uint64_t** p : {&d_outoff_, &d_ntile_, &d_bound_, &d_tilebase_, &d_scrbase_})
     *p = (uint64_t*)dmalloc(maxw * 8ull);
-  if (mode == 1)
+  if (mode == 1) {
     for (uint64_t** p : {&d_nseg_, &d_scr_, &d_segbase_, &d_outlen_})
       *p = (uint64_t*)dmalloc(maxw * 8ull);
+    d_saif_nets_ = (uint32_t*)dmalloc(saif_net_list_.size() * 4);
+    d_saif_job_ = (uint32_t*)dmalloc(saif_net_job_.size() * 4);
+  }
   d_part_ = (uint64_t*)dmalloc(scan_part_size(maxw) * 8ull);
   d_gnseg_ = (uint32_t*)dmalloc(maxw * 4ull);
   d_plan_ = (GatePlan*)dmalloc(maxw * sizeof(GatePlan));
@@ -464,10 +591,32 @@ void Job::prepare(const std::vector<SimInstance>& inst, uint64_t pool_words) {
   // level runs out of it (the level is run again)
   ovf_words_ = ovf_words_opt ? ovf_words_opt : (1ull << 24);
   ensure((void**)&d_ovf_, cap_ovf_, ovf_words_ * 4);
+  if (mode == 0) {
+    d_store_ = (uint64_t*)dmalloc(store.size() * 8);
+    d_soff_ = (uint64_t*)dmalloc(n_sig_ * 8ull);
+    d_slen_ = (uint32_t*)dmalloc(n_sig_ * 4ull);
+    d_snet_ = (uint32_t*)dmalloc(n_sig_ * 4ull);
+    d_sjob_ = (uint32_t*)dmalloc(n_sig_ * 4ull);
+    d_scnt_ = (uint64_t*)dmalloc(n_sig_ * 8ull);
+    d_sbase_ = (uint64_t*)dmalloc(n_sig_ * 8ull);
+    d_sfirst_ = (uint32_t*)dmalloc(n_sig_ * 4ull);
+    d_spart_ = (uint64_t*)dmalloc(scan_part_size(n_sig_) * 8ull);
+    d_tail_len_ = (uint32_t*)dmalloc(n_nets_ * 4ull);
+    d_tail_off_ = (uint64_t*)dmalloc(n_nets_ * 8ull);
+    for (uint64_t** p : {&d_tlen_g_, &d_tsrc_g_, &d_toff_g_}) *p = (uint64_t*)dmalloc(n_gates_ * 8ull);
+    d_gpart_ = (uint64_t*)dmalloc(scan_part_size(n_gates_) * 8ull);
+    d_saif_chunk_ = (unsigned long long*)dmalloc((size_t)n_nets_ * kSaifWords * 8);
+    d_wtot_ = (unsigned long long*)dmalloc(n_nets_ * 8ull);
+    ensure((void**)&d_tails_, cap_tails_, std::max<size_t>(n_gates_, 1) * 4);
+    ensure((void**)&d_cwin_, cap_cwin_, inst.size() * sizeof(ChunkWin));
+    stim_words_ = store.size() + n_sig_;  // one chunk: every entry and a state entry per signal
+  } else {
+    stim_words_ = words.size();
+  }
 
   // event pool: a caller-given size is fixed unless pool_grow is set; size 0
   // starts at four times the stimulus (at least 64 MiB); a growable pool is
-  // enlarged between levels as needed
+  // enlarged between levels and chunks as needed
   pool_auto_ = pool_words == 0 || pool_grow;
   if (pool_words == 0) {
     size_t free_b = 0, total_b = 0;
@@ -475,23 +624,30 @@ void Job::prepare(const std::vector<SimInstance>& inst, uint64_t pool_words) {
     pool_words = std::min<uint64_t>((uint64_t)(free_b * 0.85) / 4,
                                     std::max<uint64_t>(1ull << 24, 4 * stim_words_));
   }
-  if (pool_words < stim_words_ + 1024) pool_words = stim_words_ + 1024;
+  if (mode == 1 && pool_words < stim_words_ + 1024) pool_words = stim_words_ + 1024;
+  if (pool_words < 1024) pool_words = 1024;
   pool_words_ = pool_words;
   HC(hipMalloc((void**)&d_pool_, (pool_words_ + kPoolPad) * sizeof(uint32_t)));
   dev_bytes_ += (pool_words_ + kPoolPad) * sizeof(uint32_t);
 
-  if (!words.empty())
-    HC(hipMemcpyAsync(d_pool_, words.data(), words.size() * 4, hipMemcpyHostToDevice, stream_));
-  // device copies of the initial waveform tables (graph replays start from them)
-  d_woff_init_ = (uint64_t*)dmalloc(n_nets_ * sizeof(uint64_t));
-  d_wlen_init_ = (uint32_t*)dmalloc(n_nets_ * sizeof(uint32_t));
-  HC(hipMemcpyAsync(d_woff_init_, h_woff_init_.data(), n_nets_ * 8ull, hipMemcpyHostToDevice, stream_));
-  HC(hipMemcpyAsync(d_wlen_init_, h_wlen_init_.data(), n_nets_ * 4ull, hipMemcpyHostToDevice, stream_));
-  if (!saif_net_list_.empty()) {
-    HC(hipMemcpyAsync(d_saif_nets_, saif_net_list_.data(), saif_net_list_.size() * 4,
-                      hipMemcpyHostToDevice, stream_));
-    HC(hipMemcpyAsync(d_saif_job_, saif_net_job_.data(), saif_net_job_.size() * 4,
-                      hipMemcpyHostToDevice, stream_));
+  if (mode == 1) {
+    if (!words.empty())
+      HC(hipMemcpyAsync(d_pool_, words.data(), words.size() * 4, hipMemcpyHostToDevice, stream_));
+    if (!saif_net_list_.empty()) {
+      HC(hipMemcpyAsync(d_saif_nets_, saif_net_list_.data(), saif_net_list_.size() * 4,
+                        hipMemcpyHostToDevice, stream_));
+      HC(hipMemcpyAsync(d_saif_job_, saif_net_job_.data(), saif_net_job_.size() * 4,
+                        hipMemcpyHostToDevice, stream_));
+    }
+  } else {
+    if (!store.empty())
+      HC(hipMemcpyAsync(d_store_, store.data(), store.size() * 8, hipMemcpyHostToDevice, stream_));
+    if (n_sig_) {
+      HC(hipMemcpyAsync(d_soff_, soff.data(), n_sig_ * 8ull, hipMemcpyHostToDevice, stream_));
+      HC(hipMemcpyAsync(d_slen_, h_slen_.data(), n_sig_ * 4ull, hipMemcpyHostToDevice, stream_));
+      HC(hipMemcpyAsync(d_snet_, h_snet_.data(), n_sig_ * 4ull, hipMemcpyHostToDevice, stream_));
+      HC(hipMemcpyAsync(d_sjob_, sjob.data(), n_sig_ * 4ull, hipMemcpyHostToDevice, stream_));
+    }
   }
   HC(hipMemcpyAsync(d_win_, win_.data(), win_.size() * 8, hipMemcpyHostToDevice, stream_));
   HC(hipStreamSynchronize(stream_));
@@ -504,10 +660,67 @@ void Job::prepare(const std::vector<SimInstance>& inst, uint64_t pool_words) {
   stats_.n_pins = prog_->n_pins;
 }
 
-// Fused path, per level: plan the tiles (k_plan2, two scans), describe them
-// (k_tile_desc), walk every tile independently into the level scratch
-// (k_level), join the tiles of each gate and finish its SAIF (k_gate_carry),
-// place the gates in the pool (scan) and copy the outputs there
+void Job::stop_point() {
+  if (should_stop && should_stop()) throw GlsError(GLS_E_CANCELLED, "job cancelled or past its deadline");
+}
+
+// Chunk k covering [F, Fn): its look-back start, its relative-time bounds
+// and the move of the previous chunk's tails (made in the previous chunk's
+// time) to this chunk's time.
+Job::Chunk Job::make_chunk(size_t k, uint64_t F, uint64_t Fn, uint64_t prev_lb) const {
+  Chunk c;
+  c.F = F;
+  c.Fn = Fn;
+  c.lb = k == 0 ? 0 : F - wmax_;
+  c.first = k == 0 ? 1u : 0u;
+  c.lo_rel = (uint32_t)(F - c.lb + 1);
+  c.hi_rel = Fn == kChunkEnd ? 0xFFFFFFFFu : (uint32_t)(Fn - c.lb + 1);
+  c.shift = k == 0 ? 0u : (uint32_t)(c.lb - prev_lb);
+  c.slot = 0;
+  return c;
+}
+
+// The chunk's SAIF bounds per instance, in its relative time, into ChunkWin
+// slot c.slot.
+void Job::put_chunk_windows(const Chunk& c, bool capture) {
+  const size_t n = net_base_.size();
+  const uint64_t kMax = 1ull << 62;
+  auto rel = [&](uint64_t x) -> uint64_t {  // x >= lb - 1
+    const uint64_t r = x - c.lb + 1;
+    return (x - c.lb >= kMax) ? kMax : r;
+  };
+  std::vector<ChunkWin> w(n);
+  for (size_t i = 0; i < n; ++i) {
+    const uint64_t s = win_[2 * i], e = win_[2 * i + 1];
+    const uint64_t lo = std::max(s, c.F), hi = std::min(e, c.Fn);
+    ChunkWin& x = w[i];
+    if (hi <= lo) {
+      x.lo = x.hi = 0;
+    } else {
+      x.lo = rel(lo);
+      x.hi = rel(hi);
+    }
+    // toggles at t in (s, e]: t_rel > rel(s) and t_rel <= rel(e), where
+    // every entry's t_rel is >= 1 (a bound before lb - 1 counts as 0)
+    x.tc_lo = s < c.lb ? 0 : rel(s);
+    x.tc_hi = e < c.lb ? 0 : rel(e);
+  }
+  const size_t need = (c.slot + 1) * n * sizeof(ChunkWin);
+  if (capture) {
+    if (need > cap_cwin_) throw GlsError(GLS_E_STATE, "chunk windows not sized for the capture");
+  } else {
+    ensure((void**)&d_cwin_, cap_cwin_, need);
+  }
+  HC(hipMemcpyAsync(d_cwin_ + c.slot * n, w.data(), n * sizeof(ChunkWin), hipMemcpyHostToDevice,
+                    stream_));
+  HC(hipStreamSynchronize(stream_));
+}
+
+// One chunk: its stimulus cut into the pool, then every level.  Per level:
+// plan the tiles (k_plan2, two scans), describe them (k_tile_desc), walk
+// every tile independently into the level scratch (k_level), join the tiles
+// of each gate and finish its SAIF share (k_gate_carry), place the gates in
+// the pool behind their tails (scan) and copy the outputs there
 // (k_compact).  Plain: one host round trip per level sizes the tile arrays
 // and the scratch.  Device-driven (graph capture): no host round trip; the
 // arrays keep the capacities learned by an earlier plain run, kernels read
@@ -517,19 +730,46 @@ void Job::prepare(const std::vector<SimInstance>& inst, uint64_t pool_words) {
 // Plain runs with an automatically sized overflow stack buffer check, at
 // the next host round trip, whether the previous level ran out of it; if so
 // the buffer doubles and that level runs again from its starting state
-// (bump pointer and event count; its outputs and SAIF counters are
-// rewritten).
-void Job::stop_point() {
-  if (should_stop && should_stop()) throw GlsError(GLS_E_CANCELLED, "job cancelled or past its deadline");
-}
-
-void Job::run_fused(std::vector<hipEvent_t>& ev_eval, bool device_driven) {
+// (bump pointer and event count; its outputs and SAIF shares are
+// rewritten).  A plain chunk that cannot fit the pool (or, with automatic
+// chunks, is projected not to) returns CHUNK_SHORTER before its results
+// are used; *grow_hint then suggests a length factor (0: none).
+int Job::run_chunk(const Chunk& c, std::vector<hipEvent_t>& ev_eval, bool device_driven,
+                   double* grow_hint) {
   uint64_t* bump = d_scal_ + S_BUMP;
   unsigned long long* evc = (unsigned long long*)(d_scal_ + S_EVC);
+  unsigned int* err = (unsigned int*)(d_scal_ + S_ERR);
   const uint32_t seg = seg_target ? seg_target : kDefaultSeg;
   const bool retry = !device_driven && (ovf_words_opt == 0 || pool_grow);
   const size_t nlev = level_start_.size() - 1;
-  size_t prev = SIZE_MAX;          // last level run
+  const ChunkWin* cw = d_cwin_ + c.slot * net_base_.size();
+  TailArgs T;
+  T.tail_len = d_tail_len_;
+  T.tail_off = d_tail_off_;
+  T.tails = d_tails_;
+  T.shift = c.shift;
+  T.first = c.first;
+  *grow_hint = 0;
+
+  // the chunk's stimulus at the start of the pool
+  HC(launch_stim_count(d_store_, d_soff_, d_slen_, n_sig_, c.lb, c.Fn == kChunkEnd ? ~0ull : c.Fn,
+                       d_scnt_, d_sfirst_, stream_));
+  if (n_sig_) HC(launch_scan(d_scnt_, d_sbase_, n_sig_, d_spart_, bump, nullptr, stream_));
+  else HC(launch_fill_u64((unsigned long long*)bump, 1, 0ull, stream_));
+  uint64_t stim_chunk = 0;
+  if (!device_driven) {
+    HC(hipMemcpyAsync(&stim_chunk, bump, 8, hipMemcpyDeviceToHost, stream_));
+    HC(hipStreamSynchronize(stream_));
+    if (stim_chunk + 1024 > pool_words_) {
+      // nothing in the pool to keep yet
+      if (pool_auto_) refresh_pool(std::max<uint64_t>(2 * stim_chunk, 2 * pool_words_));
+      if (stim_chunk + 1024 > pool_words_) return CHUNK_SHORTER;
+    }
+  }
+  HC(launch_stim_copy(d_store_, d_soff_, d_snet_, n_sig_, c.lb, d_scnt_, d_sfirst_, d_sbase_, d_pool_,
+                      pool_words_, d_woff_, d_wlen_, err, stream_));
+
+  size_t prev = SIZE_MAX;  // last level run
   uint64_t prev_bump = 0, prev_evc = 0, prev_err = 0;
   // true when the previous level overflowed and was rolled back
   auto rolled_back = [&](const uint64_t* h) {
@@ -550,6 +790,7 @@ void Job::run_fused(std::vector<hipEvent_t>& ev_eval, bool device_driven) {
     }
     return true;
   };
+  const bool projected = !device_driven && chunk_ps == 0;
   size_t next = 0;  // level to run next
   while (next <= nlev) {
     const size_t l = next++;
@@ -567,8 +808,7 @@ void Job::run_fused(std::vector<hipEvent_t>& ev_eval, bool device_driven) {
     uint32_t g0 = level_start_[l], ng = level_start_[l + 1] - g0;
     if (ng == 0) continue;
     const GateRec* gates = prog_->d_grec + g0;
-    HC(launch_plan2(gates, ng, d_wlen_, seg, d_plan_, d_ntile_, d_bound_,
-                    (unsigned int*)(d_scal_ + S_ERR), stream_));
+    HC(launch_plan2(gates, ng, d_wlen_, seg, d_plan_, d_ntile_, d_bound_, err, d_tail_len_, stream_));
     HC(launch_scan(d_ntile_, d_tilebase_, ng, d_part_, d_scal_ + S_TOT_A, nullptr, stream_));
     HC(launch_scan(d_bound_, d_scrbase_, ng, d_part_, d_scal_ + S_TOT_B, nullptr, stream_));
     uint64_t ntiles = tiles_cap_;
@@ -594,15 +834,28 @@ void Job::run_fused(std::vector<hipEvent_t>& ev_eval, bool device_driven) {
       ntiles = h[S_TOT_A];
       const uint64_t scr = h[S_TOT_B], used = h[S_BUMP];
       if (ntiles >= 0xFFFFFFF0ull) throw GlsError(GLS_E_RANGE, "too many tiles in one level");
-      ensure((void**)&d_tiles_, cap_tiles_, ntiles * sizeof(TileDesc));
-      ensure((void**)&d_tout_, cap_tout_, ntiles * sizeof(TileOut));
-      ensure((void**)&d_toff_, cap_toff_, ntiles * 4);
+      // automatic chunks: a chunk projected (by its gates so far) not to fit
+      // the largest pool is cut short before it runs further
+      if (projected && g0 >= std::max<uint32_t>(64, n_gates_ / 32) && used > stim_chunk) {
+        const double proj = (double)(used - stim_chunk) * n_gates_ / g0;
+        const double cap = (double)(pool_auto_ ? pool_max_words() : pool_words_);
+        if (((double)stim_chunk + proj) * 1.35 > cap) {
+          *grow_hint = std::max(0.05, 0.85 * (cap / 1.35 - stim_chunk) / proj);
+          return CHUNK_SHORTER;
+        }
+      }
+      if (!try_ensure((void**)&d_tiles_, cap_tiles_, ntiles * sizeof(TileDesc)) ||
+          !try_ensure((void**)&d_tout_, cap_tout_, ntiles * sizeof(TileOut)) ||
+          !try_ensure((void**)&d_toff_, cap_toff_, ntiles * 4))
+        return CHUNK_SHORTER;
       grow_pool(used + 2 * scr, used);
       if (used <= pool_words_ && 2 * scr <= pool_words_ - used) {
         pool_limit = pool_words_ - scr;
         scratch = d_pool_ + pool_limit;
       } else {
-        ensure((void**)&d_scratch_, cap_scratch_, scr * 4);
+        // outputs in the pool, scratch apart
+        if (used + scr > pool_words_ && !grow_pool(used + scr, used)) return CHUNK_SHORTER;
+        if (!try_ensure((void**)&d_scratch_, cap_scratch_, scr * 4)) return CHUNK_SHORTER;
         scratch = d_scratch_;
       }
       max_tiles_ = std::max(max_tiles_, ntiles);
@@ -630,17 +883,20 @@ void Job::run_fused(std::vector<hipEvent_t>& ev_eval, bool device_driven) {
     a.ovf_cap = ovf_words_;
     a.ovf_used = (unsigned long long*)(d_scal_ + S_OVF_USED);
     a.ev_count = evc;
-    a.err = (unsigned int*)(d_scal_ + S_ERR);
-    a.win = d_win_;
+    a.err = err;
+    a.cwin = cw;
     a.cap = cap_;
     a.prefetch = prefetch;
     a.ticket = (unsigned long long*)(d_scal_ + S_TICKET);
     a.tiles_walked = (unsigned long long*)(d_scal_ + S_TILES);
     a.stk = nullptr;
+    a.lo_rel = c.lo_rel;
+    a.hi_rel = c.hi_rel;
+    a.first = c.first;
     const unsigned nb = (unsigned)std::min<uint64_t>(ntiles, (uint64_t)std::max(1, level_blocks_));
     if (stack_mode == 1) {
-      const uint64_t words = (uint64_t)std::max(1, level_blocks_) * (cap_ + 64);
-      if (!device_driven) ensure((void**)&d_stk_, cap_stk_, words * 4);
+      const uint64_t w = (uint64_t)std::max(1, level_blocks_) * (cap_ + 64);
+      if (!device_driven) ensure((void**)&d_stk_, cap_stk_, w * 4);
       a.stk = d_stk_;
     }
     if (!device_driven) {
@@ -657,14 +913,146 @@ void Job::run_fused(std::vector<hipEvent_t>& ev_eval, bool device_driven) {
     }
     // gate counts go to d_bound_ (free again once the scratch bases are scanned)
     HC(launch_gate_carry(gates, ng, d_tilebase_, d_ntile_, d_tout_, d_toff_, d_bound_, d_wlen_,
-                         d_saif_, d_win_, (const unsigned int*)(d_scal_ + S_ERR), stream_));
+                         d_saif_chunk_, cw, err, T, stream_));
     HC(launch_scan(d_bound_, d_outoff_, ng, d_part_, d_scal_ + S_TOT_OUT, bump, stream_));
     HC(launch_pool_guard(d_scal_, pool_limit, stream_));
     HC(launch_compact(d_tiles_, d_scal_ + S_TOT_A, ntiles, d_tout_, d_toff_, d_scrbase_, scratch,
-                      d_outoff_, d_pool_, d_woff_, d_wlen_, (const unsigned int*)(d_scal_ + S_ERR),
-                      d_saif_, d_win_, saif_tc, stream_));
+                      d_outoff_, d_pool_, d_woff_, d_wlen_, err, d_saif_chunk_, cw, saif_tc, T,
+                      stream_));
     HC(launch_bump(bump, d_scal_ + S_TOT_OUT, stream_));
   }
+  return CHUNK_OK;
+}
+
+// Gate-output waveforms of a finished chunk to the host: each gate net's
+// entries after its tail, in absolute time.
+void Job::fetch_chunk_waves(const Chunk& c) {
+  std::vector<uint64_t> off(n_nets_);
+  std::vector<uint32_t> len(n_nets_), tl(n_nets_);
+  uint64_t used = 0;
+  HC(hipMemcpyAsync(off.data(), d_woff_, n_nets_ * 8ull, hipMemcpyDeviceToHost, stream_));
+  HC(hipMemcpyAsync(len.data(), d_wlen_, n_nets_ * 4ull, hipMemcpyDeviceToHost, stream_));
+  HC(hipMemcpyAsync(tl.data(), d_tail_len_, n_nets_ * 4ull, hipMemcpyDeviceToHost, stream_));
+  HC(hipMemcpyAsync(&used, d_scal_ + S_BUMP, 8, hipMemcpyDeviceToHost, stream_));
+  HC(hipStreamSynchronize(stream_));
+  std::vector<uint32_t> pool(used);
+  if (used) HC(hipMemcpyAsync(pool.data(), d_pool_, used * 4, hipMemcpyDeviceToHost, stream_));
+  HC(hipStreamSynchronize(stream_));
+  if (waves_.size() != n_nets_) waves_.assign(n_nets_, {});
+  for (const GateRec& g : prog_->h_grec) {
+    const uint32_t n = g.out_net;
+    if (off[n] + len[n] > used) throw GlsError(GLS_E_STATE, "waveform outside the event pool");
+    for (uint32_t q = tl[n]; q < len[n]; ++q) {
+      const uint32_t w = pool[off[n] + q];
+      const uint64_t t = (uint64_t)(w >> 2) + c.lb - 1;  // rel >= 1
+      waves_[n].push_back((t << 2) | (w & 3u));
+    }
+  }
+}
+
+// After a chunk: its SAIF shares join the totals, its waveforms go to the
+// host (keep_waves, plain runs), and the gate outputs' tails for the next
+// chunk are cut from their waveforms.
+void Job::end_chunk(const Chunk& c, bool device_driven) {
+  HC(launch_saif_accum(prog_->d_grec, n_gates_, d_saif_chunk_, d_saif_, d_wlen_, d_tail_len_, d_wtot_,
+                       stream_));
+  if (!device_driven && keep_waves) fetch_chunk_waves(c);
+  if (c.Fn == kChunkEnd || n_gates_ == 0) return;
+  const uint32_t key = (uint32_t)((c.Fn - wmax_) - c.lb + 1);  // lb of the next chunk
+  HC(launch_tail_len(prog_->d_grec, n_gates_, d_woff_, d_wlen_, d_pool_, key, d_tlen_g_, d_tsrc_g_,
+                     stream_));
+  HC(launch_scan(d_tlen_g_, d_toff_g_, n_gates_, d_gpart_, d_scal_ + S_TAILS, nullptr, stream_));
+  if (!device_driven) {
+    uint64_t total = 0;
+    HC(hipMemcpyAsync(&total, d_scal_ + S_TAILS, 8, hipMemcpyDeviceToHost, stream_));
+    HC(hipStreamSynchronize(stream_));
+    ensure((void**)&d_tails_, cap_tails_, std::max<uint64_t>(total, 1) * 4);
+    max_tails_ = std::max(max_tails_, total);
+  }
+  HC(launch_tail_copy(prog_->d_grec, n_gates_, d_tlen_g_, d_tsrc_g_, d_toff_g_, d_pool_, d_tails_,
+                      cap_tails_ / 4, d_tail_len_, d_tail_off_, (unsigned int*)(d_scal_ + S_ERR),
+                      stream_));
+}
+
+// A plain run of every chunk.  Automatic chunks: the first chunk is the
+// whole window; a chunk projected not to fit the largest pool (or that does
+// not fit) is run again shorter, the pool taking the largest size free HBM
+// allows, and each next chunk aims at half of that pool by the output rate
+// of the one before.
+void Job::run_chunks(std::vector<hipEvent_t>& ev_eval) {
+  HC(launch_set_scal(d_scal_, 0, stream_));
+  HC(launch_fill_u64(d_saif_, (uint64_t)n_nets_ * kSaifWords, 0ull, stream_));
+  HC(launch_fill_u64(d_wtot_, n_nets_, 0ull, stream_));
+  HC(launch_saif_stim(d_store_, d_soff_, d_slen_, d_snet_, d_sjob_, n_sig_, d_saif_, d_win_, stream_));
+  ensure((void**)&d_tails_, cap_tails_, std::max<size_t>(n_gates_, 1) * 4);
+  HC(launch_tails_init(prog_->d_grec, n_gates_, d_tail_len_, d_tail_off_, d_tails_, stream_));
+  waves_.clear();
+  if (keep_waves) waves_.assign(n_nets_, {});
+  sched_.clear();
+  stats_.chunk_retries = 0;
+  max_bump_ = 0;
+  max_tails_ = n_gates_;
+  const uint64_t lmax = kRelSpan - wmax_ - 8;  // relative times of a chunk stay in the event word
+  const uint64_t lmin = wmax_ + 1;             // the look-back of chunk 1 starts after t = 0
+  uint64_t L;
+  if (chunk_ps == kNoChunks) {
+    if (horizon_ >= lmax) throw GlsError(GLS_E_RANGE, "times reach 2^30 - 2^18: the run needs time chunks");
+    L = horizon_;
+  } else if (chunk_ps) {
+    if (chunk_ps < lmin)
+      throw GlsError(GLS_E_ARG, "chunk length must exceed the largest gate delay");
+    L = std::min<uint64_t>(chunk_ps, lmax);
+  } else {
+    L = std::min<uint64_t>(horizon_, lmax);
+  }
+  // a chunk that did not fit caps every later chunk at the length it was
+  // run again with: chunks after an empty stretch cannot grow back into a
+  // dense one, and the number of retries is bounded (each at least halves)
+  uint64_t l_cap = lmax;
+  uint64_t F = 0, prev_lb = 0;
+  size_t k = 0;
+  for (;;) {
+    const uint64_t Fn = (F + L >= horizon_) ? kChunkEnd : F + L;
+    const Chunk c = make_chunk(k, F, Fn, prev_lb);
+    uint64_t snap[16];
+    HC(hipMemcpyAsync(snap, d_scal_, sizeof snap, hipMemcpyDeviceToHost, stream_));
+    HC(hipStreamSynchronize(stream_));
+    put_chunk_windows(c, false);
+    double hint = 0;
+    if (run_chunk(c, ev_eval, false, &hint) != CHUNK_OK) {
+      if (chunk_ps != 0)
+        throw GlsError(GLS_E_OOM, "event pool exhausted (raise the pool size or allow time chunks)");
+      HC(hipStreamSynchronize(stream_));
+      HC(hipMemcpyAsync(d_scal_, snap, sizeof snap, hipMemcpyHostToDevice, stream_));
+      uint64_t nl = hint > 0 ? (uint64_t)((double)L * std::min(0.5, hint)) : L / 2;
+      if (L <= lmin) throw GlsError(GLS_E_OOM, "event pool exhausted by the shortest time chunk");
+      L = std::max<uint64_t>(nl, lmin);
+      l_cap = std::min(l_cap, L);
+      ++stats_.chunk_retries;
+      refresh_pool(pool_max_words());  // the chunk starts over: nothing to keep
+      continue;
+    }
+    uint64_t used = 0;
+    HC(hipMemcpyAsync(&used, d_scal_ + S_BUMP, 8, hipMemcpyDeviceToHost, stream_));
+    HC(hipStreamSynchronize(stream_));
+    max_bump_ = std::max(max_bump_, used);
+    end_chunk(c, false);
+    sched_.push_back(Fn);
+    if (Fn == kChunkEnd) break;
+    if (chunk_ps == 0) {
+      // next length: half of the largest pool at this chunk's output rate
+      const double target = 0.5 * (double)(pool_auto_ ? pool_max_words() : pool_words_);
+      const double rate = (double)std::max<uint64_t>(used, 1) / (double)L;
+      const double want = target / rate;
+      L = (uint64_t)std::min<double>(std::max<double>(want, (double)L / 4), (double)L * 4);
+      L = std::max<uint64_t>(std::min<uint64_t>(L, l_cap), lmin);
+      if (pool_auto_ && used > pool_words_ / 2) refresh_pool(pool_max_words());
+    }
+    prev_lb = c.lb;
+    F = Fn;
+    ++k;
+  }
+  stats_.chunks = sched_.size();
 }
 
 void Job::run_two_pass(std::vector<hipEvent_t>& ev_eval) {
@@ -716,10 +1104,10 @@ void Job::run() {
   level_blocks_ = level_blocks_opt ? (int)level_blocks_opt : level_max_blocks(cap_, stack_mode == 0);
   stats_.level_blocks = (uint64_t)level_blocks_;
   const std::vector<uint64_t> sig = {cap_, (uint64_t)level_blocks_, seg_target, prefetch,
-                                     (uint64_t)mode, stack_mode};
+                                     (uint64_t)mode, stack_mode, chunk_ps, keep_waves};
   last_replay_ = false;
   if (graph && gexec_ && sig == graph_sig_) {
-    // replay the captured level sequence: no host round trip between levels
+    // replay the captured chunk and level sequence: no host round trip
     auto w0 = std::chrono::steady_clock::now();
     hipEvent_t e0, e1;
     HC(hipEventCreate(&e0));
@@ -741,6 +1129,10 @@ void Job::run() {
     stats_.eval_launches = 1;
     last_replay_ = true;
     finish_stats(h, ms);
+    if (keep_waves) {  // one chunk (checked at capture): the pool holds every waveform
+      waves_.assign(n_nets_, {});
+      fetch_chunk_waves(make_chunk(0, 0, kChunkEnd, 0));
+    }
     stats_.wall_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - w0).count();
     return;
   }
@@ -748,21 +1140,25 @@ void Job::run() {
   hipEvent_t e0, e1;
   HC(hipEventCreate(&e0));
   HC(hipEventCreate(&e1));
-  uint64_t scal[16] = {0};
-  scal[S_BUMP] = stim_words_;
-  HC(hipMemcpyAsync(d_woff_, h_woff_init_.data(), n_nets_ * 8ull, hipMemcpyHostToDevice, stream_));
-  HC(hipMemcpyAsync(d_wlen_, h_wlen_init_.data(), n_nets_ * 4ull, hipMemcpyHostToDevice, stream_));
-  HC(hipMemcpyAsync(d_scal_, scal, sizeof scal, hipMemcpyHostToDevice, stream_));
-  HC(launch_fill_u64(d_saif_, (uint64_t)n_nets_ * kSaifWords, 0ull, stream_));
-  HC(hipStreamSynchronize(stream_));
-
+  if (mode == 1) {
+    uint64_t scal[16] = {0};
+    scal[S_BUMP] = stim_words_;
+    HC(hipMemcpyAsync(d_woff_, h_woff_init_.data(), n_nets_ * 8ull, hipMemcpyHostToDevice, stream_));
+    HC(hipMemcpyAsync(d_wlen_, h_wlen_init_.data(), n_nets_ * 4ull, hipMemcpyHostToDevice, stream_));
+    HC(hipMemcpyAsync(d_scal_, scal, sizeof scal, hipMemcpyHostToDevice, stream_));
+    HC(launch_fill_u64(d_saif_, (uint64_t)n_nets_ * kSaifWords, 0ull, stream_));
+    HC(hipStreamSynchronize(stream_));
+  }
   HC(hipEventRecord(e0, stream_));
-  HC(launch_saif_nets(d_saif_nets_, d_saif_job_, (uint32_t)saif_net_list_.size(), d_woff_, d_wlen_,
-                      d_pool_, d_saif_, d_win_, stream_));
   std::vector<hipEvent_t> ev_eval;
   try {
-    if (mode == 1) run_two_pass(ev_eval);
-    else run_fused(ev_eval);
+    if (mode == 1) {
+      HC(launch_saif_nets(d_saif_nets_, d_saif_job_, (uint32_t)saif_net_list_.size(), d_woff_,
+                          d_wlen_, d_pool_, d_saif_, d_win_, stream_));
+      run_two_pass(ev_eval);
+    } else {
+      run_chunks(ev_eval);
+    }
   } catch (...) {
     (void)hipStreamSynchronize(stream_);
     for (hipEvent_t e : ev_eval) (void)hipEventDestroy(e);
@@ -789,6 +1185,22 @@ void Job::run() {
   HC(hipMemcpyAsync(h, d_scal_, sizeof h, hipMemcpyDeviceToHost, stream_));
   HC(hipStreamSynchronize(stream_));
   finish_stats(h, ms);
+  if (mode == 1 && keep_waves) {
+    // two-pass: every waveform sits in the pool in absolute time
+    std::vector<uint64_t> off(n_nets_);
+    std::vector<uint32_t> len(n_nets_), pool(h[S_BUMP]);
+    HC(hipMemcpyAsync(off.data(), d_woff_, n_nets_ * 8ull, hipMemcpyDeviceToHost, stream_));
+    HC(hipMemcpyAsync(len.data(), d_wlen_, n_nets_ * 4ull, hipMemcpyDeviceToHost, stream_));
+    if (!pool.empty())
+      HC(hipMemcpyAsync(pool.data(), d_pool_, pool.size() * 4, hipMemcpyDeviceToHost, stream_));
+    HC(hipStreamSynchronize(stream_));
+    waves_.assign(n_nets_, {});
+    for (const GateRec& g : prog_->h_grec)
+      for (uint32_t q = 0; q < len[g.out_net]; ++q) {
+        const uint32_t w = pool[off[g.out_net] + q];
+        waves_[g.out_net].push_back(((uint64_t)(w >> 2) << 2) | (w & 3u));
+      }
+  }
   stats_.wall_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - w0).count();
   if (graph && mode == 0 && sig != graph_sig_) {
     build_graph();
@@ -796,33 +1208,53 @@ void Job::run() {
   }
 }
 
-// Capture one device-driven execution of every level into a graph.
+// Capture one device-driven execution of every chunk and level into a graph
+// (the chunk sequence of the plain run that precedes it).
 void Job::build_graph() {
   if (gexec_) {
     HC(hipGraphExecDestroy(gexec_));
     gexec_ = nullptr;
   }
+  if (keep_waves && sched_.size() > 1)
+    throw GlsError(GLS_E_UNSUPPORTED, "graph replay keeps waveforms of one-chunk runs only");
   tiles_cap_ = std::max<uint64_t>(1, max_tiles_);
   scratch_cap_ = std::max<uint64_t>(1, max_scratch_);
   ensure((void**)&d_tiles_, cap_tiles_, tiles_cap_ * sizeof(TileDesc));
   ensure((void**)&d_tout_, cap_tout_, tiles_cap_ * sizeof(TileOut));
   ensure((void**)&d_toff_, cap_toff_, tiles_cap_ * 4);
-  // the replay's bump sequence is the eager run's: a scratch at the pool top
-  // stays clear of every level's outputs when it fits above the final bump
-  const uint64_t used = stats_.pool_words_used;
-  graph_scratch_in_pool_ = used <= pool_words_ && scratch_cap_ <= pool_words_ - used;
+  ensure((void**)&d_tails_, cap_tails_, std::max<uint64_t>(max_tails_, 1) * 4);
+  // the replay's bump sequence is the plain run's: a scratch at the pool top
+  // stays clear of every level's outputs when it fits above the largest bump
+  graph_scratch_in_pool_ = max_bump_ <= pool_words_ && scratch_cap_ <= pool_words_ - max_bump_;
   if (!graph_scratch_in_pool_) ensure((void**)&d_scratch_, cap_scratch_, scratch_cap_ * 4);
+  // every chunk's windows in a slot of its own
+  const size_t K = sched_.size();
+  ensure((void**)&d_cwin_, cap_cwin_, std::max<size_t>(K, 1) * net_base_.size() * sizeof(ChunkWin));
+  std::vector<Chunk> chunks;
+  uint64_t F = 0, prev_lb = 0;
+  for (size_t k = 0; k < K; ++k) {
+    Chunk c = make_chunk(k, F, sched_[k], prev_lb);
+    c.slot = k;
+    put_chunk_windows(c, true);
+    chunks.push_back(c);
+    prev_lb = c.lb;
+    F = sched_[k];
+  }
   hipGraph_t g = nullptr;
   HC(hipStreamBeginCapture(stream_, hipStreamCaptureModeRelaxed));
   try {
-    HC(hipMemcpyAsync(d_woff_, d_woff_init_, n_nets_ * 8ull, hipMemcpyDeviceToDevice, stream_));
-    HC(hipMemcpyAsync(d_wlen_, d_wlen_init_, n_nets_ * 4ull, hipMemcpyDeviceToDevice, stream_));
-    HC(launch_set_scal(d_scal_, stim_words_, stream_));
+    HC(launch_set_scal(d_scal_, 0, stream_));
     HC(launch_fill_u64(d_saif_, (uint64_t)n_nets_ * kSaifWords, 0ull, stream_));
-    HC(launch_saif_nets(d_saif_nets_, d_saif_job_, (uint32_t)saif_net_list_.size(), d_woff_,
-                        d_wlen_, d_pool_, d_saif_, d_win_, stream_));
+    HC(launch_fill_u64(d_wtot_, n_nets_, 0ull, stream_));
+    HC(launch_saif_stim(d_store_, d_soff_, d_slen_, d_snet_, d_sjob_, n_sig_, d_saif_, d_win_,
+                        stream_));
+    HC(launch_tails_init(prog_->d_grec, n_gates_, d_tail_len_, d_tail_off_, d_tails_, stream_));
     std::vector<hipEvent_t> none;
-    run_fused(none, true);
+    for (const Chunk& c : chunks) {
+      double hint = 0;
+      run_chunk(c, none, true, &hint);
+      end_chunk(c, true);
+    }
   } catch (...) {
     (void)hipStreamEndCapture(stream_, &g);
     if (g) (void)hipGraphDestroy(g);
@@ -845,19 +1277,29 @@ void Job::finish_stats(const uint64_t* h, double ms) {
     throw GlsError(GLS_E_STATE, "a gate's input waveform was out of time order; results discarded");
   stats_.kernel_ms = ms;
   stats_.n_events = h[S_EVC];
-  stats_.pool_words_used = h[S_BUMP];
+  stats_.pool_words_used = mode == 1 ? h[S_BUMP] : max_bump_;
   stats_.tiles = h[S_TILES];
   stats_.tiles_hbm = h[S_TILES_HBM];
   stats_.pool_words = pool_words_;
+  stats_.chunks = mode == 1 ? 1 : sched_.size();
   // algorithmic bytes: every input waveform read once per reading gate, every
-  // output written once, 3 u64 SAIF counters per net, CSR metadata
-  std::vector<uint32_t> wlen(n_nets_);
-  HC(hipMemcpyAsync(wlen.data(), d_wlen_, n_nets_ * 4ull, hipMemcpyDeviceToHost, stream_));
-  HC(hipStreamSynchronize(stream_));
+  // output written once (whole-run waveform lengths), 3 u64 SAIF counters per
+  // net, CSR metadata
+  std::vector<uint64_t> W(n_nets_, 0);
+  if (mode == 1) {
+    std::vector<uint32_t> wlen(n_nets_);
+    HC(hipMemcpyAsync(wlen.data(), d_wlen_, n_nets_ * 4ull, hipMemcpyDeviceToHost, stream_));
+    HC(hipStreamSynchronize(stream_));
+    for (uint32_t n = 0; n < n_nets_; ++n) W[n] = wlen[n];
+  } else {
+    HC(hipMemcpyAsync(W.data(), d_wtot_, n_nets_ * 8ull, hipMemcpyDeviceToHost, stream_));
+    HC(hipStreamSynchronize(stream_));
+    for (uint32_t s = 0; s < n_sig_; ++s) W[h_snet_[s]] = h_slen_[s];
+  }
   uint64_t in_w = 0, out_w = 0;
   for (const GateRec& g : prog_->h_grec) {
-    for (int j = 0; j < g.k; ++j) in_w += wlen[g.in_net[j]];
-    out_w += wlen[g.out_net];
+    for (int j = 0; j < g.k; ++j) in_w += W[g.in_net[j]];
+    out_w += W[g.out_net];
   }
   if (prog_->h_grec.size() != n_gates_) in_w = out_w = 0;
   stats_.in_words = in_w;
@@ -870,25 +1312,6 @@ void Job::fetch_saif(std::vector<uint64_t>& v) const {
   hip_check(hipMemcpyAsync(v.data(), d_saif_, v.size() * 8, hipMemcpyDeviceToHost, stream_),
             "saif D2H");
   hip_check(hipStreamSynchronize(stream_), "saif D2H");
-}
-
-void Job::fetch_waves(std::vector<uint64_t>& off, std::vector<uint32_t>& len,
-                      std::vector<uint32_t>& pool) const {
-  off.resize(n_nets_);
-  len.resize(n_nets_);
-  uint64_t used = 0;
-  hip_check(hipMemcpyAsync(off.data(), d_woff_, n_nets_ * 8ull, hipMemcpyDeviceToHost, stream_),
-            "woff D2H");
-  hip_check(hipMemcpyAsync(len.data(), d_wlen_, n_nets_ * 4ull, hipMemcpyDeviceToHost, stream_),
-            "wlen D2H");
-  hip_check(hipMemcpyAsync(&used, d_scal_, 8, hipMemcpyDeviceToHost, stream_), "bump D2H");
-  hip_check(hipStreamSynchronize(stream_), "waves D2H");
-  pool.resize(used);
-  if (used) {
-    hip_check(hipMemcpyAsync(pool.data(), d_pool_, used * 4, hipMemcpyDeviceToHost, stream_),
-              "pool D2H");
-    hip_check(hipStreamSynchronize(stream_), "pool D2H");
-  }
 }
 
 }  // namespace gls

@@ -743,7 +743,10 @@ static int vcd_code(char c) {
   }
 }
 
-static const uint64_t kMaxTime = (1ull << 30) - (1ull << 17);
+// Stimulus times are 32-bit (the reference's device path keeps u32 times,
+// src/simulate.cu:2479-2482); the simulation moves every time chunk to its
+// own time base, so any time below 2^32 is simulated.
+static const uint64_t kMaxTime = (1ull << 32) - 1;
 
 namespace {
 
@@ -928,9 +931,11 @@ void parse_vcd_body(const std::string& s, size_t a, size_t b, const VcdHeader& H
       ++p;
       uint64_t t = 0;
       size_t q = p;
-      while (p < n && std::isdigit((unsigned char)s[p])) t = t * 10 + (uint64_t)(s[p++] - '0');
+      while (p < n && std::isdigit((unsigned char)s[p])) {
+        t = t * 10 + (uint64_t)(s[p++] - '0');
+        if (t > kMaxTime) throw GlsError(GLS_E_RANGE, "vcd: time of 2^32 or more");
+      }
       if (p == q) throw GlsError(GLS_E_FORMAT, "vcd: bad timestamp");
-      if (t > kMaxTime) throw GlsError(GLS_E_RANGE, "vcd: time beyond the 30-bit event word");
       if (t < cur_t) throw GlsError(GLS_E_FORMAT, "vcd: timestamps go backwards");
       cur_t = t;
       if (!R.has_stamp) R.first_stamp = t;

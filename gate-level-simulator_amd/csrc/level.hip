@@ -303,10 +303,19 @@ __device__ __forceinline__ void stage_tile(const LevelArgs& A, const TileDesc& D
 // overwrites the buffer's last word instead and sets bad.  LDS stacks are
 // not guarded (an LDS access outside the workgroup's allocation is
 // discarded by the hardware; all inputs are time-ordered by construction).
+//
+// Chunks: index 0 of every waveform is the net's state entry (its value
+// before the chunk's look-back window), so segment 0 starts at index 1 and
+// takes its starting values from index 0.  Candidates with output times
+// below lo_rel (made by look-back events) only pre-empt; candidates at or
+// above hi_rel belong to the next chunk and are dropped (they could only
+// pre-empt entries at or above hi_rel, and none are kept); only events at or
+// after lo_rel are counted as this chunk's gate events.
 template <int K, bool LDS, typename SP>
 __device__ __forceinline__ Lane<SP> lane_walk(const TileCtx<K, LDS>& X, uint32_t ti,
                                               const uint8_t* s_lut, const uint16_t* s_dt,
-                                              SP stk_base, uint32_t stk_words, uint32_t& bad) {
+                                              SP stk_base, uint32_t stk_words, uint32_t& bad,
+                                              uint32_t lo_rel, uint32_t hi_rel, bool first) {
   const int lane = threadIdx.x;
   const uint32_t c = ti * kLanes + lane;
   const uint32_t kX = 1u;
@@ -339,13 +348,14 @@ __device__ __forceinline__ Lane<SP> lane_walk(const TileCtx<K, LDS>& X, uint32_t
     }
 #pragma unroll
     for (int j = 0; j < K; ++j) {
+      // pj >= 1: index 0 is the state entry, and every event time is >= 1
       uint32_t pj;
-      if (c == 0) pj = 0;
+      if (c == 0) pj = 1;
       else if ((uint32_t)j == X.lead) pj = pl;
       else pj = X.B[j] + lb_words(X.src[j], bound[j], tau_c << 2);
       soff += pj - X.L[j];
       r[j] = pj - X.B[j];
-      idx |= (pj > 0 ? (X.src[j][r[j] - 1] & 3u) : kX) << (2 * j);
+      idx |= (X.src[j][r[j] - 1] & 3u) << (2 * j);
       // LDS ranges end with an INF sentinel
       h[j] = (LDS || r[j] < bound[j]) ? X.src[j][r[j]] : INF;
     }
@@ -355,9 +365,12 @@ __device__ __forceinline__ Lane<SP> lane_walk(const TileCtx<K, LDS>& X, uint32_t
     const uint32_t stk_lim = stk_words - s0;
     SP st = stk_base + s0;
     ln.st = st;
-    uint32_t fprev = c == 0 ? kX : lut_eval<K>(X, s_lut, idx);
-    if (c == 0) {  // every output starts with (0, X) (src/simulate.cpp:882-887)
-      top = kX;
+    const bool init = c == 0 && first;
+    // f before the first event: X at t = 0 (src/simulate.cpp:933), else the
+    // function of the starting values
+    uint32_t fprev = init ? kX : lut_eval<K>(X, s_lut, idx);
+    if (init) {  // every output starts with (0, X) (src/simulate.cpp:882-887); t = 0 is rel 1
+      top = (1u << 2) | kX;
       st[0] = top;
       sp = 1;
       cnt = 1;
@@ -386,19 +399,23 @@ __device__ __forceinline__ Lane<SP> lane_walk(const TileCtx<K, LDS>& X, uint32_t
       idx = nidx;
       fprev = f;
       const bool own = t < tau_n;
-      nown += own ? 1u : 0u;
+      nown += (own && t >= lo_rel) ? 1u : 0u;
       if (sc != 0) {
         const uint32_t o = t + s_dt[dt_index(K, sc, mask, fl)];
         if (own) {
-          // preemption: drop queued entries at or after o (src/simulate.cpp:991-1007)
-          while (sp > 0 && (top >> 2) >= o) stack_pop(st, sp, top, cnt);
-          const uint32_t e = (o << 2) | f;
-          const bool room = !kGuard || sp < stk_lim;
-          bad |= room ? 0u : 1u;
-          cnt += ((sp == 0 || ((e ^ top) & 3u)) && room) ? 1u : 0u;
-          st[room ? sp : stk_lim - 1] = e;
-          sp += room ? 1u : 0u;
-          top = room ? e : top;
+          if (o < hi_rel) {
+            // preemption: drop queued entries at or after o (src/simulate.cpp:991-1007)
+            while (sp > 0 && (top >> 2) >= o) stack_pop(st, sp, top, cnt);
+            if (o >= lo_rel) {
+              const uint32_t e = (o << 2) | f;
+              const bool room = !kGuard || sp < stk_lim;
+              bad |= room ? 0u : 1u;
+              cnt += ((sp == 0 || ((e ^ top) & 3u)) && room) ? 1u : 0u;
+              st[room ? sp : stk_lim - 1] = e;
+              sp += room ? 1u : 0u;
+              top = room ? e : top;
+            }
+          }
         } else {
           hmin = min(hmin, o);
         }
@@ -456,9 +473,10 @@ __device__ __forceinline__ void walk_k(const LevelArgs& A, const TileDesc& D, bo
     uint32_t bad = 0;
     if (A.stk) {
       finish(lane_walk<K, true>(X, D.ti, s_lut, s_dt, A.stk + (uint64_t)blockIdx.x * (A.cap + kLanes),
-                                A.cap + kLanes, bad));
+                                A.cap + kLanes, bad, A.lo_rel, A.hi_rel, A.first != 0));
     } else {
-      finish(lane_walk<K, true>(X, D.ti, s_lut, s_dt, (lds_u32*)s_st, A.cap + kLanes, bad));
+      finish(lane_walk<K, true>(X, D.ti, s_lut, s_dt, (lds_u32*)s_st, A.cap + kLanes, bad, A.lo_rel,
+                                A.hi_rel, A.first != 0));
     }
     if (bad) atomicOr(A.err, ERR_ORDER);
     return;
@@ -483,7 +501,8 @@ __device__ __forceinline__ void walk_k(const LevelArgs& A, const TileDesc& D, bo
     return;
   }
   uint32_t bad = 0;
-  finish(lane_walk<K, false>(X, D.ti, s_lut, s_dt, A.ovf + base, (uint32_t)need, bad));
+  finish(lane_walk<K, false>(X, D.ti, s_lut, s_dt, A.ovf + base, (uint32_t)need, bad, A.lo_rel,
+                             A.hi_rel, A.first != 0));
   if (bad) atomicOr(A.err, ERR_ORDER);
 }
 
@@ -651,7 +670,7 @@ __global__ __launch_bounds__(kLanes) void k_level(LevelArgs A) {
     __syncthreads();
     // the emit's scalars, loaded now so the walk covers their latency
     uint32_t* dst = A.scratch + tile_scratch(A.scr_base, D);
-    const unsigned long long w_start = A.win[2 * D.job], w_end = A.win[2 * D.job + 1];
+    const unsigned long long w_start = A.cwin[D.job].lo, w_end = A.cwin[D.job].hi;
     // the next tile comes from a ticket (dynamic schedule: workgroups that
     // drew slow tiles, e.g. ones too long for LDS, take fewer, so the level's
     // tail stays short); its descriptor load completes during the walk
@@ -717,8 +736,8 @@ __global__ void k_gate_carry(const GateRec* __restrict__ gates, uint32_t ng,
                              const uint64_t* __restrict__ ntile, const TileOut* __restrict__ tout,
                              uint32_t* __restrict__ tile_off, uint64_t* __restrict__ gate_cnt,
                              uint32_t* __restrict__ wlen, unsigned long long* __restrict__ saif,
-                             const unsigned long long* __restrict__ win,
-                             const unsigned int* __restrict__ err) {
+                             const ChunkWin* __restrict__ cwin, const unsigned int* __restrict__ err,
+                             TailArgs T) {
   const int lane = threadIdx.x & 63;
   const uint32_t g = blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64;
   if (g >= ng) return;  // uniform over the wavefront
@@ -730,10 +749,20 @@ __global__ void k_gate_carry(const GateRec* __restrict__ gates, uint32_t ng,
     }
     return;
   }
-  const unsigned long long start = win[2 * G.job], end = win[2 * G.job + 1];
+  const ChunkWin W = cwin[G.job];
+  const unsigned long long start = W.lo, end = W.hi;
   const uint64_t t0 = tile_base[g], n = ntile[g];
-  // carried from the previous step: count so far, last value and time
-  uint32_t cnt = 0, lv = 0, lt = 0, has = 0;
+  // carried from the previous step: count so far, last value and time.  The
+  // gate's waveform starts with its tail (state entry and look-back
+  // entries); after chunk 0 the first new entry is compared with the tail's
+  // last value, which holds from before the chunk (time 0 is clipped to
+  // the chunk's SAIF start).
+  const uint32_t tl = T.tail_len[G.out_net];
+  uint32_t cnt = tl, lv = 0, lt = 0, has = 0;
+  if (!T.first && tl) {
+    lv = T.tails[T.tail_off[G.out_net] + tl - 1] & 3u;
+    has = 1;
+  }
   unsigned long long acc[3] = {0, 0, 0}, tc = 0;
   for (uint64_t c = 0; c < n; c += 64) {
     const bool in = c + (uint64_t)lane < n;
@@ -758,7 +787,7 @@ __global__ void k_gate_carry(const GateRec* __restrict__ gates, uint32_t ng,
       acc[1] += S.t1;
       acc[2] += S.tx;
       // the tile's first entry against the previous tile's last value
-      tc += S.tc + ((p_has && is_toggle(p_lv, my_fv) && S.ft > start && S.ft <= end) ? 1u : 0u);
+      tc += S.tc + ((p_has && is_toggle(p_lv, my_fv) && S.ft > W.tc_lo && S.ft <= W.tc_hi) ? 1u : 0u);
     }
     // inclusive scan of the joined counts over the step
     uint32_t incl = mine;
@@ -791,16 +820,18 @@ __global__ void k_gate_carry(const GateRec* __restrict__ gates, uint32_t ng,
 }
 
 // Copy every tile's joined output from the scratch into the gate's dense
-// waveform in the pool (one wavefront per tile, coalesced).  When the level
-// does not fit the pool (error flag set by k_pool_guard) nothing is copied
-// and the level's waveforms are left empty.
+// waveform in the pool (one wavefront per tile, coalesced); the gate's first
+// tile also copies the gate's tail (state and look-back entries, moved to
+// this chunk's time base) to the front.  When the level does not fit the
+// pool (error flag set by k_pool_guard) nothing is copied and the level's
+// waveforms are left empty.
 __global__ void k_compact(const TileDesc* __restrict__ tiles, const uint64_t* __restrict__ ntiles,
                           const TileOut* __restrict__ tout, const uint32_t* __restrict__ tile_off,
                           const uint64_t* __restrict__ scr_base, const uint32_t* __restrict__ scratch,
                           const uint64_t* __restrict__ gate_base, uint32_t* __restrict__ pool,
                           uint64_t* __restrict__ woff, uint32_t* __restrict__ wlen,
                           const unsigned int* __restrict__ err, unsigned long long* __restrict__ saif,
-                          const unsigned long long* __restrict__ win, uint32_t count_tc) {
+                          const ChunkWin* __restrict__ cwin, uint32_t count_tc, TailArgs T) {
   const int lane = threadIdx.x & 63;
   const uint64_t waves = (uint64_t)gridDim.x * (blockDim.x / 64);
   const bool failed = (*err & ERR_CAPACITY) != 0;
@@ -814,7 +845,16 @@ __global__ void k_compact(const TileDesc* __restrict__ tiles, const uint64_t* __
       }
       continue;
     }
-    if (D.ti == 0 && lane == 0) woff[D.out_net] = gate_base[D.g];
+    if (D.ti == 0) {
+      if (lane == 0) woff[D.out_net] = gate_base[D.g];
+      const uint32_t tl = T.tail_len[D.out_net];
+      const uint32_t* ts = T.tails + T.tail_off[D.out_net];
+      uint32_t* td = pool + gate_base[D.g];
+      for (uint32_t q = lane; q < tl; q += 64) {
+        const uint32_t w = ts[q], tt = w >> 2;
+        td[q] = ((tt > T.shift ? tt - T.shift : 0u) << 2) | (w & 3u);
+      }
+    }
     const TileOut S = tout[t];
     if (!((S.flags >> 4) & 1u)) continue;
     const uint32_t off = tile_off[t] & 0x7FFFFFFFu, skip = tile_off[t] >> 31;
@@ -836,12 +876,12 @@ __global__ void k_compact(const TileDesc* __restrict__ tiles, const uint64_t* __
       // SAIF TC inside the tile: 0<->1 changes between its consecutive
       // entries (the first entry against the previous tile is k_gate_carry's)
       const uint32_t* e = src - skip;
-      const unsigned long long start = win[2 * D.job], end = win[2 * D.job + 1];
+      const ChunkWin W = cwin[D.job];
       uint32_t c = 0;
       for (uint32_t q = lane + 1; q < S.cnt; q += 64) {
         const uint32_t a = e[q - 1], b = e[q];
-        const unsigned long long t = b >> 2;
-        c += (is_toggle(a & 3u, b & 3u) && t > start && t <= end) ? 1u : 0u;
+        const unsigned long long tt = b >> 2;
+        c += (is_toggle(a & 3u, b & 3u) && tt > W.tc_lo && tt <= W.tc_hi) ? 1u : 0u;
       }
       c = wave_sum(c);
       if (lane == 0 && c) atomicAdd(saif + (uint64_t)kSaifWords * D.out_net + 3, (unsigned long long)c);
@@ -858,7 +898,8 @@ __global__ void k_set_scal(uint64_t* scal, uint64_t bump) {
 __global__ void k_plan2(const GateRec* __restrict__ gates, uint32_t ng,
                         const uint32_t* __restrict__ wlen, uint32_t seg_target,
                         GatePlan* __restrict__ plan, uint64_t* __restrict__ ntile,
-                        uint64_t* __restrict__ bound, unsigned int* __restrict__ err) {
+                        uint64_t* __restrict__ bound, unsigned int* __restrict__ err,
+                        const uint32_t* __restrict__ tail_len) {
   uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= ng) return;
   const GateRec& G = gates[g];
@@ -894,8 +935,9 @@ __global__ void k_plan2(const GateRec* __restrict__ gates, uint32_t ng,
   const uint32_t nt = (ns + kLanes - 1) / kLanes;
   ntile[g] = nt;
   // scratch words of the gate's tile outputs: every tile keeps at most its
-  // own input events + 1 entries
-  bound[g] = N + nt + 1;
+  // own input events + 1 entries; the gate's tail is added so that the
+  // level's waveforms (tails + joined entries) never exceed the scratch
+  bound[g] = N + nt + 1 + tail_len[G.out_net];
 }
 
 // per (tile, input): the tile's descriptor (gate, tile index within the
@@ -991,8 +1033,9 @@ __global__ void k_fill_u64(unsigned long long* p, uint64_t n, unsigned long long
 
 hipError_t launch_plan2(const GateRec* gates, uint32_t ng, const uint32_t* wlen,
                         uint32_t seg_target, GatePlan* plan, uint64_t* ntile, uint64_t* bound,
-                        unsigned int* err, hipStream_t st) {
-  k_plan2<<<(ng + 255) / 256, 256, 0, st>>>(gates, ng, wlen, seg_target, plan, ntile, bound, err);
+                        unsigned int* err, const uint32_t* tail_len, hipStream_t st) {
+  k_plan2<<<(ng + 255) / 256, 256, 0, st>>>(gates, ng, wlen, seg_target, plan, ntile, bound, err,
+                                            tail_len);
   return hipGetLastError();
 }
 
@@ -1046,11 +1089,11 @@ hipError_t launch_pool_guard(uint64_t* scal, uint64_t pool_words, hipStream_t st
 hipError_t launch_gate_carry(const GateRec* gates, uint32_t ng, const uint64_t* tile_base,
                              const uint64_t* ntile, const TileOut* tout, uint32_t* tile_off,
                              uint64_t* gate_cnt, uint32_t* wlen, unsigned long long* saif,
-                             const unsigned long long* win, const unsigned int* err,
+                             const ChunkWin* cwin, const unsigned int* err, const TailArgs& T,
                              hipStream_t st) {
   if (ng == 0) return hipSuccess;
   k_gate_carry<<<(ng + 3) / 4, 256, 0, st>>>(gates, ng, tile_base, ntile, tout, tile_off, gate_cnt,
-                                             wlen, saif, win, err);  // one wavefront per gate
+                                             wlen, saif, cwin, err, T);  // one wavefront per gate
   return hipGetLastError();
 }
 
@@ -1058,12 +1101,13 @@ hipError_t launch_compact(const TileDesc* tiles, const uint64_t* ntiles_dev, uin
                           const TileOut* tout, const uint32_t* tile_off, const uint64_t* scr_base,
                           const uint32_t* scratch, const uint64_t* gate_base, uint32_t* pool,
                           uint64_t* woff, uint32_t* wlen, const unsigned int* err,
-                          unsigned long long* saif, const unsigned long long* win, uint32_t count_tc,
-                          hipStream_t st) {
+                          unsigned long long* saif, const ChunkWin* cwin, uint32_t count_tc,
+                          const TailArgs& T, hipStream_t st) {
   if (ntiles_cap == 0) return hipSuccess;
   const uint64_t blocks = std::min<uint64_t>((ntiles_cap + 3) / 4, 8192);  // 4 wavefronts each
   k_compact<<<(unsigned)blocks, 256, 0, st>>>(tiles, ntiles_dev, tout, tile_off, scr_base, scratch,
-                                              gate_base, pool, woff, wlen, err, saif, win, count_tc);
+                                              gate_base, pool, woff, wlen, err, saif, cwin, count_tc,
+                                              T);
   return hipGetLastError();
 }
 

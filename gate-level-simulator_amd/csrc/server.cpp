@@ -118,8 +118,9 @@ void Server::worker(int slot) {
         if (wake) cv_done_.notify_all();
         if (stop_ && queue_.empty()) break;
         // admission: the oldest queued job whose footprint fits what is free;
-        // a job may pass the head of the queue only while the head does not fit
-        for (auto it = queue_.begin(); it != queue_.end(); ++it) {
+        // a job may pass the head of the queue only while the head does not
+        // fit.  A job that can never fit fails and the scan goes on past it.
+        for (auto it = queue_.begin(); it != queue_.end();) {
           Rec* c = jobs_[*it].get();
           bool resident = false;
           const uint64_t need = c->reserve ? c->reserve : estimate(*c, &resident);
@@ -130,15 +131,19 @@ void Server::worker(int slot) {
           if (need > opt_.device_budget) {  // can never run: fail it
             c->info.status = GLS_E_OOM;
             c->t_end = Clock::now();
-            queue_.erase(it);
+            c->t_start = c->t_end;
+            c->info.queue_ms = ms_between(c->t_submit, c->t_end);
+            c->info.latency_ms = c->info.queue_ms;
+            it = queue_.erase(it);
             cv_done_.notify_all();
-            break;
+            continue;
           }
           if (in_use_ + need <= opt_.device_budget) {
             r = c;
             queue_.erase(it);
             break;
           }
+          ++it;
         }
         if (r) break;
         if (any_deadline) cv_work_.wait_until(lk, next_deadline);

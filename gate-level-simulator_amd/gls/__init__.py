@@ -38,9 +38,11 @@ class SimOpts(ctypes.Structure):
                 ("level_blocks", ctypes.c_uint32), ("no_prefetch", ctypes.c_uint32),
                 ("graph", ctypes.c_uint32), ("hbm_stacks", ctypes.c_uint32),
                 ("pool_grow", ctypes.c_uint32), ("ovf_words", ctypes.c_uint32),
-                ("saif_tc", ctypes.c_uint32)]
+                ("saif_tc", ctypes.c_uint32), ("pad0", ctypes.c_uint32),
+                ("chunk_ps", ctypes.c_uint64)]
 
 
+NO_CHUNKS = (1 << 64) - 1  # chunk_ps: one time chunk, never split
 PATH_FUSED = 0
 PATH_TWO_PASS = 1
 
@@ -53,7 +55,8 @@ class RunStats(ctypes.Structure):
                 ("wall_ms", ctypes.c_double), ("eval_ms", ctypes.c_double),
                 ("eval_launches", ctypes.c_uint64), ("level_blocks", ctypes.c_uint64),
                 ("pool_bytes", ctypes.c_uint64), ("pool_used_bytes", ctypes.c_uint64),
-                ("tiles", ctypes.c_uint64), ("tiles_hbm", ctypes.c_uint64)]
+                ("tiles", ctypes.c_uint64), ("tiles_hbm", ctypes.c_uint64),
+                ("chunks", ctypes.c_uint64), ("chunk_retries", ctypes.c_uint64)]
 
 
 TUNE_MAX = 16
@@ -130,6 +133,8 @@ SIGNATURES = [
     ("gls_run_net_saif", _I, [_P, _S, ctypes.POINTER(_U64)]),
     ("gls_run_waveform", _I, [_P, _S, ctypes.POINTER(ctypes.c_uint32), ctypes.c_size_t,
                               ctypes.POINTER(ctypes.c_size_t)]),
+    ("gls_run_waveform64", _I, [_P, _S, ctypes.POINTER(_U64), ctypes.c_size_t,
+                                ctypes.POINTER(ctypes.c_size_t)]),
     ("gls_run_waves_hash", _I, [_P, ctypes.POINTER(_U64)]),
     ("gls_run_free", None, [_P]),
     ("gls_prepare_batch", _I, [_P, ctypes.c_size_t, _PP, _PP, ctypes.POINTER(_U64),
@@ -333,6 +338,14 @@ class Run(_Handle):
         _check(load().gls_run_waveform(self.ptr, _b(net), buf, n.value, ctypes.byref(n)))
         return [(w >> 2, w & 3) for w in buf[:n.value]]
 
+    def waveform64(self, net):
+        """Waveform with times of any size: [(time, value code)]."""
+        n = ctypes.c_size_t()
+        _check(load().gls_run_waveform64(self.ptr, _b(net), None, 0, ctypes.byref(n)))
+        buf = (_U64 * max(1, n.value))()
+        _check(load().gls_run_waveform64(self.ptr, _b(net), buf, n.value, ctypes.byref(n)))
+        return [(w >> 2, w & 3) for w in buf[:n.value]]
+
     def waves_hash(self):
         h = _U64()
         _check(load().gls_run_waves_hash(self.ptr, ctypes.byref(h)))
@@ -438,25 +451,24 @@ class Context(_Handle):
         return f.value, t.value
 
     @staticmethod
-    def _opts(pool_bytes, seg_events, keep_waves, path, level_blocks=0, prefetch=True,
-              graph=False, hbm_stacks=False, pool_grow=False, ovf_words=0, saif_tc=False):
+    def _opts(pool_bytes=0, seg_events=0, keep_waves=False, path=PATH_FUSED, level_blocks=0,
+              prefetch=True, graph=False, hbm_stacks=False, pool_grow=False, ovf_words=0,
+              saif_tc=False, chunk_ps=0):
         return SimOpts(pool_bytes, seg_events, 1 if keep_waves else 0, path, level_blocks,
                        0 if prefetch else 1, 1 if graph else 0, 1 if hbm_stacks else 0,
-                       1 if pool_grow else 0, ovf_words, 1 if saif_tc else 0)
+                       1 if pool_grow else 0, ovf_words, 1 if saif_tc else 0, 0, chunk_ps)
 
-    def prepare(self, design, stim, dumpon, dumpoff, pool_bytes=0, seg_events=0, keep_waves=False,
-                path=PATH_FUSED, level_blocks=0, prefetch=True, graph=False,
-                hbm_stacks=False, pool_grow=False, ovf_words=0, saif_tc=False):
+    def prepare(self, design, stim, dumpon, dumpoff, **opts):
+        """Options: pool_bytes, seg_events, keep_waves, path, level_blocks,
+        prefetch, graph, hbm_stacks, pool_grow, ovf_words, saif_tc, chunk_ps
+        (gls_sim_opts)."""
         h = ctypes.c_void_p()
-        o = self._opts(pool_bytes, seg_events, keep_waves, path, level_blocks, prefetch, graph,
-                       hbm_stacks, pool_grow, ovf_words, saif_tc)
+        o = self._opts(**opts)
         _check(load().gls_prepare(self.ptr, design.ptr, stim.ptr, dumpon, dumpoff, ctypes.byref(o),
                                   ctypes.byref(h)))
         return Run(h.value)
 
-    def prepare_batch(self, designs, stims, dumpons, dumpoffs, pool_bytes=0, seg_events=0,
-                      keep_waves=False, level_blocks=0, prefetch=True,
-                      graph=False, hbm_stacks=False, pool_grow=False, ovf_words=0, saif_tc=False):
+    def prepare_batch(self, designs, stims, dumpons, dumpoffs, **opts):
         n = len(designs)
         assert len(stims) == n and len(dumpons) == n and len(dumpoffs) == n
         D = (ctypes.c_void_p * n)(*[d.ptr for d in designs])
@@ -464,18 +476,16 @@ class Context(_Handle):
         on = (_U64 * n)(*dumpons)
         off = (_U64 * n)(*dumpoffs)
         h = ctypes.c_void_p()
-        o = self._opts(pool_bytes, seg_events, keep_waves, PATH_FUSED, level_blocks, prefetch,
-                       graph, hbm_stacks, pool_grow, ovf_words, saif_tc)
+        if opts.get("path", PATH_FUSED) != PATH_FUSED:
+            raise ValueError("batches run on the fused path")
+        o = self._opts(**opts)
         _check(load().gls_prepare_batch(self.ptr, n, D, S, on, off, ctypes.byref(o),
                                         ctypes.byref(h)))
         return Run(h.value)
 
-    def simulate(self, design, stim, dumpon, dumpoff, pool_bytes=0, seg_events=0, keep_waves=False,
-                 path=PATH_FUSED, level_blocks=0, prefetch=True, graph=False,
-                 hbm_stacks=False, pool_grow=False, ovf_words=0, saif_tc=False):
+    def simulate(self, design, stim, dumpon, dumpoff, **opts):
         h = ctypes.c_void_p()
-        o = self._opts(pool_bytes, seg_events, keep_waves, path, level_blocks, prefetch, graph,
-                       hbm_stacks, pool_grow, ovf_words, saif_tc)
+        o = self._opts(**opts)
         _check(load().gls_simulate(self.ptr, design.ptr, stim.ptr, dumpon, dumpoff, ctypes.byref(o),
                                    ctypes.byref(h)))
         return Run(h.value)

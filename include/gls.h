@@ -120,7 +120,16 @@ typedef struct {
   uint32_t saif_tc;      /* 1: SAIF records also carry (TC n), the number of 0<->1 transitions
                             in (dumpon, dumpoff].  0 (default): the reference's (T0)(T1)(TX)
                             records */
+  uint32_t pad0;
+  uint64_t chunk_ps;     /* fused path: time-chunk length in VCD time units.  0 (default):
+                            automatic -- one chunk when the run fits HBM, else chunks sized to
+                            the event pool; GLS_NO_CHUNKS: one chunk, a pool too small fails
+                            with GLS_E_OOM; else a fixed length (above the largest gate
+                            delay).  Results do not depend on it. */
 } gls_sim_opts;
+
+/* chunk_ps value: never split the window. */
+#define GLS_NO_CHUNKS (~0ull)
 
 /* Level-kernel variants: one fused LDS-tiled kernel per level, or the
  * two-pass form (evaluation into scratch, then compaction). */
@@ -155,6 +164,8 @@ typedef struct {
   uint64_t pool_used_bytes; /* pool bytes holding waveforms at the end of the run */
   uint64_t tiles;           /* fused path: tiles walked (a re-run level counts twice) */
   uint64_t tiles_hbm;       /* of them too long for LDS (inputs read from HBM) */
+  uint64_t chunks;          /* time chunks of the run (1: the whole window at once) */
+  uint64_t chunk_retries;   /* chunks run again shorter because they did not fit the pool */
 } gls_run_stats;
 int gls_run_stats_get(const gls_run* r, gls_run_stats* st);
 /* SAIF writer (DumpSaif, src/simulate.cu:113-481 / src/simulate.cpp:37-380). */
@@ -167,9 +178,13 @@ int gls_run_net_saif(const gls_run* r, const char* net, uint64_t t0t1tx[3]);
 /* Toggle count of one net: 0<->1 transitions in (dumpon, dumpoff] (no counterpart in the
  * reference, whose SAIF has no TC field; needs the saif_tc option). */
 int gls_run_net_toggles(const gls_run* r, const char* net, uint64_t* tc);
-/* Waveform of one net as (time << 2 | value) words (needs keep_waves). */
+/* Waveform of one net as (time << 2 | value) words (needs keep_waves);
+ * GLS_E_RANGE when a time is 2^30 or more. */
 int gls_run_waveform(const gls_run* r, const char* net, uint32_t* words, size_t cap, size_t* n);
-/* Order-independent 64-bit hash over every net's waveform (needs keep_waves). */
+/* Same with 64-bit words, for any time. */
+int gls_run_waveform64(const gls_run* r, const char* net, uint64_t* words, size_t cap, size_t* n);
+/* Order-independent 64-bit hash over every gate output's waveform (needs
+ * keep_waves); each event enters as the low 32 bits of (time << 2 | value). */
 int gls_run_waves_hash(const gls_run* r, uint64_t* hash);
 void gls_run_free(gls_run* r);
 

@@ -48,19 +48,28 @@ def test_no_device_is_an_error_not_a_crash():
 
 
 def test_binding_keyword_options():
+    import ctypes
     # every option of gls_sim_opts reaches the struct through each entry point
     import inspect
     fields = [n for n, _ in gls.SimOpts._fields_]
     assert fields == ["pool_bytes", "seg_events", "keep_waves", "path", "level_blocks",
-                      "no_prefetch", "graph", "hbm_stacks", "pool_grow", "ovf_words", "saif_tc"]
-    o = gls.Context._opts(8, 2, True, gls.PATH_FUSED, 3, False, True, True, True, 77, True)
+                      "no_prefetch", "graph", "hbm_stacks", "pool_grow", "ovf_words", "saif_tc",
+                      "pad0", "chunk_ps"]
+    # the C layout: chunk_ps is 8-byte aligned after the eleven 4-byte fields
+    assert gls.SimOpts.chunk_ps.offset == 56 and ctypes.sizeof(gls.SimOpts) == 64
+    o = gls.Context._opts(pool_bytes=8, seg_events=2, keep_waves=True, path=gls.PATH_FUSED,
+                          level_blocks=3, prefetch=False, graph=True, hbm_stacks=True,
+                          pool_grow=True, ovf_words=77, saif_tc=True, chunk_ps=12345)
     assert (o.pool_bytes, o.seg_events, o.keep_waves, o.level_blocks, o.no_prefetch, o.graph,
-            o.hbm_stacks, o.pool_grow, o.ovf_words, o.saif_tc) == (8, 2, 1, 3, 1, 1, 1, 1, 77, 1)
+            o.hbm_stacks, o.pool_grow, o.ovf_words, o.saif_tc, o.chunk_ps) == \
+        (8, 2, 1, 3, 1, 1, 1, 1, 77, 1, 12345)
+    # every keyword reaches the struct through each entry point (they share _opts)
+    params = inspect.signature(gls.Context._opts).parameters
+    for k in ("seg_events", "level_blocks", "prefetch", "graph", "hbm_stacks", "pool_grow",
+              "ovf_words", "saif_tc", "chunk_ps"):
+        assert k in params, k
     for fn in (gls.Context.prepare, gls.Context.simulate, gls.Context.prepare_batch):
-        params = inspect.signature(fn).parameters
-        for k in ("seg_events", "level_blocks", "prefetch", "graph", "hbm_stacks", "pool_grow",
-                  "ovf_words", "saif_tc"):
-            assert k in params, (fn.__name__, k)
+        assert "opts" in inspect.signature(fn).parameters, fn.__name__
 
 
 def test_null_handles_and_bad_arguments_are_status_codes():

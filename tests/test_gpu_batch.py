@@ -105,7 +105,7 @@ def test_autotune_keeps_results(ctx):
 def test_two_pass_rejects_batches(ctx):
     insts = mixed_instances()[:2]
     with pytest.raises(gls.GlsError) as e:
-        o = gls.Context._opts(0, 0, False, gls.PATH_TWO_PASS)
+        o = gls.Context._opts(path=gls.PATH_TWO_PASS)
         import ctypes
         n = 2
         D = (ctypes.c_void_p * n)(*[i[0].ptr for i in insts])
@@ -147,6 +147,24 @@ def test_server_rejects_jobs_over_budget(ctx):
     srv = gls.Server(ctx, device_budget=1 << 20, max_running=1)
     j = srv.submit(d, s, 0, 200001)
     assert srv.wait(j) == -3  # GLS_E_OOM: the estimate can never fit
+    srv.close()
+
+
+def test_server_oversize_job_does_not_block_the_queue(ctx):
+    # one worker, an oversize job at the head of the queue and a small job
+    # behind it: the oversize job fails with GLS_E_OOM, the small one still
+    # runs (the admission scan goes on past a job it fails), and close()
+    # (which drains) returns
+    d, s = golden_cases.make("c0_small")
+    on, off = golden_cases.CASES["c0_small"]["window"]
+    ref = ctx.simulate(d, s, on, off).saif_text()
+    s_big = gls.Stimulus.generate(d, 5, 20_000_000, 100.0, 0.01)
+    srv = gls.Server(ctx, device_budget=256 << 20, max_running=1)
+    big = srv.submit(d, s_big, 0, 20_000_001)
+    small = srv.submit(d, s, on, off)
+    assert srv.wait(small, 120.0) == 0
+    assert srv.saif_text(small) == ref
+    assert srv.wait(big, 10.0) == -3
     srv.close()
 
 

@@ -220,10 +220,11 @@ def test_overflow_buffer_grows(ctx):
 
 
 def test_pool_exhaustion_is_reported(ctx):
-    # the pool is clamped to the stimulus plus 1024 words: level 0 cannot fit
+    # one time chunk only, the pool clamped to 4 KiB: level 0 cannot fit
+    # (automatic chunks would cut the window to fit, see test_gpu_chunks)
     d, s = golden_cases.make("c0_small")
     with pytest.raises(gls.GlsError) as e:
-        ctx.simulate(d, s, 0, 200001, pool_bytes=4096)
+        ctx.simulate(d, s, 0, 200001, pool_bytes=4096, chunk_ps=gls.NO_CHUNKS)
     assert e.value.status == "GLS_E_OOM"
     # the context stays usable after the failure
     r = ctx.simulate(d, s, 0, 200001, keep_waves=True)
