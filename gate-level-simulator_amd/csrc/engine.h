@@ -250,6 +250,7 @@ class Job {
   std::vector<uint64_t> sched_;  // F_1 .. F_K of the last run
   uint64_t max_tiles_ = 0, tiles_cap_ = 0, max_scratch_ = 0, scratch_cap_ = 0;
   uint64_t max_tails_ = 0, max_bump_ = 0;
+  uint64_t pool_reallocs_ = 0;  // pool allocations made while running
   bool graph_scratch_in_pool_ = false;  // captured levels keep their scratch at the pool top
   uint32_t* d_stk_ = nullptr;
   size_t cap_stk_ = 0;

@@ -2,6 +2,8 @@
 
 #include <algorithm>
 #include <chrono>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 
 #include "../../include/gls_status.h"
@@ -355,6 +357,16 @@ static std::shared_ptr<DevProgram> union_image(const std::vector<std::shared_ptr
 
 // ------------------------------------------------------------------ job ---
 
+// GLS_TRACE=1: one line per time chunk on stderr (bounds, outcome, pool
+// words used, host time, pool reallocations).
+static bool trace_on() {
+  static const bool on = [] {
+    const char* e = std::getenv("GLS_TRACE");
+    return e && *e && *e != '0';
+  }();
+  return on;
+}
+
 Job::Job(int device, hipStream_t stream) : device_(device), stream_(stream) {}
 
 Job::~Job() {
@@ -424,6 +436,7 @@ bool Job::grow_pool(uint64_t need, uint64_t used) {
     (void)hipGetLastError();
     return false;
   }
+  ++pool_reallocs_;
   if (used) HC(hipMemcpyAsync(p, d_pool_, used * 4, hipMemcpyDeviceToDevice, stream_));
   HC(hipStreamSynchronize(stream_));
   HC(hipFree(d_pool_));
@@ -451,7 +464,10 @@ uint64_t Job::pool_max_words() const {
 // that can be allocated above the current size).  Returns whether the pool
 // grew.
 bool Job::refresh_pool(uint64_t words) {
-  if (!pool_auto_ || words <= pool_words_) return false;
+  // a new pool only for a real gain: each allocation of a large pool costs
+  // host time
+  if (!pool_auto_ || words <= pool_words_ + pool_words_ / 4) return false;
+  ++pool_reallocs_;
   HC(hipStreamSynchronize(stream_));
   HC(hipFree(d_pool_));
   d_pool_ = nullptr;
@@ -1019,7 +1035,20 @@ void Job::run_chunks(std::vector<hipEvent_t>& ev_eval) {
     HC(hipStreamSynchronize(stream_));
     put_chunk_windows(c, false);
     double hint = 0;
-    if (run_chunk(c, ev_eval, false, &hint) != CHUNK_OK) {
+    const auto tc0 = std::chrono::steady_clock::now();
+    const uint64_t re0 = pool_reallocs_;
+    const int res = run_chunk(c, ev_eval, false, &hint);
+    if (trace_on()) {
+      uint64_t b = 0;
+      HC(hipMemcpyAsync(&b, d_scal_ + S_BUMP, 8, hipMemcpyDeviceToHost, stream_));
+      HC(hipStreamSynchronize(stream_));
+      std::fprintf(stderr, "gls: chunk %zu [%llu, %llu) L=%llu %s bump=%llu pool=%llu reallocs=%llu %.1f ms\n",
+                   k, (unsigned long long)F, (unsigned long long)(Fn == kChunkEnd ? 0 : Fn),
+                   (unsigned long long)L, res == CHUNK_OK ? "ok" : "shorter", (unsigned long long)b,
+                   (unsigned long long)pool_words_, (unsigned long long)(pool_reallocs_ - re0),
+                   std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - tc0).count());
+    }
+    if (res != CHUNK_OK) {
       if (chunk_ps != 0)
         throw GlsError(GLS_E_OOM, "event pool exhausted (raise the pool size or allow time chunks)");
       HC(hipStreamSynchronize(stream_));

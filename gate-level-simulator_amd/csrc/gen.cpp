@@ -1,6 +1,7 @@
 #include "gen.h"
 
 #include <algorithm>
+#include <thread>
 #include <cmath>
 #include <cstring>
 
@@ -266,8 +267,64 @@ Netlist generate_netlist(const NetlistSpec& sp, const Library& lib,
   return nl;
 }
 
+namespace {
+
+// The stimulus walk draws from one SplitMix64 stream in a fixed order.  A
+// draw is a function of its index alone (the generator's state is a
+// counter), so blocks of draws and the logarithms of their uniforms are
+// computed ahead on several threads, and the serial walk only consumes
+// them: the result is the serial pass's, bit for bit, on any thread count.
+class AheadStream {
+ public:
+  AheadStream(uint64_t seed, int threads) : seed_(seed), threads_(std::max(1, threads)) {}
+  uint64_t next() {
+    ensure();
+    return raw_[idx_++ - base_];
+  }
+  double u01() { return to_u01(next()); }
+  // det_log(u01()) of the next draw
+  double log_u01() {
+    ensure();
+    return logu_[idx_++ - base_];
+  }
+
+ private:
+  static double to_u01(uint64_t r) { return ((r >> 11) + 1) * (1.0 / 9007199254740992.0); }
+  void ensure() {
+    if (idx_ >= base_ && idx_ < base_ + raw_.size()) return;
+    base_ = idx_;
+    const size_t B = raw_.empty() ? (1u << 16) : std::min<size_t>(raw_.size() * 4, 1u << 22);
+    raw_.resize(B);
+    logu_.resize(B);
+    auto work = [&](size_t a, size_t b) {
+      for (size_t i = a; i < b; ++i) {
+        // the (base + i + 1)-th draw of a SplitMix64 seeded with seed_
+        uint64_t z = seed_ + (base_ + i + 1) * 0x9E3779B97F4A7C15ull;
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        raw_[i] = z ^ (z >> 31);
+        logu_[i] = det_log(to_u01(raw_[i]));
+      }
+    };
+    const int T = B >= (1u << 18) ? threads_ : 1;
+    if (T == 1) {
+      work(0, B);
+      return;
+    }
+    std::vector<std::thread> th;
+    for (int t = 0; t < T; ++t) th.emplace_back(work, B * t / T, B * (t + 1) / T);
+    for (auto& x : th) x.join();
+  }
+  uint64_t seed_, idx_ = 0, base_ = 0;
+  int threads_;
+  std::vector<uint64_t> raw_;
+  std::vector<double> logu_;
+};
+
+}  // namespace
+
 Stimulus generate_stimulus(const Netlist& nl, const StimulusSpec& sp) {
-  SplitMix64 rng(sp.seed ^ 0xA5A5A5A5DEADBEEFull);
+  AheadStream rng(sp.seed ^ 0xA5A5A5A5DEADBEEFull, host_threads(0));
   Stimulus st;
   st.scope = nl.top;
   for (size_t n = 0; n < nl.net_names.size(); ++n) {
@@ -278,7 +335,7 @@ Stimulus generate_stimulus(const Netlist& nl, const StimulusSpec& sp) {
     w.push(0, cur);
     double t = 0;
     for (;;) {
-      double gap = -sp.mean_gap * det_log(rng.u01());
+      double gap = -sp.mean_gap * rng.log_u01();
       uint64_t g = (uint64_t)std::floor(gap + 0.5);
       if (g < 1) g = 1;
       t += (double)g;
