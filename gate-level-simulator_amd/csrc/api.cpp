@@ -404,6 +404,8 @@ int gls_prepare_batch(gls_ctx* c, size_t n, const gls_design* const* designs,
   r->job = std::make_unique<Job>(c->device, c->stream);
   apply_opts(*r->job, opts);
   r->job->prepare(inst, opts ? opts->pool_bytes / 4 : 0);
+  // sized once here: executions only copy into it
+  r->saif.resize((size_t)r->job->stats().n_nets * kSaifWords);
   *out = r.release();
   API_END
 }
@@ -417,6 +419,15 @@ int gls_run_execute(gls_run* r) {
   API_BEGIN
   if (!r) throw GlsError(GLS_E_ARG, "null argument");
   r->job->run();
+  r->job->fetch_saif(r->saif);
+  r->executed = true;
+  API_END
+}
+
+int gls_run_execute_static(gls_run* r) {
+  API_BEGIN
+  if (!r) throw GlsError(GLS_E_ARG, "null argument");
+  r->job->run_static();
   r->job->fetch_saif(r->saif);
   r->executed = true;
   API_END
@@ -458,6 +469,7 @@ int gls_run_stats_get(const gls_run* r, gls_run_stats* st) {
   st->pool_used_bytes = j.pool_words_used * 4;
   st->chunks = j.chunks;
   st->chunk_retries = j.chunk_retries;
+  st->device_allocs = j.device_allocs;
   API_END
 }
 

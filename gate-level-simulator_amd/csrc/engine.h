@@ -35,6 +35,7 @@ struct JobStats {
   uint64_t tiles = 0, tiles_hbm = 0;  // fused tiles walked, and of them on the HBM path
   uint64_t chunks = 0;          // time chunks of the run (fused path)
   uint64_t chunk_retries = 0;   // chunks run again shorter (pool too small)
+  uint64_t device_allocs = 0;   // device allocations made by the last execution
 };
 
 // Host and device image of one compiled program for one delay time scale:
@@ -121,6 +122,13 @@ class Job {
   void prepare(const std::vector<SimInstance>& inst, uint64_t pool_words);
   // Run all levels.  Throws GlsError.
   void run();
+  // run() with no device allocation: the per-level scratch is carved from
+  // one static device array of kStaticScratchBytes per GPU; the pool cannot
+  // grow (time chunks shrink to fit it instead).  Fused path, no graph, no
+  // kept waveforms, stacks in LDS.  Calls that overlap on one GPU share the
+  // array and race.
+  void run_static();
+  static constexpr size_t kStaticScratchBytes = 256ull << 20;
   // Results (union net space; instance i's nets start at net_base(i))
   void fetch_saif(std::vector<uint64_t>& t0t1tx) const;
   // Waveforms of the gate outputs (absolute (time << 2 | value) words; other
@@ -251,6 +259,8 @@ class Job {
   uint64_t max_tiles_ = 0, tiles_cap_ = 0, max_scratch_ = 0, scratch_cap_ = 0;
   uint64_t max_tails_ = 0, max_bump_ = 0;
   uint64_t pool_reallocs_ = 0;  // pool allocations made while running
+  uint64_t allocs_ = 0;         // device allocations made since the run started
+  bool static_ = false;         // run_static(): no allocation, scratch from the static array
   bool graph_scratch_in_pool_ = false;  // captured levels keep their scratch at the pool top
   uint32_t* d_stk_ = nullptr;
   size_t cap_stk_ = 0;

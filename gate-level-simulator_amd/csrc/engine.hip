@@ -5,6 +5,8 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <map>
+#include <mutex>
 
 #include "../../include/gls_status.h"
 
@@ -382,6 +384,8 @@ Job::~Job() {
 void* Job::dmalloc(size_t bytes) {
   void* p = nullptr;
   bytes = std::max<size_t>(bytes, 256);
+  if (static_) throw GlsError(GLS_E_STATE, "device allocation during an allocation-free execution");
+  ++allocs_;
   HC(hipMalloc(&p, bytes));
   owned_.push_back(p);
   dev_bytes_ += bytes;
@@ -389,13 +393,21 @@ void* Job::dmalloc(size_t bytes) {
 }
 
 void Job::ensure(void** p, size_t& cap, size_t need) {
-  if (!try_ensure(p, cap, need)) HC(hipErrorOutOfMemory);
+  if (try_ensure(p, cap, need)) return;
+  if (static_)
+    throw GlsError(GLS_E_RANGE, "the run needs more scratch than the allocation-free execution's "
+                                "static array holds");
+  HC(hipErrorOutOfMemory);
 }
 
 // As ensure(), but an allocation that fails leaves the buffer released
 // (cap 0) and returns false instead of throwing.
 bool Job::try_ensure(void** p, size_t& cap, size_t need) {
   if (need <= cap && *p) return true;
+  // the allocation-free execution never frees or allocates: a buffer of the
+  // static scratch that is too small fails the request
+  if (static_) return false;
+  ++allocs_;
   size_t n = std::max<size_t>(need, cap + cap / 2);
   n = std::max<size_t>(n, 4096);
   if (*p) HC(hipFree(*p));
@@ -424,7 +436,7 @@ bool Job::try_ensure(void** p, size_t& cap, size_t need) {
 // grows.  A captured graph holds the old pool address and is dropped.
 bool Job::grow_pool(uint64_t need, uint64_t used) {
   if (need <= pool_words_) return true;
-  if (!pool_auto_) return false;
+  if (!pool_auto_ || static_) return false;
   size_t free_b = 0, total_b = 0;
   HC(hipMemGetInfo(&free_b, &total_b));
   const uint64_t room = (uint64_t)(free_b * 0.9) / 4;  // the old pool stays during the copy
@@ -437,6 +449,7 @@ bool Job::grow_pool(uint64_t need, uint64_t used) {
     return false;
   }
   ++pool_reallocs_;
+  ++allocs_;
   if (used) HC(hipMemcpyAsync(p, d_pool_, used * 4, hipMemcpyDeviceToDevice, stream_));
   HC(hipStreamSynchronize(stream_));
   HC(hipFree(d_pool_));
@@ -466,8 +479,9 @@ uint64_t Job::pool_max_words() const {
 bool Job::refresh_pool(uint64_t words) {
   // a new pool only for a real gain: each allocation of a large pool costs
   // host time
-  if (!pool_auto_ || words <= pool_words_ + pool_words_ / 4) return false;
+  if (!pool_auto_ || static_ || words <= pool_words_ + pool_words_ / 4) return false;
   ++pool_reallocs_;
+  ++allocs_;
   HC(hipStreamSynchronize(stream_));
   HC(hipFree(d_pool_));
   d_pool_ = nullptr;
@@ -756,7 +770,7 @@ int Job::run_chunk(const Chunk& c, std::vector<hipEvent_t>& ev_eval, bool device
   unsigned long long* evc = (unsigned long long*)(d_scal_ + S_EVC);
   unsigned int* err = (unsigned int*)(d_scal_ + S_ERR);
   const uint32_t seg = seg_target ? seg_target : kDefaultSeg;
-  const bool retry = !device_driven && (ovf_words_opt == 0 || pool_grow);
+  const bool retry = !device_driven && !static_ && (ovf_words_opt == 0 || pool_grow);
   const size_t nlev = level_start_.size() - 1;
   const ChunkWin* cw = d_cwin_ + c.slot * net_base_.size();
   TailArgs T;
@@ -854,7 +868,7 @@ int Job::run_chunk(const Chunk& c, std::vector<hipEvent_t>& ev_eval, bool device
       // the largest pool is cut short before it runs further
       if (projected && g0 >= std::max<uint32_t>(64, n_gates_ / 32) && used > stim_chunk) {
         const double proj = (double)(used - stim_chunk) * n_gates_ / g0;
-        const double cap = (double)(pool_auto_ ? pool_max_words() : pool_words_);
+        const double cap = (double)(pool_auto_ && !static_ ? pool_max_words() : pool_words_);
         if (((double)stim_chunk + proj) * 1.35 > cap) {
           *grow_hint = std::max(0.05, 0.85 * (cap / 1.35 - stim_chunk) / proj);
           return CHUNK_SHORTER;
@@ -915,7 +929,7 @@ int Job::run_chunk(const Chunk& c, std::vector<hipEvent_t>& ev_eval, bool device
       if (!device_driven) ensure((void**)&d_stk_, cap_stk_, w * 4);
       a.stk = d_stk_;
     }
-    if (!device_driven) {
+    if (!device_driven && !static_) {
       hipEvent_t e_a, e_b;
       HC(hipEventCreate(&e_a));
       HC(hipEventCreate(&e_b));
@@ -1070,7 +1084,7 @@ void Job::run_chunks(std::vector<hipEvent_t>& ev_eval) {
     if (Fn == kChunkEnd) break;
     if (chunk_ps == 0) {
       // next length: half of the largest pool at this chunk's output rate
-      const double target = 0.5 * (double)(pool_auto_ ? pool_max_words() : pool_words_);
+      const double target = 0.5 * (double)(pool_auto_ && !static_ ? pool_max_words() : pool_words_);
       const double rate = (double)std::max<uint64_t>(used, 1) / (double)L;
       const double want = target / rate;
       L = (uint64_t)std::min<double>(std::max<double>(want, (double)L / 4), (double)L * 4);
@@ -1166,9 +1180,12 @@ void Job::run() {
     return;
   }
   auto w0 = std::chrono::steady_clock::now();
-  hipEvent_t e0, e1;
-  HC(hipEventCreate(&e0));
-  HC(hipEventCreate(&e1));
+  allocs_ = 0;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (!static_) {
+    HC(hipEventCreate(&e0));
+    HC(hipEventCreate(&e1));
+  }
   if (mode == 1) {
     uint64_t scal[16] = {0};
     scal[S_BUMP] = stim_words_;
@@ -1178,7 +1195,7 @@ void Job::run() {
     HC(launch_fill_u64(d_saif_, (uint64_t)n_nets_ * kSaifWords, 0ull, stream_));
     HC(hipStreamSynchronize(stream_));
   }
-  HC(hipEventRecord(e0, stream_));
+  if (e0) HC(hipEventRecord(e0, stream_));
   std::vector<hipEvent_t> ev_eval;
   try {
     if (mode == 1) {
@@ -1191,16 +1208,21 @@ void Job::run() {
   } catch (...) {
     (void)hipStreamSynchronize(stream_);
     for (hipEvent_t e : ev_eval) (void)hipEventDestroy(e);
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
     throw;
   }
-  HC(hipEventRecord(e1, stream_));
-  HC(hipEventSynchronize(e1));
   float ms = 0;
-  HC(hipEventElapsedTime(&ms, e0, e1));
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
+  if (e0) {
+    HC(hipEventRecord(e1, stream_));
+    HC(hipEventSynchronize(e1));
+    HC(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+  } else {
+    HC(hipStreamSynchronize(stream_));
+    ms = (float)std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - w0).count();
+  }
   double eval_ms = 0;
   for (size_t i = 0; i + 1 < ev_eval.size(); i += 2) {
     float x = 0;
@@ -1231,10 +1253,108 @@ void Job::run() {
       }
   }
   stats_.wall_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - w0).count();
-  if (graph && mode == 0 && sig != graph_sig_) {
+  stats_.device_allocs = allocs_;
+  if (graph && mode == 0 && sig != graph_sig_ && !static_) {
     build_graph();
     graph_sig_ = sig;
   }
+}
+
+// ------------------------------------------------ allocation-free run ---
+
+// The scratch of the allocation-free execution: one array per GPU, held
+// until the process ends.  Host code reaches it only through the address
+// hipGetSymbolAddress gives for the current device (the array's name in host
+// code is a host-side stand-in, not a device address).
+static __device__ __align__(256) unsigned char scratch[Job::kStaticScratchBytes];
+
+// Device address of this device's `scratch`, looked up once per device with
+// that device current.
+static unsigned char* static_scratch(int device) {
+  static std::mutex mu;
+  static std::map<int, unsigned char*> ptrs;
+  std::lock_guard<std::mutex> lk(mu);
+  auto it = ptrs.find(device);
+  if (it != ptrs.end()) return it->second;
+  void* p = nullptr;
+  hip_check(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)");
+  if (!p) throw GlsError(GLS_E_DEVICE, "no device address for the static scratch");
+  ptrs[device] = (unsigned char*)p;
+  return (unsigned char*)p;
+}
+
+// Every per-level and per-chunk buffer that run() would allocate on demand
+// is carved from the static array: chunk windows, gate-output tails (a
+// sixteenth of the array), the overflow stack buffer (2^22 words, or a
+// smaller ovf_words), and tile descriptors / summaries / offsets in the
+// rest.  The level scratch lives in the pool's free top (a level that does
+// not fit there makes its chunk shorter).  None of these buffers is read
+// before the run writes it, so nothing is cleared.  The job's own buffers
+// are put back afterwards.
+void Job::run_static() {
+  HC(hipSetDevice(device_));
+  if (mode != 0) throw GlsError(GLS_E_UNSUPPORTED, "the allocation-free execution runs the fused path");
+  if (graph) throw GlsError(GLS_E_UNSUPPORTED, "the allocation-free execution does not capture graphs");
+  if (keep_waves)
+    throw GlsError(GLS_E_UNSUPPORTED, "the allocation-free execution does not keep waveforms");
+  if (stack_mode) throw GlsError(GLS_E_UNSUPPORTED, "the allocation-free execution keeps stacks in LDS");
+  unsigned char* base = static_scratch(device_);
+  size_t off = 0;
+  auto carve = [&](size_t bytes) -> void* {
+    bytes = (bytes + 255) / 256 * 256;
+    if (off + bytes > kStaticScratchBytes)
+      throw GlsError(GLS_E_RANGE, "the batch needs more than the allocation-free execution's static array");
+    void* p = base + off;
+    off += bytes;
+    return p;
+  };
+  struct Saved {
+    void* p;
+    size_t cap;
+  };
+  void** bufs[] = {(void**)&d_cwin_, (void**)&d_tails_, (void**)&d_ovf_, (void**)&d_tiles_,
+                   (void**)&d_tout_, (void**)&d_toff_, (void**)&d_scratch_};
+  size_t* caps[] = {&cap_cwin_, &cap_tails_, &cap_ovf_, &cap_tiles_, &cap_tout_, &cap_toff_,
+                    &cap_scratch_};
+  Saved saved[7];
+  for (int i = 0; i < 7; ++i) saved[i] = {*bufs[i], *caps[i]};
+  const uint64_t saved_ovf = ovf_words_;
+  const size_t cw = net_base_.size() * sizeof(ChunkWin);
+  d_cwin_ = (ChunkWin*)carve(cw);
+  cap_cwin_ = cw;
+  const size_t tb = kStaticScratchBytes / 16;
+  d_tails_ = (uint32_t*)carve(tb);
+  cap_tails_ = tb;
+  ovf_words_ = std::min<uint64_t>(ovf_words_opt ? ovf_words_opt : (1ull << 22), 1ull << 22);
+  d_ovf_ = (uint32_t*)carve(ovf_words_ * 4);
+  cap_ovf_ = ovf_words_ * 4;
+  const size_t per_tile = sizeof(TileDesc) + sizeof(TileOut) + 4;
+  const size_t tiles = (kStaticScratchBytes - off - 3 * 256) / per_tile;
+  d_tiles_ = (TileDesc*)carve(tiles * sizeof(TileDesc));
+  cap_tiles_ = tiles * sizeof(TileDesc);
+  d_tout_ = (TileOut*)carve(tiles * sizeof(TileOut));
+  cap_tout_ = tiles * sizeof(TileOut);
+  d_toff_ = (uint32_t*)carve(tiles * 4);
+  cap_toff_ = tiles * 4;
+  d_scratch_ = nullptr;  // the level scratch stays in the pool
+  cap_scratch_ = 0;
+  auto restore = [&] {
+    for (int i = 0; i < 7; ++i) {
+      *bufs[i] = saved[i].p;
+      *caps[i] = saved[i].cap;
+    }
+    ovf_words_ = saved_ovf;
+    static_ = false;
+  };
+  static_ = true;
+  try {
+    run();
+  } catch (...) {
+    (void)hipStreamSynchronize(stream_);
+    restore();
+    throw;
+  }
+  restore();
 }
 
 // Capture one device-driven execution of every chunk and level into a graph

@@ -56,7 +56,8 @@ class RunStats(ctypes.Structure):
                 ("eval_launches", ctypes.c_uint64), ("level_blocks", ctypes.c_uint64),
                 ("pool_bytes", ctypes.c_uint64), ("pool_used_bytes", ctypes.c_uint64),
                 ("tiles", ctypes.c_uint64), ("tiles_hbm", ctypes.c_uint64),
-                ("chunks", ctypes.c_uint64), ("chunk_retries", ctypes.c_uint64)]
+                ("chunks", ctypes.c_uint64), ("chunk_retries", ctypes.c_uint64),
+                ("device_allocs", ctypes.c_uint64)]
 
 
 TUNE_MAX = 16
@@ -126,6 +127,7 @@ SIGNATURES = [
     ("gls_ctx_destroy", None, [_P]),
     ("gls_prepare", _I, [_P, _P, _P, _U64, _U64, ctypes.POINTER(SimOpts), _PP]),
     ("gls_run_execute", _I, [_P]),
+    ("gls_run_execute_static", _I, [_P]),
     ("gls_simulate", _I, [_P, _P, _P, _U64, _U64, ctypes.POINTER(SimOpts), _PP]),
     ("gls_run_stats_get", _I, [_P, ctypes.POINTER(RunStats)]),
     ("gls_run_write_saif", _I, [_P, _S]),
@@ -303,6 +305,12 @@ class Run(_Handle):
 
     def execute(self):
         _check(load().gls_run_execute(self.ptr))
+        return self
+
+    def execute_static(self):
+        """gls_run_execute_static: the same execution with no device
+        allocation (scratch from the library's static device array)."""
+        _check(load().gls_run_execute_static(self.ptr))
         return self
 
     def stats(self):

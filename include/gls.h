@@ -145,6 +145,22 @@ int gls_prepare(gls_ctx* c, const gls_design* d, const gls_stim* s, uint64_t dum
  * and the SAIF reduction of src/simulate.cu:1563-1784).  May be called again
  * on the same run; each call recomputes from the resident stimulus. */
 int gls_run_execute(gls_run* r);
+/* Allocation-free variant of gls_run_execute (no counterpart in the
+ * reference; for callers that cannot allocate device memory: code under
+ * stream capture, tight serving loops, harnesses that forbid hipMalloc).
+ * The run must have been prepared (gls_prepare / gls_prepare_batch hold the
+ * inputs and outputs in device memory); this call makes no hipMalloc,
+ * hipFree or other allocating call.  Its per-level scratch (tile
+ * descriptors and summaries, overflow stacks, time-chunk tails and windows)
+ * comes from one static device array of GLS_STATIC_SCRATCH_BYTES per GPU,
+ * held until the process ends; the event pool does not grow (time chunks
+ * shrink to fit it, a level whose scratch does not fit the pool's free top
+ * fails with GLS_E_OOM at the shortest chunk) and a run whose tiles or tails
+ * exceed the array fails with GLS_E_RANGE.  Fused path only, without graph
+ * capture, kept waveforms or HBM stacks (GLS_E_UNSUPPORTED).  Calls that
+ * overlap on one GPU share the array and race: serialize them. */
+#define GLS_STATIC_SCRATCH_BYTES (256ull << 20)
+int gls_run_execute_static(gls_run* r);
 /* prepare + execute */
 int gls_simulate(gls_ctx* c, const gls_design* d, const gls_stim* s, uint64_t dumpon,
                  uint64_t dumpoff, const gls_sim_opts* opts, gls_run** out);
@@ -166,6 +182,7 @@ typedef struct {
   uint64_t tiles_hbm;       /* of them too long for LDS (inputs read from HBM) */
   uint64_t chunks;          /* time chunks of the run (1: the whole window at once) */
   uint64_t chunk_retries;   /* chunks run again shorter because they did not fit the pool */
+  uint64_t device_allocs;   /* device allocations made by the last execution */
 } gls_run_stats;
 int gls_run_stats_get(const gls_run* r, gls_run_stats* st);
 /* SAIF writer (DumpSaif, src/simulate.cu:113-481 / src/simulate.cpp:37-380). */
