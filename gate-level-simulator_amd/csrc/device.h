@@ -32,8 +32,9 @@ static_assert(sizeof(GateDev) == 80, "GateDev layout");
 
 // Gate record of the fused level kernel, 32 bytes.  The delay of an output
 // event is one lookup in the gate's delay table (u16 entries at dtab +
-// dt_off) indexed by dt_index(): the output edge class s (1..4), the set of
-// inputs with an event at this time, and the subset of those that fell.
+// dt_off) indexed by dt_index(): the output edge class s (1..4) and, per
+// input j, whether it has an event at this time (bit 2j) and whether that
+// event fell (bit 2j+1).
 struct alignas(16) GateRec {
   uint32_t in_net[4];
   uint32_t out_net;
@@ -46,11 +47,15 @@ struct alignas(16) GateRec {
 };
 static_assert(sizeof(GateRec) == 32, "GateRec layout");
 
-// Delay-table entries of a gate with k inputs, and the index of one entry.
+// Delay-table entries of a gate with k inputs, and the index of one entry
+// (mask: inputs with an event; fell: the subset of them that fell).
 inline constexpr uint32_t dt_size(uint32_t k) { return 4u << (2 * k); }
 __host__ __device__ inline uint32_t dt_index(uint32_t k, uint32_t s, uint32_t mask,
                                              uint32_t fell) {
-  return ((s - 1) << (2 * k)) | (mask << k) | fell;
+  uint32_t bits = 0;
+  for (uint32_t j = 0; j < k; ++j)
+    bits |= (((mask >> j) & 1u) << (2 * j)) | (((fell >> j) & 1u) << (2 * j + 1));
+  return ((s - 1) << (2 * k)) | bits;
 }
 
 // Per-gate segmentation of one level: segment c covers the leader input's

@@ -175,46 +175,27 @@ struct TileCtx {
 template <int K, bool LDS>
 __device__ __forceinline__ uint32_t lut_eval(const TileCtx<K, LDS>& X, const uint8_t* s_lut,
                                              uint32_t idx) {
-  if (K <= 2) return (X.lut[0] >> (2 * idx)) & 3u;
-  if (K == 3) {
-    // 64 entries = 128 bits: pick the 64-bit half, then shift
-    const uint64_t lo = (uint64_t)X.lut[0] | ((uint64_t)X.lut[1] << 32);
-    const uint64_t hi = (uint64_t)X.lut[2] | ((uint64_t)X.lut[3] << 32);
-    const uint64_t w = (idx & 32u) ? hi : lo;
-    return (uint32_t)(w >> (2 * (idx & 31u))) & 3u;
-  }
+  // fan-in <= 2: 16 entries of 2 bits in one register; else the staged table
+  if (K <= 2) return __builtin_amdgcn_ubfe(X.lut[0], 2 * idx, 2);
   return s_lut[idx];
 }
 
-// Inputs that fell (CheckDelay == 2: to 0 from X/Z/1, or from 1 to X/Z)
-// between the packed input values p and c (2 bits per input), as a bit mask.
-__device__ __forceinline__ uint32_t fell_bits(uint32_t p, uint32_t c) {
-  const uint32_t M = 0x55u;
-  const uint32_t c0 = ~(c | (c >> 1)) & M;   // now 0
-  const uint32_t pn0 = (p | (p >> 1)) & M;   // was not 0
-  const uint32_t p1 = p & (p >> 1) & M;      // was 1
-  const uint32_t cxz = (c ^ (c >> 1)) & M;   // now X or Z
-  uint32_t x = (c0 & pn0) | (p1 & cxz);      // bit 2j per input j
-  x = (x | (x >> 1)) & 0x33u;
-  return (x | (x >> 2)) & 0x0Fu;
+// Delay-table key bits of one event (dt_index): per input j, bit 2j = it has
+// an event (from `ev`, already at even positions) and bit 2j+1 = it fell
+// (CheckDelay == 2: to 0 from X/Z/1, or from 1 to X/Z) between the packed
+// input values p and c (2 bits per input).  At bit 2j+1 of c, c << 1, p and
+// p << 1 sit the input's c1, c0, p1, p0.
+__device__ __forceinline__ uint32_t event_bits(uint32_t p, uint32_t c, uint32_t ev) {
+  const uint32_t cs = c << 1, ps = p << 1;
+  const uint32_t to0 = ~c & ~cs & (p | ps);        // now 0, was not 0
+  const uint32_t from1 = (p & ps) & (c ^ cs);      // was 1, now X or Z
+  return ((to0 | from1) & 0xAAu) | ev;
 }
 
-// Remove the top entry of a lane's stack, keeping the count of value changes.
-template <typename SP>
-__device__ __forceinline__ void stack_pop(SP st, uint32_t& sp, uint32_t& top,
-                                          uint32_t& cnt) {
-  --sp;
-  // st[0] when the stack empties: the entry just removed, in bounds, unused
-  const uint32_t below = st[sp ? sp - 1 : 0u];
-  const uint32_t nt = sp ? below : 0u;
-  cnt -= (sp == 0 || ((nt ^ top) & 3u)) ? 1u : 0u;
-  top = nt;
-}
-
-// What one lane's walk over its segment leaves behind: its stack of kept
-// candidates (st[0..sp)), the top entry, the number of value changes in the
-// stack, and the number of gate events it owned.  SP: pointer to the stack
-// (LDS or HBM).
+// What one lane's walk over its segment leaves behind: its kept candidates
+// (st[0..sp), in time order), the last of them, the number of value changes
+// among them, and the number of gate events it owned.  SP: pointer to the
+// lane's region (LDS or HBM).
 template <typename SP>
 struct Lane {
   SP st;
@@ -267,8 +248,8 @@ __device__ __forceinline__ void range_layout(const TileDesc& D, uint32_t P[4]) {
 
 // Issue the loads of tile D into LDS: its input ranges when they fit (laid
 // out by range_layout; the 16-byte copies may also bring up to 3 words
-// before and after a range, inside its slot), its delay table and (fan-in 4)
-// its truth table.  Completion: wait_vm().
+// before and after a range, inside its slot), its delay table and (fan-in
+// 3 and 4) its truth table.  Completion: wait_vm().
 __device__ __forceinline__ void stage_tile(const LevelArgs& A, const TileDesc& D, bool fits,
                                            uint32_t* s_in, uint32_t* s_lut, uint32_t* s_dt) {
   const uint32_t lane = threadIdx.x;
@@ -291,26 +272,43 @@ __device__ __forceinline__ void stage_tile(const LevelArgs& A, const TileDesc& D
   const uint32_t* dsrc = (const uint32_t*)(A.dtab + D.dt_off);
   for (uint32_t i = 0; i < nw; i += kLanes)
     if (i + lane < nw) dma_word(dsrc + i + lane, s_dt + i);
-  if (D.k == 4) dma_word((const uint32_t*)(A.luts + D.lut * 256u) + lane, s_lut);
+  if (D.k >= 3) dma_word((const uint32_t*)(A.luts + D.lut * 256u) + lane, s_lut);
 }
 
-// One lane's walk over its time segment: merge the inputs in time order,
-// evaluate, pick the delay, keep the preemption stack.
-// stk_words: size of the tile's stack buffer at stk_base.  A lane's stack
-// region runs from its offset to the next lane's; for time-ordered inputs a
-// lane never pushes more than its region holds.  Stacks in HBM are guarded
-// anyway: a push that would leave the buffer (inputs out of order)
-// overwrites the buffer's last word instead and sets bad.  LDS stacks are
-// not guarded (an LDS access outside the workgroup's allocation is
-// discarded by the hardware; all inputs are time-ordered by construction).
+// One lane's walk over its time segment, in two passes over the lane's
+// region of the stack buffer.
 //
-// Chunks: index 0 of every waveform is the net's state entry (its value
-// before the chunk's look-back window), so segment 0 starts at index 1 and
-// takes its starting values from index 0.  Candidates with output times
-// below lo_rel (made by look-back events) only pre-empt; candidates at or
-// above hi_rel belong to the next chunk and are dropped (they could only
-// pre-empt entries at or above hi_rel, and none are kept); only events at or
-// after lo_rel are counted as this chunk's gate events.
+// Pass 1 merges the inputs in time order and evaluates every event without
+// branches: 4-state truth table, edge class (CheckDelay,
+// src/simulate.cpp:1178-1201), delay (one lookup in the gate's table, which
+// folds AddDelay, src/simulate.cpp:1012-1176) and candidate output time
+// o = t + d; each candidate of an owned event is appended as (o << 2 | f).
+// Events in the halo (the next segment's first D_max) only record the
+// smallest output time they make.
+//
+// Pass 2 applies the preemption rule of src/simulate.cpp:990-1009 in its
+// closed form: a candidate is kept iff its output time is strictly below
+// that of every later candidate (those of the lane and of the halo).  It runs
+// from the last candidate to the first, keeping the running minimum,
+// compacting kept candidates towards the end of the region (in time order)
+// and counting value changes among them.  Chunks: candidates below lo_rel
+// (made by look-back events) are never kept but still pre-empt; candidates at
+// or above hi_rel belong to the next chunk and are not kept (they pre-empt
+// only entries that are not kept either).  Only events at or after lo_rel
+// count as this chunk's gate events.
+//
+// Index 0 of every waveform is the net's state entry (its value before the
+// chunk's look-back window), so segment 0 starts at index 1 and takes its
+// starting values from index 0.
+//
+// stk_words: size of the tile's stack buffer at stk_base.  A lane's region
+// runs from its offset to the next lane's and holds its owned events + 1
+// words, enough for every candidate (and, in segment 0 of chunk 0, the
+// initial (0, X) entry).  Regions in HBM are guarded anyway: a write that
+// would leave the buffer (inputs out of order) lands on the buffer's last
+// word instead and sets bad.  LDS regions are not guarded (an LDS access
+// outside the workgroup's allocation is discarded by the hardware; all
+// inputs are time-ordered by construction).
 template <int K, bool LDS, typename SP>
 __device__ __forceinline__ Lane<SP> lane_walk(const TileCtx<K, LDS>& X, uint32_t ti,
                                               const uint8_t* s_lut, const uint16_t* s_dt,
@@ -334,97 +332,98 @@ __device__ __forceinline__ Lane<SP> lane_walk(const TileCtx<K, LDS>& X, uint32_t
   constexpr bool kGuard = !std::is_same<SP, lds_u32*>::value;
   Lane<SP> ln;
   ln.st = stk_base;
-  // sp: stack depth, top: top entry, cnt: value changes in the stack
-  uint32_t sp = 0, top = 0, cnt = 0, nown = 0, hmin = INF;
-  if (c < X.nseg) {
-    // segment bounds from the leader
-    uint32_t tau_c = 0, tau_n = 0x3FFFFFFFu, halo_end = 0x3FFFFFFFu;
-    uint32_t soff = lane, idx = 0;
-    const uint32_t pl = c * X.stride;
-    if (c > 0) tau_c = srcL[pl - BL] >> 2;
-    if (c + 1 < X.nseg) {
-      tau_n = srcL[pl + X.stride - BL] >> 2;
-      halo_end = tau_n + X.dmax;
-    }
+  ln.sp = ln.top = ln.cnt = ln.nown = 0;
+  if (c >= X.nseg) return ln;
+  // segment bounds from the leader
+  uint32_t tau_c = 0, tau_n = 0x3FFFFFFFu, halo_end = 0x3FFFFFFFu;
+  uint32_t soff = lane, idx = 0;
+  const uint32_t pl = c * X.stride;
+  if (c > 0) tau_c = srcL[pl - BL] >> 2;
+  if (c + 1 < X.nseg) {
+    tau_n = srcL[pl + X.stride - BL] >> 2;
+    halo_end = tau_n + X.dmax;
+  }
+#pragma unroll
+  for (int j = 0; j < K; ++j) {
+    // pj >= 1: index 0 is the state entry, and every event time is >= 1
+    uint32_t pj;
+    if (c == 0) pj = 1;
+    else if ((uint32_t)j == X.lead) pj = pl;
+    else pj = X.B[j] + lb_words(X.src[j], bound[j], tau_c << 2);
+    soff += pj - X.L[j];
+    r[j] = pj - X.B[j];
+    idx |= (X.src[j][r[j] - 1] & 3u) << (2 * j);
+    // LDS ranges end with an INF sentinel
+    h[j] = (LDS || r[j] < bound[j]) ? X.src[j][r[j]] : INF;
+  }
+  // guarded regions: the lane's words stay below the buffer end (a lane
+  // whose offset is already past it keeps the buffer's last word)
+  const uint32_t s0 = kGuard ? min(soff, stk_words - 1) : soff;
+  const uint32_t stk_lim = stk_words - s0;
+  SP st = stk_base + s0;
+  const bool init = c == 0 && first;
+  // f before the first event: X at t = 0 (src/simulate.cpp:933), else the
+  // function of the starting values
+  uint32_t fprev = init ? kX : lut_eval<K>(X, s_lut, idx);
+  uint32_t nc = 0, nown = 0, hmin = INF;
+  if (init) {  // every output starts with (0, X) (src/simulate.cpp:882-887); t = 0 is rel 1
+    st[0] = (1u << 2) | kX;
+    nc = 1;
+  }
+  // pass 1: merge and evaluate
+  for (;;) {
+    uint32_t w = h[0];
+#pragma unroll
+    for (int j = 1; j < K; ++j) w = min(w, h[j]);
+    const uint32_t t = w >> 2;
+    if (t >= halo_end) break;  // also the end of every input (INF)
+    // every input with an event at time t
+    const uint32_t lim = w | 3u;
+    uint32_t ev = 0, nidx = idx;
 #pragma unroll
     for (int j = 0; j < K; ++j) {
-      // pj >= 1: index 0 is the state entry, and every event time is >= 1
-      uint32_t pj;
-      if (c == 0) pj = 1;
-      else if ((uint32_t)j == X.lead) pj = pl;
-      else pj = X.B[j] + lb_words(X.src[j], bound[j], tau_c << 2);
-      soff += pj - X.L[j];
-      r[j] = pj - X.B[j];
-      idx |= (X.src[j][r[j] - 1] & 3u) << (2 * j);
-      // LDS ranges end with an INF sentinel
-      h[j] = (LDS || r[j] < bound[j]) ? X.src[j][r[j]] : INF;
+      const bool take = h[j] <= lim;
+      ev |= take ? (1u << (2 * j)) : 0u;
+      nidx = take ? ((nidx & ~(3u << (2 * j))) | ((h[j] & 3u) << (2 * j))) : nidx;
+      r[j] += take ? 1u : 0u;
     }
-    // guarded stacks: the lane's entries stay below the buffer end (a lane
-    // whose offset is already past it keeps the buffer's last word)
-    const uint32_t s0 = kGuard ? min(soff, stk_words - 1) : soff;
-    const uint32_t stk_lim = stk_words - s0;
-    SP st = stk_base + s0;
-    ln.st = st;
-    const bool init = c == 0 && first;
-    // f before the first event: X at t = 0 (src/simulate.cpp:933), else the
-    // function of the starting values
-    uint32_t fprev = init ? kX : lut_eval<K>(X, s_lut, idx);
-    if (init) {  // every output starts with (0, X) (src/simulate.cpp:882-887); t = 0 is rel 1
-      top = (1u << 2) | kX;
-      st[0] = top;
-      sp = 1;
-      cnt = 1;
+#pragma unroll
+    for (int j = 0; j < K; ++j) h[j] = (LDS || r[j] < bound[j]) ? X.src[j][r[j]] : INF;
+    const uint32_t f = lut_eval<K>(X, s_lut, nidx);
+    const uint32_t sc = edge_cls(fprev, f);
+    const uint32_t key = (((sc - 1u) & 3u) << (2 * K)) | event_bits(idx, nidx, ev);
+    idx = nidx;
+    fprev = f;
+    const uint32_t o = t + s_dt[key];  // sc = 0 reads a valid entry that is not used
+    const bool own = t < tau_n;
+    nown += (own && t >= lo_rel) ? 1u : 0u;
+    const bool cand = sc != 0u;
+    if (own) {
+      const bool room = !kGuard || nc < stk_lim;
+      bad |= room ? 0u : 1u;
+      st[room ? nc : stk_lim - 1] = (o << 2) | f;
+      nc += (cand && room) ? 1u : 0u;
     }
-    for (;;) {
-      uint32_t w = h[0];
-#pragma unroll
-      for (int j = 1; j < K; ++j) w = min(w, h[j]);
-      const uint32_t t = w >> 2;
-      if (t >= halo_end) break;  // also the end of every input (INF)
-      // every input with an event at time t
-      const uint32_t lim = w | 3u;
-      uint32_t mask = 0, nidx = idx;
-#pragma unroll
-      for (int j = 0; j < K; ++j) {
-        const bool take = h[j] <= lim;
-        mask |= take ? (1u << j) : 0u;
-        nidx = take ? ((nidx & ~(3u << (2 * j))) | ((h[j] & 3u) << (2 * j))) : nidx;
-        r[j] += take ? 1u : 0u;
-      }
-#pragma unroll
-      for (int j = 0; j < K; ++j) h[j] = (LDS || r[j] < bound[j]) ? X.src[j][r[j]] : INF;
-      const uint32_t f = lut_eval<K>(X, s_lut, nidx);
-      const uint32_t sc = edge_cls(fprev, f);
-      const uint32_t fl = fell_bits(idx, nidx);
-      idx = nidx;
-      fprev = f;
-      const bool own = t < tau_n;
-      nown += (own && t >= lo_rel) ? 1u : 0u;
-      if (sc != 0) {
-        const uint32_t o = t + s_dt[dt_index(K, sc, mask, fl)];
-        if (own) {
-          if (o < hi_rel) {
-            // preemption: drop queued entries at or after o (src/simulate.cpp:991-1007)
-            while (sp > 0 && (top >> 2) >= o) stack_pop(st, sp, top, cnt);
-            if (o >= lo_rel) {
-              const uint32_t e = (o << 2) | f;
-              const bool room = !kGuard || sp < stk_lim;
-              bad |= room ? 0u : 1u;
-              cnt += ((sp == 0 || ((e ^ top) & 3u)) && room) ? 1u : 0u;
-              st[room ? sp : stk_lim - 1] = e;
-              sp += room ? 1u : 0u;
-              top = room ? e : top;
-            }
-          }
-        } else {
-          hmin = min(hmin, o);
-        }
-      }
-    }
-    // the next segment's candidates pre-empt this segment's tail
-    while (sp > 0 && (top >> 2) >= hmin) stack_pop(st, sp, top, cnt);
+    hmin = (cand && !own) ? min(hmin, o) : hmin;
   }
-  ln.sp = sp;
+  // pass 2: keep the candidates strictly earlier than every later one,
+  // compacted (in time order) to the end of the region
+  uint32_t runmin = min(hmin, hi_rel), j = nc, cnt = 0, top = 0, lastv = 4u, has = 0;
+  for (uint32_t i = nc; i-- > 0;) {
+    const uint32_t e = st[i];
+    const uint32_t o = e >> 2;
+    const bool keep = o >= lo_rel && o < runmin;
+    runmin = min(runmin, o);
+    st[keep ? j - 1 : i] = e;
+    j -= keep ? 1u : 0u;
+    const uint32_t v = e & 3u;
+    cnt += (keep && v != lastv) ? 1u : 0u;
+    lastv = keep ? v : lastv;
+    top = (keep && !has) ? e : top;
+    has |= keep ? 1u : 0u;
+  }
+  ln.st = st + j;
+  ln.sp = nc - j;
   ln.top = top;
   ln.cnt = cnt;
   ln.nown = nown;
@@ -646,7 +645,14 @@ __global__ __launch_bounds__(kLanes) void k_level(LevelArgs A) {
   unsigned long long wave_ev = 0;  // gate events of this wavefront (lane 0)
   unsigned long long wave_tiles = 0, wave_hbm = 0;
   const uint64_t ntiles = *A.ntiles;  // uniform: one scalar load
-  uint64_t tk = blockIdx.x;
+  // Tiles are handed out in batches of B consecutive tiles (the first batch
+  // of workgroup b is b's, later ones come from a ticket counter): one atomic
+  // per batch instead of per tile (a single counter serves only so many
+  // atomics per second), and consecutive tiles of one gate share its tables
+  // in L2.  Small levels use B = 1 so every workgroup gets a tile.
+  const uint64_t B = max(1ull, min(8ull, (unsigned long long)(ntiles / (16ull * gridDim.x))));
+  uint64_t tk = (uint64_t)blockIdx.x * B;
+  uint64_t bend = min(tk + B, ntiles);  // end of the current batch
   bool fits = false;
   uint32_t dw = 0;  // descriptor of tile tk, one dword per lane
   if (tk < ntiles) {
@@ -671,13 +677,18 @@ __global__ __launch_bounds__(kLanes) void k_level(LevelArgs A) {
     // the emit's scalars, loaded now so the walk covers their latency
     uint32_t* dst = A.scratch + tile_scratch(A.scr_base, D);
     const unsigned long long w_start = A.cwin[D.job].lo, w_end = A.cwin[D.job].hi;
-    // the next tile comes from a ticket (dynamic schedule: workgroups that
-    // drew slow tiles, e.g. ones too long for LDS, take fewer, so the level's
-    // tail stays short); its descriptor load completes during the walk
-    uint64_t tn = 0;
-    if (lane == 0) tn = gridDim.x + atomicAdd(A.ticket, 1ull);
-    tn = ((uint64_t)__builtin_amdgcn_readfirstlane((uint32_t)(tn >> 32)) << 32) |
-         (uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)tn);
+    // the next tile: the next of this batch, or the first of a batch drawn
+    // from the ticket counter (dynamic schedule: workgroups that drew slow
+    // tiles, e.g. ones too long for LDS, take fewer, so the level's tail stays
+    // short); its descriptor load completes during the walk
+    uint64_t tn = tk + 1, bend_n = bend;
+    if (tn >= bend) {
+      uint64_t t0 = 0;
+      if (lane == 0) t0 = (uint64_t)gridDim.x * B + atomicAdd(A.ticket, (unsigned long long)B);
+      tn = ((uint64_t)__builtin_amdgcn_readfirstlane((uint32_t)(t0 >> 32)) << 32) |
+           (uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)t0);
+      bend_n = min(tn + B, ntiles);
+    }
     const uint32_t dw_n = tn < ntiles ? desc_word(A.tiles + tn) : 0u;
     bool fits_n = false;
     auto finish = [&](const auto& ln) {
@@ -706,6 +717,7 @@ __global__ __launch_bounds__(kLanes) void k_level(LevelArgs A) {
     ++wave_tiles;
     wave_hbm += fits ? 0u : 1u;
     tk = tn;
+    bend = bend_n;
     fits = fits_n;
     dw = dw_n;
   }
