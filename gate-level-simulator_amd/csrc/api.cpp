@@ -16,6 +16,7 @@
 #include "engine.h"
 #include "formats.h"
 #include "gen.h"
+#include "interdat.h"
 #include "server.h"
 
 using namespace gls;
@@ -26,8 +27,9 @@ struct gls_design {
   Library lib;
   Netlist nl;
   std::string vlib_text, gv_text, sdf_text;  // empty when generated
-  // compiled programs by stimulus signal list (the VCD decides which nets
-  // are driven from outside)
+  // compiled programs by the set of netlist nets the stimulus drives (the
+  // VCD decides which nets are driven from outside; its other signals and
+  // their order do not change the program)
   std::mutex mu;
   std::map<std::vector<std::string>, std::shared_ptr<const Program>> progs;
 };
@@ -49,12 +51,22 @@ struct gls_run {
 };
 
 namespace gls {
+static std::vector<std::string> program_key(const Netlist& nl, const std::vector<std::string>& names) {
+  std::vector<std::string> k;
+  for (auto& n : names)
+    if (nl.net(n) >= 0) k.push_back(n);
+  std::sort(k.begin(), k.end());
+  k.erase(std::unique(k.begin(), k.end()), k.end());
+  return k;
+}
+
 std::shared_ptr<const Program> design_program(gls_design* d, const std::vector<std::string>& names) {
+  auto key = program_key(d->nl, names);
   std::lock_guard<std::mutex> lk(d->mu);
-  auto it = d->progs.find(names);
+  auto it = d->progs.find(key);
   if (it != d->progs.end()) return it->second;
   auto pg = std::make_shared<const Program>(compile_program(d->nl, d->lib, names));
-  d->progs.emplace(names, pg);
+  d->progs.emplace(std::move(key), pg);
   return pg;
 }
 
@@ -193,23 +205,27 @@ static void ensure_texts(const gls_design* d, std::string& v, std::string& g, st
   }
 }
 
+static std::vector<std::string> declared_inputs(const Netlist& nl) {
+  std::vector<std::string> r;
+  for (size_t n = 0; n < nl.net_names.size(); ++n)
+    if (nl.net_is_input[n]) r.push_back(nl.net_names[n]);
+  return r;
+}
+
+// inter.dat: the `inter` archive of the reference's process step, then the
+// library, netlist, program (compiled against the declared inputs) and
+// source texts (src/process.cpp:604-606 writes, src/simulate.cu:1835-1838
+// reads).
 int gls_design_save(const gls_design* d, const char* path) {
   API_BEGIN
   if (!d || !path) throw GlsError(GLS_E_ARG, "null argument");
-  std::string v, g, s;
-  ensure_texts(d, v, g, s);
-  std::string o = "GLS-INTER 1\n";
-  o += "vlib " + std::to_string(v.size()) + "\n" + v + "\n";
-  o += "gv " + std::to_string(g.size()) + "\n" + g + "\n";
-  o += "sdf " + std::to_string(s.size()) + "\n" + s + "\n";
-  write_file(path, o);
+  auto pg = design_program(const_cast<gls_design*>(d), declared_inputs(d->nl));
+  DesignTexts t{d->vlib_text, d->gv_text, d->sdf_text};
+  write_file(path, inter_file_text(d->lib, d->nl, *pg, t));
   API_END
 }
 
-int gls_design_load(const char* path, gls_design** out) {
-  API_BEGIN
-  if (!path || !out) throw GlsError(GLS_E_ARG, "null argument");
-  std::string a = read_file(path);
+static int load_text_sections(const std::string& a, gls_design** out) {
   size_t p = 0;
   auto line = [&]() {
     size_t q = a.find('\n', p);
@@ -218,7 +234,7 @@ int gls_design_load(const char* path, gls_design** out) {
     p = q + 1;
     return r;
   };
-  if (line() != "GLS-INTER 1") throw GlsError(GLS_E_FORMAT, "inter.dat: bad magic");
+  line();
   std::string parts[3];
   const char* keys[3] = {"vlib", "gv", "sdf"};
   for (int i = 0; i < 3; ++i) {
@@ -231,8 +247,26 @@ int gls_design_load(const char* path, gls_design** out) {
     parts[i] = a.substr(p, n);
     p += n + 1;
   }
-  int rc = gls_design_from_texts(parts[0].c_str(), parts[1].c_str(), parts[2].c_str(), out);
-  if (rc != GLS_OK) return rc;
+  return gls_design_from_texts(parts[0].c_str(), parts[1].c_str(), parts[2].c_str(), out);
+}
+
+int gls_design_load(const char* path, gls_design** out) {
+  API_BEGIN
+  if (!path || !out) throw GlsError(GLS_E_ARG, "null argument");
+  std::string a = read_file(path);
+  // files of the earlier layout: the three source texts
+  if (a.compare(0, 12, "GLS-INTER 1\n") == 0) return load_text_sections(a, out);
+  auto d = std::make_unique<gls_design>();
+  Program pg;
+  DesignTexts t;
+  if (!read_inter_section(a, d->lib, d->nl, pg, t))
+    throw GlsError(GLS_E_FORMAT, "inter.dat: no compiled section");
+  d->vlib_text = std::move(t.vlib);
+  d->gv_text = std::move(t.gv);
+  d->sdf_text = std::move(t.sdf);
+  d->progs.emplace(program_key(d->nl, declared_inputs(d->nl)),
+                   std::make_shared<const Program>(std::move(pg)));
+  *out = d.release();
   API_END
 }
 
@@ -247,17 +281,10 @@ int gls_design_write(const gls_design* d, const char* vlib, const char* gv, cons
   API_END
 }
 
-static std::vector<std::string> declared_inputs(const Netlist& nl) {
-  std::vector<std::string> r;
-  for (size_t n = 0; n < nl.net_names.size(); ++n)
-    if (nl.net_is_input[n]) r.push_back(nl.net_names[n]);
-  return r;
-}
-
 int gls_design_info_get(const gls_design* d, gls_design_info* info) {
   API_BEGIN
   if (!d || !info) throw GlsError(GLS_E_ARG, "null argument");
-  Program pg = compile_program(d->nl, d->lib, declared_inputs(d->nl));
+  const Program& pg = *design_program(const_cast<gls_design*>(d), declared_inputs(d->nl));
   info->n_instances = d->nl.insts.size();
   info->n_nets = d->nl.net_names.size();
   info->n_inputs = declared_inputs(d->nl).size();

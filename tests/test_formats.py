@@ -56,6 +56,40 @@ def test_roundtrip_files_and_inter(tmp_path):
     assert s.info() == s2.info()
 
 
+def test_inter_archive_and_compiled_section(tmp_path):
+    # inter.dat opens with the `inter` object in text-archive syntax (the
+    # reference's process step writes it, src/process.cpp:604-606), then the
+    # compiled section; loading takes the program from that section
+    # (program digest equal without compiling again), a damaged section is
+    # GLS_E_FORMAT, and files of the three-text layout still load
+    vlib, gv, sdf, _ = kat_cases.texts(kat_cases.CASES[0])
+    d = gls.Design.from_texts(vlib, gv, sdf)
+    p = str(tmp_path / "inter.dat")
+    d.save_inter(p)
+    raw = open(p, "rb").read()
+    assert raw.startswith(b"22 serialization::archive 17 ")
+    at = raw.index(b"\nGLS-CSR 1\n")
+    text = raw[:at].decode()
+    assert " 2 U0 " in text and " 10 GEN_INV_D1 " in text and " 3 top " in text
+    assert raw.endswith(b"\nGLS-CSR-AT %020d\n" % at)
+    d2 = gls.Design.load_inter(p)
+    assert d2.info() == d.info() and d2.digest() == d.digest()
+    bad = str(tmp_path / "bad.dat")
+    open(bad, "wb").write(raw[:at + 40] + raw[at + 41:])
+    with pytest.raises(gls.GlsError) as e:
+        gls.Design.load_inter(bad)
+    assert e.value.status == "GLS_E_FORMAT"
+    open(bad, "wb").write(raw[:at])
+    with pytest.raises(gls.GlsError) as e:
+        gls.Design.load_inter(bad)
+    assert e.value.status == "GLS_E_FORMAT"
+    old = str(tmp_path / "old.dat")
+    body = "GLS-INTER 1\n" + "".join("%s %d\n%s\n" % (k, len(t), t)
+                                      for k, t in (("vlib", vlib), ("gv", gv), ("sdf", sdf)))
+    open(old, "w").write(body)
+    assert gls.Design.load_inter(old).info() == d.info()
+
+
 def test_kat_designs_parse(tmp_path):
     for case in kat_cases.CASES:
         vlib, gv, sdf, vcd = kat_cases.texts(case)
@@ -97,7 +131,7 @@ def test_malformed_vcd(tmp_path):
     with pytest.raises(gls.GlsError) as e:
         gls.Stimulus.from_vcd(str(p))
     assert e.value.status == "GLS_E_FORMAT"
-    p.write_text("$timescale 1ps $end\n$enddefinitions $end\n#2000000000\n")
+    p.write_text("$timescale 1ps $end\n$enddefinitions $end\n#5000000000\n")  # past 2^32 - 1
     with pytest.raises(gls.GlsError) as e:
         gls.Stimulus.from_vcd(str(p))
     assert e.value.status == "GLS_E_RANGE"

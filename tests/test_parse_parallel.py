@@ -232,12 +232,12 @@ def test_compile_same_on_one_thread(texts, tmp_path):
 
 
 def test_inter_roundtrip_keeps_the_generated_design(tmp_path):
-    """Generator -> inter.dat writer -> threaded readers: the same netlist and
-    program, bit for bit, as the generator's own."""
+    """Generator -> inter.dat writer -> loader: the same netlist and program,
+    bit for bit, as the generator's own."""
     d = gls.Design.generate(1, 0x5EED0001, 200000, 40, 1024)
     p = str(tmp_path / "inter.dat")
     d.save_inter(p)
-    assert os.path.getsize(p) > (16 << 20)  # gv and SDF both over the parallel threshold
+    assert os.path.getsize(p) > (16 << 20)
     d2 = gls.Design.load_inter(p)
     assert d2.digest() == d.digest()
     assert d2.info() == d.info()
