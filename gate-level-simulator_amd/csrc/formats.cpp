@@ -398,7 +398,7 @@ Netlist parse_gv_text(const std::string& text, const Library& lib, int threads) 
   const size_t n_stmt = first.size() - 1;
   // 2 + 3, in rounds that bound the memory held by tokenised statements
   threads = host_threads(threads);
-  if (text.size() < (8u << 20)) threads = 1;
+  if (text.size() < parallel_min_bytes()) threads = 1;
   int hidden_const = 0;
   auto const_net = [&](const std::string& v) {
     std::string nm = "gv_net_const" + std::to_string(hidden_const++);
@@ -613,7 +613,7 @@ void parse_sdf_text(const std::string& text, Netlist& nl, int threads) {
   for (size_t i = 0; i < nl.insts.size(); ++i) by_name[nl.insts[i].name] = (int)i;
   const size_t n = text.size();
   threads = host_threads(threads);
-  if (n < (8u << 20)) threads = 1;
+  if (n < parallel_min_bytes()) threads = 1;
   // stretch starts: the line after a line that trims to ")"
   std::vector<size_t> cut{0};
   for (int i = 1; i < threads; ++i) {
@@ -1038,7 +1038,7 @@ Stimulus parse_vcd_text(const std::string& s, int threads) {
   VcdHeader H = parse_vcd_header(s, st);
   const size_t a = H.body, n = s.size();
   threads = host_threads(threads);
-  if (n - a < (8u << 20) || s.find("$comment", a) != std::string::npos) threads = 1;
+  if (n - a < parallel_min_bytes() || s.find("$comment", a) != std::string::npos) threads = 1;
   std::vector<size_t> cut{a};
   for (int i = 1; i < threads; ++i) {
     size_t target = a + (n - a) / (size_t)threads * (size_t)i;

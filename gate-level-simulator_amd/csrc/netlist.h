@@ -28,6 +28,17 @@ inline int host_threads(int requested) {
   return (int)std::min<unsigned>(16, std::max(1u, std::thread::hardware_concurrency()));
 }
 
+// Texts below this size are read on one thread: 8 MiB, or the
+// GLS_PAR_MIN_BYTES environment variable when set to a positive count (the
+// sanitizer checks lower it to run the parallel readers on small texts).
+inline size_t parallel_min_bytes() {
+  if (const char* e = std::getenv("GLS_PAR_MIN_BYTES")) {
+    const long long v = std::atoll(e);
+    if (v > 0) return (size_t)v;
+  }
+  return (size_t)8 << 20;
+}
+
 enum : uint8_t { V0 = 0, VX = 1, VZ = 2, V1 = 3 };
 
 // Primitive ids follow the reference's function_map

@@ -203,6 +203,38 @@ CASES = [
          # t0: a and const both have entries -> num_in=2, f=0 fall: min(20, 2)=2
          # t100: f=1 rise, num_in=1 -> a1 rise 10 -> 110
          expect={"y": [(0, X), (2, 0), (110, 3)]}),
+    # `assign p = m` (src/PreProcess.cpp:780-863): an instance input on the
+    # alias reads the waveform of the net it names, here a gate output, so
+    # this is two_levels with the alias between the cells.
+    dict(name="assign_gate_output",
+         insts=["GEN_INV_D1 U0 (.i(a), .zn(m));", "GEN_BUF_D1 U1 (.i(p), .z(y));"],
+         assigns=[("p", "m")],
+         sdf={"U0": ["(IOPATH i zn (10:10:10) (20:20:20))"], "U1": ["(IOPATH i z (5:5:5) (6:6:6))"]},
+         stim={"a": [(0, 0), (100, 1)]},
+         expect={"m": [(0, X), (10, 3), (120, 0)], "y": [(0, X), (15, 3), (126, 0)]}),
+    # alias of a stimulus net: buf_pass with the alias on the cell input
+    dict(name="assign_stimulus",
+         insts=["GEN_BUF_D1 U0 (.i(p), .z(y));"],
+         assigns=[("p", "a")],
+         sdf={"U0": ["(IOPATH i z (5:5:5) (50:50:50))"]},
+         stim={"a": [(0, 0), (100, 1), (120, 0)]},
+         expect={"y": [(0, X), (50, 0), (105, 3), (170, 0)]}),
+    # `assign k = 1'b1`: a constant net (value 1 from time 0), as const_input
+    dict(name="assign_constant",
+         insts=["GEN_AND2_D1 U0 (.a1(a), .a2(k), .z(y));"],
+         assigns=[("k", "1'b1")],
+         sdf={"U0": ["(IOPATH a1 z (10:10:10) (20:20:20))", "(IOPATH a2 z (1:1:1) (2:2:2))"]},
+         stim={"a": [(0, 0), (100, 1)]},
+         expect={"y": [(0, X), (2, 0), (110, 3)]}),
+    # the stimulus drives the alias net itself: the VCD waveform wins over
+    # the alias (precedence VCD > gate output > constant > assign,
+    # src/simulate.cpp:1617-1664)
+    dict(name="assign_overridden_by_stimulus",
+         insts=["GEN_INV_D1 U0 (.i(a), .zn(m));", "GEN_BUF_D1 U1 (.i(p), .z(y));"],
+         assigns=[("p", "m")],
+         sdf={"U0": ["(IOPATH i zn (10:10:10) (20:20:20))"], "U1": ["(IOPATH i z (5:5:5) (6:6:6))"]},
+         stim={"a": [(0, 0), (100, 1)], "p": [(0, 1), (150, 0)]},
+         expect={"m": [(0, X), (10, 3), (120, 0)], "y": [(0, X), (5, 3), (156, 0)]}),
 ]
 
 
@@ -225,10 +257,16 @@ def texts(case):
             net = part[part.index("(") + 1:part.index(")")]
             if net not in nets_in and not net.startswith("1'b"):
                 wires.add(net)
+    for left, right in case.get("assigns", []):
+        for net in (left, right):
+            if net not in nets_in and not net.startswith("1'b"):
+                wires.add(net)
     gv = "module top ( );\n"
     gv += "  input " + ", ".join(nets_in) + ";\n"
     if wires:
         gv += "  wire " + ", ".join(sorted(wires)) + ";\n"
+    for left, right in case.get("assigns", []):
+        gv += "  assign %s = %s;\n" % (left, right)
     for inst in case["insts"]:
         gv += "  " + inst + "\n"
     gv += "endmodule\n"

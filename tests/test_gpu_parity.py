@@ -42,12 +42,13 @@ def test_known_answers(ctx, case):
     d = gls.Design.from_texts(vlib, gv, sdf)
     import tempfile
     with tempfile.TemporaryDirectory() as t:
-        p = os.path.join(t, "s.vcd")
-        open(p, "w").write(vcd)
-        s = gls.Stimulus.from_vcd(p)
+        paths = kat_cases.write_case(case, t)
+        s = gls.Stimulus.from_vcd(paths[3])
         r = ctx.simulate(d, s, 0, 1000, keep_waves=True)
+        saif = oracle_bind.run(*paths, 0, 1000, t)["saif"]
     for net, want in case["expect"].items():
         assert r.waveform(net) == want, net
+    assert r.saif_text() == saif
 
 
 @pytest.mark.parametrize("seed", [1, 2, 3, 4, 5, 6])
