@@ -157,7 +157,7 @@ void compile_run(const Netlist& nl, const Library& lib, const std::vector<uint8_
     const CellType& ct = lib.types[inst.type];
     std::vector<int> dnets = inst.in_nets();
     if ((int)dnets.size() > kMaxK)
-      throw GlsError(GLS_E_UNSUPPORTED, "instance " + inst.name + " has more than 4 distinct input nets");
+      throw GlsError(GLS_E_UNSUPPORTED, "instance " + inst.name + " has more than 6 distinct input nets");
     std::vector<int> rnets(dnets.size());
     for (size_t j = 0; j < dnets.size(); ++j) {
       rnets[j] = resolve(dnets[j]);

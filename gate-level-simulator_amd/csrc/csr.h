@@ -17,7 +17,9 @@
 
 namespace gls {
 
-constexpr int kMaxK = 4;          // distinct input nets per gate on the device
+// Distinct input nets per gate: up to 6, as the reference's GPU path
+// (MAX_IN_NUM, src/simulate.cu:25).
+constexpr int kMaxK = 6;
 constexpr uint16_t kNoDelay = 0xFFFF;
 
 // Delay kinds within a gate's per-input record.
@@ -35,7 +37,7 @@ struct CompiledGate {
   int inst = -1;          // instance index
   int out_net = -1;       // net id written by this gate
   int k = 0;              // distinct inputs
-  int in_net[kMaxK] = {-1, -1, -1, -1};
+  int in_net[kMaxK] = {-1, -1, -1, -1, -1, -1};
   int lut = -1;           // index into Program::luts
   // IOPATH records per distinct input, in SDF order: (kind, rise, fall)
   struct Rec { int kind; double rise, fall; };

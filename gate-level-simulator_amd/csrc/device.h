@@ -30,26 +30,46 @@ struct alignas(16) GateDev {
 };
 static_assert(sizeof(GateDev) == 80, "GateDev layout");
 
-// Gate record of the fused level kernel, 32 bytes.  The delay of an output
-// event is one lookup in the gate's delay table (u16 entries at dtab +
-// dt_off) indexed by dt_index(): the output edge class s (1..4) and, per
-// input j, whether it has an event at this time (bit 2j) and whether that
-// event fell (bit 2j+1).
+// Gate record of the fused level path, 48 bytes.  Fan-in 1..4: the delay of
+// an output event is one lookup in the gate's delay table (u16 entries at
+// dtab + dt_off) indexed by dt_index(): the output edge class s (1..4) and,
+// per input j, whether it has an event at this time (bit 2j) and whether
+// that event fell (bit 2j+1).  Fan-in 5..6 ("wide" gates): the table at
+// dt_off holds per-input delays instead (wide_dt_index), and the truth
+// table of 4^k entries spans 4^k / 256 consecutive 256-byte blocks from
+// block `lut`.
 struct alignas(16) GateRec {
-  uint32_t in_net[4];
+  uint32_t in_net[6];
   uint32_t out_net;
   uint32_t dt_off;
-  uint16_t lut;
+  uint16_t lut;   // first 256-byte truth-table block
   uint8_t k;
   uint8_t pad0;
   uint16_t dmax;
   uint16_t job;  // instance of a batch (SAIF window)
+  uint32_t pad1[2];
 };
-static_assert(sizeof(GateRec) == 32, "GateRec layout");
+static_assert(sizeof(GateRec) == 48, "GateRec layout");
+
+// Largest fan-in of the lane-parallel walk; gates above it (up to 6
+// distinct inputs, the reference's MAX_IN_NUM, src/simulate.cu:25) are
+// walked by the wide-gate kernel.
+constexpr uint32_t kTileK = 4;
+constexpr uint32_t kWideK = 6;
 
 // Delay-table entries of a gate with k inputs, and the index of one entry
 // (mask: inputs with an event; fell: the subset of them that fell).
 inline constexpr uint32_t dt_size(uint32_t k) { return 4u << (2 * k); }
+
+// Wide gates: per input j, output edge class s (1..4) and whether the
+// input's own event fell: the smallest delay among its edge-independent
+// IOPATH and the one of its edge kind (tmin, 0xFFFF: none), and the delay
+// used when it is the only input with an event (tone).
+constexpr uint32_t kWideDtSize = kWideK * 4 * 2 * 2;
+__host__ __device__ inline uint32_t wide_dt_index(uint32_t j, uint32_t s, uint32_t fell,
+                                                  uint32_t one) {
+  return (((j * 4 + (s - 1)) * 2 + fell) << 1) | one;
+}
 __host__ __device__ inline uint32_t dt_index(uint32_t k, uint32_t s, uint32_t mask,
                                              uint32_t fell) {
   uint32_t bits = 0;
@@ -155,6 +175,28 @@ struct LevelArgs {
   uint32_t hi_rel;             // F_k+1 in chunk time (0xFFFFFFFF: last chunk): candidates
                                // at or above it belong to the next chunk and are dropped
   uint32_t first;              // 1 in chunk 0: outputs start with (0, X), f before t = 0 is X
+};
+
+// Arguments of the wide-gate kernel (gates of fan-in 5..6 of one level).
+struct WideArgs {
+  const GateRec* gates;        // the level's gates
+  const uint32_t* wide;        // program gate indices of the level's wide gates
+  uint32_t nwide;
+  uint32_t g0;                 // program index of the level's first gate
+  const uint64_t* tile_base;   // per gate of the level: its (single) tile
+  const TileDesc* tiles;
+  const uint64_t* woff;
+  const uint32_t* wlen;
+  const uint32_t* pool;
+  const uint8_t* luts;
+  const uint16_t* dtab;
+  const uint64_t* scr_base;
+  uint32_t* scratch;
+  TileOut* tout;
+  unsigned long long* ev_count;
+  unsigned int* err;
+  const ChunkWin* cwin;
+  uint32_t lo_rel, hi_rel, first;
 };
 
 // LDS bytes of one fused-level workgroup for an input capacity of `cap`

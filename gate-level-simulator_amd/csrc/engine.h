@@ -48,14 +48,20 @@ struct DevProgram {
   std::vector<GateRec> h_grec;
   std::vector<GateDev> h_gdev;
   std::vector<uint16_t> h_dtab;
-  std::vector<uint8_t> h_luts;   // 256 bytes per table
-  std::vector<uint32_t> h_lutp;  // 16 packed words per table
+  std::vector<uint8_t> h_luts;   // 256-byte blocks (a fan-in 5 / 6 table spans 4 / 16)
+  std::vector<uint32_t> h_lutp;  // 16 packed words per block
+  // wide gates (fan-in 5..6): their indices in h_grec, level by level
+  // (level l: h_wide[wide_start[l] .. wide_start[l+1]))
+  std::vector<uint32_t> h_wide;
+  std::vector<uint32_t> wide_start;
+  uint32_t max_k = 0;            // largest fan-in
   // device tables (valid once is_resident(); one allocation, d_block)
   GateRec* d_grec = nullptr;
   GateDev* d_gdev = nullptr;
   uint16_t* d_dtab = nullptr;
   uint8_t* d_luts = nullptr;
   uint32_t* d_lutp = nullptr;
+  uint32_t* d_wide = nullptr;
   std::mutex upload_mu;
   DevProgram() = default;
   DevProgram(const DevProgram&) = delete;

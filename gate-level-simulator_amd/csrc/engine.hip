@@ -64,6 +64,7 @@ hipError_t launch_compact(const TileDesc* tiles, const uint64_t* ntiles_dev, uin
                           const TailArgs& T, hipStream_t st);
 hipError_t launch_set_scal(uint64_t* scal, uint64_t bump, hipStream_t st);
 hipError_t launch_level(const LevelArgs& a, unsigned nblocks, hipStream_t st);
+hipError_t launch_wide(const WideArgs& a, hipStream_t st);
 // chunk.hip: time chunks
 hipError_t launch_stim_count(const uint64_t* store, const uint64_t* soff, const uint32_t* slen,
                              uint32_t ns, uint64_t lb, uint64_t ub, uint64_t* cnt, uint32_t* first,
@@ -171,6 +172,32 @@ static void fill_delay_table(const GateDelays& D, int k, uint16_t* tab) {
       }
 }
 
+// AddDelay for gates of fan-in 5..6, the same rules as fill_delay_table
+// kept per input (a full table would hold 4 << 12 entries): tmin and tone
+// of every input, output edge class and input edge, at wide_dt_index().
+static void fill_wide_delays(const GateDelays& D, int k, uint16_t* tab) {
+  const uint16_t NO = kNoDelay;
+  for (int j = 0; j < (int)kWideK; ++j)
+    for (uint32_t s = 1; s <= 4; ++s)
+      for (uint32_t fall = 0; fall < 2; ++fall) {
+        uint32_t m = 0xFFFF, one = 1;
+        if (j < k) {
+          const uint16_t* both = D.v[j][DK_BOTH];
+          const uint16_t* kind = D.v[j][fall ? DK_NEG : DK_POS];
+          if (both[0] != NO) m = std::min(m, sel_delay(both[0], both[1], (int)s));
+          if (kind[0] != NO) m = std::min(m, sel_delay(kind[0], kind[1], (int)s));
+          if (D.nrec[j] == 1) {
+            for (int kd = 0; kd < 3; ++kd)
+              if (D.v[j][kd][0] != NO) one = sel_delay(D.v[j][kd][0], D.v[j][kd][1], (int)s);
+          } else if (D.nrec[j] >= 2) {
+            one = kind[0] != NO ? sel_delay(kind[0], kind[1], (int)s) : 1;
+          }
+        }
+        tab[wide_dt_index((uint32_t)j, s, fall, 0)] = (uint16_t)m;
+        tab[wide_dt_index((uint32_t)j, s, fall, 1)] = (uint16_t)one;
+      }
+}
+
 // ------------------------------------------------------------- programs ---
 
 DevProgram::~DevProgram() {
@@ -179,12 +206,12 @@ DevProgram::~DevProgram() {
 
 size_t DevProgram::device_bytes() const {
   return h_grec.size() * sizeof(GateRec) + h_gdev.size() * sizeof(GateDev) +
-         h_dtab.size() * 2 + h_luts.size() + h_lutp.size() * 4;
+         h_dtab.size() * 2 + h_luts.size() + h_lutp.size() * 4 + h_wide.size() * 4;
 }
 
 bool DevProgram::is_resident() const { return resident.load(std::memory_order_acquire); }
 
-// The five tables share one device allocation and are published only after
+// The six tables share one device allocation and are published only after
 // it and every copy succeeded: a failed upload leaves the image
 // non-resident (all pointers null) and the next upload starts over.
 void DevProgram::upload(int dev, hipStream_t st) {
@@ -193,18 +220,19 @@ void DevProgram::upload(int dev, hipStream_t st) {
     if (dev != device) throw GlsError(GLS_E_ARG, "program image belongs to another device");
     return;
   }
-  const size_t sizes[5] = {h_grec.size() * sizeof(GateRec), h_gdev.size() * sizeof(GateDev),
-                           h_dtab.size() * 2, h_luts.size(), h_lutp.size() * 4};
-  const void* srcs[5] = {h_grec.data(), h_gdev.data(), h_dtab.data(), h_luts.data(), h_lutp.data()};
-  size_t off[5], total = 0;
-  for (int i = 0; i < 5; ++i) {
+  const size_t sizes[6] = {h_grec.size() * sizeof(GateRec), h_gdev.size() * sizeof(GateDev),
+                           h_dtab.size() * 2, h_luts.size(), h_lutp.size() * 4, h_wide.size() * 4};
+  const void* srcs[6] = {h_grec.data(), h_gdev.data(), h_dtab.data(), h_luts.data(), h_lutp.data(),
+                         h_wide.data()};
+  size_t off[6], total = 0;
+  for (int i = 0; i < 6; ++i) {
     off[i] = total;
     total += (std::max<size_t>(sizes[i], 1) + 255) / 256 * 256;
   }
   char* blk = nullptr;
   HC(hipMalloc((void**)&blk, total));
   try {
-    for (int i = 0; i < 5; ++i)
+    for (int i = 0; i < 6; ++i)
       if (sizes[i]) HC(hipMemcpyAsync(blk + off[i], srcs[i], sizes[i], hipMemcpyHostToDevice, st));
     HC(hipStreamSynchronize(st));
   } catch (...) {
@@ -219,7 +247,22 @@ void DevProgram::upload(int dev, hipStream_t st) {
   d_dtab = (uint16_t*)(blk + off[2]);
   d_luts = (uint8_t*)(blk + off[3]);
   d_lutp = (uint32_t*)(blk + off[4]);
+  d_wide = (uint32_t*)(blk + off[5]);
   resident.store(true, std::memory_order_release);
+}
+
+// Wide gates of an image, level by level.
+static void index_wide(DevProgram& P) {
+  P.h_wide.clear();
+  P.wide_start.assign(1, 0);
+  P.max_k = 0;
+  for (size_t l = 0; l + 1 < P.level_start.size(); ++l) {
+    for (uint32_t g = P.level_start[l]; g < P.level_start[l + 1]; ++g) {
+      P.max_k = std::max<uint32_t>(P.max_k, P.h_grec[g].k);
+      if (P.h_grec[g].k > kTileK) P.h_wide.push_back(g);
+    }
+    P.wide_start.push_back((uint32_t)P.h_wide.size());
+  }
 }
 
 std::shared_ptr<DevProgram> build_program_image(const Program& pg, uint32_t timescale_ps) {
@@ -229,20 +272,28 @@ std::shared_ptr<DevProgram> build_program_image(const Program& pg, uint32_t time
   P->n_nets = (uint32_t)pg.net_names.size();
   P->n_pins = (uint32_t)pg.n_pins;
   P->level_start = pg.level_start;
+  // truth tables: 256-byte blocks, a fan-in 5 / 6 table spans 4 / 16
+  std::vector<uint32_t> blk_of(pg.luts.size());
+  size_t nblk = 0;
+  for (size_t i = 0; i < pg.luts.size(); ++i) {
+    blk_of[i] = (uint32_t)nblk;
+    nblk += std::max<size_t>(1, (pg.luts[i].size() + 255) / 256);
+  }
+  if (nblk > 65535) throw GlsError(GLS_E_RANGE, "too many truth tables");
   P->h_grec.resize(P->n_gates);
   P->h_gdev.resize(P->n_gates);
   for (uint32_t i = 0; i < P->n_gates; ++i) {
     const CompiledGate& g = pg.gates[i];
     if (g.k < 1) throw GlsError(GLS_E_UNSUPPORTED, "gate without inputs");
-    if (g.k > 4) throw GlsError(GLS_E_UNSUPPORTED, "gate with more than 4 distinct inputs");
+    if (g.k > (int)kWideK) throw GlsError(GLS_E_UNSUPPORTED, "gate with more than 6 distinct inputs");
     GateDelays D = gate_delays(g, timescale_ps);
     GateDev& r = P->h_gdev[i];
     std::memset(&r, 0, sizeof r);
     GateRec& q = P->h_grec[i];
     std::memset(&q, 0, sizeof q);
+    for (int j = 0; j < (int)kWideK; ++j) q.in_net[j] = j < g.k ? (uint32_t)g.in_net[j] : 0u;
     for (int j = 0; j < 4; ++j) {
-      r.in_net[j] = j < g.k ? (uint32_t)g.in_net[j] : 0u;
-      q.in_net[j] = r.in_net[j];
+      r.in_net[j] = q.in_net[j];
       r.nrec[j] = D.nrec[j];
       for (int kd = 0; kd < 3; ++kd) {
         r.dly[j][kd][0] = D.v[j][kd][0];
@@ -250,24 +301,27 @@ std::shared_ptr<DevProgram> build_program_image(const Program& pg, uint32_t time
       }
     }
     r.out_net = q.out_net = (uint32_t)g.out_net;
-    r.lut = q.lut = (uint16_t)g.lut;
+    r.lut = q.lut = (uint16_t)blk_of[g.lut];
     r.k = q.k = (uint8_t)g.k;
     r.dmax = q.dmax = D.dmax;
-    if (P->h_dtab.size() + dt_size(g.k) >= (1ull << 32))
+    const uint32_t dts = g.k > (int)kTileK ? kWideDtSize : dt_size(g.k);
+    if (P->h_dtab.size() + dts >= (1ull << 32))
       throw GlsError(GLS_E_RANGE, "delay tables exceed 2^32 entries");
     q.dt_off = (uint32_t)P->h_dtab.size();
-    P->h_dtab.resize(P->h_dtab.size() + dt_size(g.k));
-    fill_delay_table(D, g.k, P->h_dtab.data() + q.dt_off);
+    P->h_dtab.resize(P->h_dtab.size() + dts);
+    if (g.k > (int)kTileK) fill_wide_delays(D, g.k, P->h_dtab.data() + q.dt_off);
+    else fill_delay_table(D, g.k, P->h_dtab.data() + q.dt_off);
   }
-  const size_t nl = std::max<size_t>(1, pg.luts.size());
-  P->h_luts.assign(nl * 256, 1);
-  P->h_lutp.assign(nl * 16, 0);
+  nblk = std::max<size_t>(1, nblk);
+  P->h_luts.assign(nblk * 256, 1);
+  P->h_lutp.assign(nblk * 16, 0);
   for (size_t i = 0; i < pg.luts.size(); ++i) {
-    std::memcpy(&P->h_luts[i * 256], pg.luts[i].data(), std::min<size_t>(256, pg.luts[i].size()));
+    std::memcpy(&P->h_luts[(size_t)blk_of[i] * 256], pg.luts[i].data(), pg.luts[i].size());
     // packed 2 bits per entry for register lookups (fan-in <= 3)
     for (size_t e = 0; e < pg.luts[i].size() && e < 256; ++e)
-      P->h_lutp[i * 16 + e / 16] |= (uint32_t)(pg.luts[i][e] & 3u) << (2 * (e % 16));
+      P->h_lutp[(size_t)blk_of[i] * 16 + e / 16] |= (uint32_t)(pg.luts[i][e] & 3u) << (2 * (e % 16));
   }
+  index_wide(*P);
   return P;
 }
 
@@ -334,10 +388,9 @@ static std::shared_ptr<DevProgram> union_image(const std::vector<std::shared_ptr
       for (uint32_t g = p.level_start[l]; g < p.level_start[l + 1]; ++g) {
         GateRec q = p.h_grec[g];
         GateDev r = p.h_gdev[g];
-        for (int j = 0; j < 4; ++j) {
+        for (int j = 0; j < (int)kWideK; ++j)
           if (j < q.k) q.in_net[j] += net_base[i];
-          r.in_net[j] = q.in_net[j];
-        }
+        for (int j = 0; j < 4; ++j) r.in_net[j] = q.in_net[j];
         q.out_net += net_base[i];
         r.out_net = q.out_net;
         q.dt_off += b.first;
@@ -350,6 +403,7 @@ static std::shared_ptr<DevProgram> union_image(const std::vector<std::shared_ptr
     U->level_start.push_back((uint32_t)U->h_grec.size());
   }
   U->n_gates = (uint32_t)U->h_grec.size();
+  index_wide(*U);
   for (size_t i = 0; i < parts.size(); ++i) {
     U->n_nets = std::max(U->n_nets, net_base[i] + parts[i]->n_nets);
     U->n_pins += parts[i]->n_pins;
@@ -534,6 +588,8 @@ void Job::prepare(const std::vector<SimInstance>& inst, uint64_t pool_words) {
     win_.push_back(in.end);
   }
   prog_ = parts.size() == 1 ? parts[0] : union_image(parts, net_base_);
+  if (mode == 1 && prog_->max_k > kTileK)
+    throw GlsError(GLS_E_UNSUPPORTED, "the two-pass path takes gates of at most 4 distinct inputs");
   prog_->upload(device_, stream_);
   n_gates_ = prog_->n_gates;
   n_nets_ = (uint32_t)nets;
@@ -929,6 +985,27 @@ int Job::run_chunk(const Chunk& c, std::vector<hipEvent_t>& ev_eval, bool device
       if (!device_driven) ensure((void**)&d_stk_, cap_stk_, w * 4);
       a.stk = d_stk_;
     }
+    WideArgs wa;
+    wa.gates = gates;
+    wa.wide = prog_->d_wide + prog_->wide_start[l];
+    wa.nwide = prog_->wide_start[l + 1] - prog_->wide_start[l];
+    wa.g0 = g0;
+    wa.tile_base = d_tilebase_;
+    wa.tiles = d_tiles_;
+    wa.woff = d_woff_;
+    wa.wlen = d_wlen_;
+    wa.pool = d_pool_;
+    wa.luts = prog_->d_luts;
+    wa.dtab = prog_->d_dtab;
+    wa.scr_base = d_scrbase_;
+    wa.scratch = scratch;
+    wa.tout = d_tout_;
+    wa.ev_count = evc;
+    wa.err = err;
+    wa.cwin = cw;
+    wa.lo_rel = c.lo_rel;
+    wa.hi_rel = c.hi_rel;
+    wa.first = c.first;
     if (!device_driven && !static_) {
       hipEvent_t e_a, e_b;
       HC(hipEventCreate(&e_a));
@@ -937,9 +1014,11 @@ int Job::run_chunk(const Chunk& c, std::vector<hipEvent_t>& ev_eval, bool device
       ev_eval.push_back(e_b);
       HC(hipEventRecord(e_a, stream_));
       HC(launch_level(a, nb, stream_));
+      HC(launch_wide(wa, stream_));
       HC(hipEventRecord(e_b, stream_));
     } else {
       HC(launch_level(a, nb, stream_));
+      HC(launch_wide(wa, stream_));
     }
     // gate counts go to d_bound_ (free again once the scratch bases are scanned)
     HC(launch_gate_carry(gates, ng, d_tilebase_, d_ntile_, d_tout_, d_toff_, d_bound_, d_wlen_,

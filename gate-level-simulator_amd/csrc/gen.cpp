@@ -71,6 +71,12 @@ const std::string& generic_vlib_text() {
       "module GEN_OAI22_D1 (a1,a2,b1,b2,zn);\n  input a1;\n  input a2;\n  input b1;\n  input b2;\n"
       "  output zn;\n  wire net0, net1, net2;\n  or (net0, a1, a2);\n  or (net1, b1, b2);\n"
       "  and (net2, net0, net1);\n  not (zn, net2);\nendmodule\n"
+      "module GEN_AO221_D1 (a1,a2,b1,b2,c,z);\n  input a1;\n  input a2;\n  input b1;\n  input b2;\n"
+      "  input c;\n  output z;\n  wire net0, net1;\n  and (net0, a1, a2);\n  and (net1, b1, b2);\n"
+      "  or (z, net0, net1, c);\nendmodule\n"
+      "module GEN_AO222_D1 (a1,a2,b1,b2,c1,c2,z);\n  input a1;\n  input a2;\n  input b1;\n  input b2;\n"
+      "  input c1;\n  input c2;\n  output z;\n  wire net0, net1, net2;\n  and (net0, c1, c2);\n"
+      "  and (net1, a1, a2);\n  and (net2, b1, b2);\n  or (z, net0, net1, net2);\nendmodule\n"
       "`endcelldefine\n";
   return txt;
 }
@@ -82,13 +88,14 @@ struct GenCell {
   const char* name;
   const char* edge_pins;  // comma separated
 };
-static const GenCell kGenCells[12] = {
+static const GenCell kGenCells[14] = {
     {"GEN_INV_D1", ""},         {"GEN_BUF_D1", ""},
     {"GEN_AND2_D1", ""},        {"GEN_OR2_D1", ""},
     {"GEN_NAND2_D1", ""},       {"GEN_NOR2_D1", ""},
     {"GEN_XOR2_D1", "a1,a2"},   {"GEN_XNOR2_D1", "a1,a2"},
     {"GEN_MUX2_D1", "s"},       {"GEN_AOI21_D1", ""},
     {"GEN_AOI22_D1", ""},       {"GEN_OAI22_D1", ""},
+    {"GEN_AO221_D1", ""},       {"GEN_AO222_D1", ""},
 };
 
 static bool is_edge_pin(const GenCell& c, const std::string& pin) {
@@ -153,7 +160,8 @@ static std::vector<int> level_widths(const NetlistSpec& sp, SplitMix64& rng) {
 
 Netlist generate_netlist(const NetlistSpec& sp, const Library& lib,
                          std::vector<int>* level_of_cell) {
-  if (sp.n_types != 10 && sp.n_types != 12) throw GlsError(-2, "n_types must be 10 or 12");
+  if (sp.n_types != 10 && sp.n_types != 12 && sp.n_types != 14)
+    throw GlsError(-2, "n_types must be 10, 12 or 14");
   if (sp.n_levels < 1 || sp.n_cells < sp.n_levels || sp.n_pis < 4)
     throw GlsError(-2, "bad netlist spec");
   SplitMix64 rng(sp.seed);
@@ -367,6 +375,10 @@ NetlistSpec config_netlist_spec(int ci) {
       s.n_cells = 2000000; s.n_pis = 4096; s.n_levels = 60; s.n_types = 12;
       s.pareto_fanout = true; s.lognormal_widths = true;
       break;
+    case 3:  // the 200k-cell shape with 5- and 6-input cells in the mix
+      s.n_cells = 200000; s.n_pis = 1024; s.n_levels = 40; s.n_types = 14;
+      s.pareto_fanout = true;
+      break;
     default:
       throw GlsError(-2, "unknown config index");
   }
@@ -380,6 +392,7 @@ StimulusSpec config_stimulus_spec(int ci) {
     case 0: s.t_end = 2000000; s.mean_gap = 1000; break;
     case 1: s.t_end = 20000000; s.mean_gap = 500; break;
     case 2: s.t_end = 20000000; s.mean_gap = 200; break;
+    case 3: s.t_end = 20000000; s.mean_gap = 500; break;
     default: throw GlsError(-2, "unknown config index");
   }
   return s;

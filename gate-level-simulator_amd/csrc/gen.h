@@ -39,7 +39,7 @@ struct NetlistSpec {
   int n_cells = 2000;
   int n_pis = 64;
   int n_levels = 20;
-  int n_types = 10;            // 10: INV..AOI21, 12: + AOI22, OAI22
+  int n_types = 10;            // 10: INV..AOI21, 12: + AOI22, OAI22, 14: + AO221, AO222
   bool pareto_fanout = false;  // source choice weighted by Pareto(1.5) weights
   int fanout_cap = 256;        // realized fanout cap when pareto_fanout
   bool lognormal_widths = false;
@@ -55,7 +55,7 @@ struct StimulusSpec {
 };
 
 // The generic cell library text (module names and pins as in the
-// reference's examples/GENERIC_STD_CELL.vlib) for the 12 generator cells.
+// reference's examples/GENERIC_STD_CELL.vlib) for the 14 generator cells.
 const std::string& generic_vlib_text();
 
 // Build a netlist (cells, nets, SDF delays) from the spec.  `lib` must hold
@@ -66,7 +66,8 @@ Netlist generate_netlist(const NetlistSpec& spec, const Library& lib,
 // Build stimulus for the netlist's primary inputs (nets declared `input`).
 Stimulus generate_stimulus(const Netlist& nl, const StimulusSpec& spec);
 
-// Named benchmark configurations (index 0..2 of the measurement plan).
+// Named benchmark configurations (index 0..2 of the measurement plan; 3:
+// configuration 1 with 5- and 6-input cells).
 NetlistSpec config_netlist_spec(int config_index);
 StimulusSpec config_stimulus_spec(int config_index);
 

@@ -192,6 +192,15 @@ __device__ __forceinline__ uint32_t event_bits(uint32_t p, uint32_t c, uint32_t 
   return ((to0 | from1) & 0xAAu) | ev;
 }
 
+// The word after index r of an input range of `bound` words: LDS ranges
+// end with an INF sentinel at index bound (read again past the end), HBM
+// ranges are guarded.
+template <bool LDS, typename P>
+__device__ __forceinline__ uint32_t next_word(P src, uint32_t r, uint32_t bound) {
+  if (LDS) return src[min(r + 1u, bound)];
+  return r + 1u < bound ? src[r + 1u] : 0xFFFFFFFFu;
+}
+
 // What one lane's walk over its segment leaves behind: its kept candidates
 // (st[0..sp), in time order), the last of them, the number of value changes
 // among them, and the number of gate events it owned.  SP: pointer to the
@@ -253,6 +262,7 @@ __device__ __forceinline__ void range_layout(const TileDesc& D, uint32_t P[4]) {
 __device__ __forceinline__ void stage_tile(const LevelArgs& A, const TileDesc& D, bool fits,
                                            uint32_t* s_in, uint32_t* s_lut, uint32_t* s_dt) {
   const uint32_t lane = threadIdx.x;
+  if (D.k > kTileK) return;  // wide gate: walked by k_wide
   if (fits) {
     uint32_t P[4];
     range_layout(D, P);
@@ -318,7 +328,7 @@ __device__ __forceinline__ Lane<SP> lane_walk(const TileCtx<K, LDS>& X, uint32_t
   const uint32_t c = ti * kLanes + lane;
   const uint32_t kX = 1u;
   const uint32_t INF = 0xFFFFFFFFu;
-  uint32_t r[K], h[K], bound[K];
+  uint32_t r[K], h[K], n1[K], bound[K];
   typename InPtr<LDS>::T srcL = X.src[0];
   uint32_t BL = X.B[0];
 #pragma unroll
@@ -355,6 +365,7 @@ __device__ __forceinline__ Lane<SP> lane_walk(const TileCtx<K, LDS>& X, uint32_t
     idx |= (X.src[j][r[j] - 1] & 3u) << (2 * j);
     // LDS ranges end with an INF sentinel
     h[j] = (LDS || r[j] < bound[j]) ? X.src[j][r[j]] : INF;
+    n1[j] = next_word<LDS>(X.src[j], r[j], bound[j]);
   }
   // guarded regions: the lane's words stay below the buffer end (a lane
   // whose offset is already past it keeps the buffer's last word)
@@ -380,15 +391,21 @@ __device__ __forceinline__ Lane<SP> lane_walk(const TileCtx<K, LDS>& X, uint32_t
     // every input with an event at time t
     const uint32_t lim = w | 3u;
     uint32_t ev = 0, nidx = idx;
+    bool take[K];
 #pragma unroll
     for (int j = 0; j < K; ++j) {
-      const bool take = h[j] <= lim;
-      ev |= take ? (1u << (2 * j)) : 0u;
-      nidx = take ? ((nidx & ~(3u << (2 * j))) | ((h[j] & 3u) << (2 * j))) : nidx;
-      r[j] += take ? 1u : 0u;
+      take[j] = h[j] <= lim;
+      ev |= take[j] ? (1u << (2 * j)) : 0u;
+      nidx = take[j] ? ((nidx & ~(3u << (2 * j))) | ((h[j] & 3u) << (2 * j))) : nidx;
+      r[j] += take[j] ? 1u : 0u;
     }
+    // the next heads come from the look-ahead registers; the look-ahead
+    // loads are issued now and waited for only when an input advances again
 #pragma unroll
-    for (int j = 0; j < K; ++j) h[j] = (LDS || r[j] < bound[j]) ? X.src[j][r[j]] : INF;
+    for (int j = 0; j < K; ++j) {
+      h[j] = take[j] ? n1[j] : h[j];
+      n1[j] = next_word<LDS>(X.src[j], r[j], bound[j]);
+    }
     const uint32_t f = lut_eval<K>(X, s_lut, nidx);
     const uint32_t sc = edge_cls(fprev, f);
     const uint32_t key = (((sc - 1u) & 3u) << (2 * K)) | event_bits(idx, nidx, ev);
@@ -663,8 +680,9 @@ __global__ __launch_bounds__(kLanes) void k_level(LevelArgs A) {
   }
   while (tk < ntiles) {
     const TileDesc D = desc_of(dw);
+    const bool wide = D.k > kTileK;  // uniform: the tile is k_wide's
     wait_vm();
-    if (fits) {  // INF sentinel after every input range
+    if (fits && !wide) {  // INF sentinel after every input range
       uint32_t P[4];
       range_layout(D, P);
 #pragma unroll
@@ -701,13 +719,14 @@ __global__ __launch_bounds__(kLanes) void k_level(LevelArgs A) {
       }
       emit_tile(A, tk, ln, T, dst, D.out_cap, w_start, w_end, wave_ev);
     };
-    switch (D.k) {
+    switch (wide ? 0u : D.k) {
+      case 0: break;
       case 1: walk_k<1>(A, D, fits, s_in, s_st, lut8, dt16, finish); break;
       case 2: walk_k<2>(A, D, fits, s_in, s_st, lut8, dt16, finish); break;
       case 3: walk_k<3>(A, D, fits, s_in, s_st, lut8, dt16, finish); break;
       default: walk_k<4>(A, D, fits, s_in, s_st, lut8, dt16, finish); break;
     }
-    if (!A.prefetch && tn < ntiles) {
+    if ((wide || !A.prefetch) && tn < ntiles) {
       __syncthreads();
       const TileDesc Dn = desc_of(dw_n);
       fits_n = tile_words(Dn) <= A.cap;
@@ -901,6 +920,149 @@ __global__ void k_compact(const TileDesc* __restrict__ tiles, const uint64_t* __
   }
 }
 
+// Wide gates (5 or 6 distinct inputs): one thread walks a gate's whole
+// chunk, with the walk's rules in sequence (the same merge, truth table,
+// edge class, AddDelay per input, preemption in closed form, chunk
+// bounds), and leaves its tile output and summary exactly as a one-tile
+// lane walk would (k_gate_carry and k_compact then treat it like any other
+// tile).  The candidates are built in the tile's output region itself
+// (it holds every input word + 1) and compacted in place.
+__global__ void k_wide(WideArgs A) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= A.nwide) return;
+  // a level over the captured capacities was skipped (no descriptors)
+  if (*A.err & ERR_CAPACITY) return;
+  const uint32_t g = A.wide[i] - A.g0;
+  const GateRec& G = A.gates[g];
+  const uint64_t tk = A.tile_base[g];
+  const uint32_t cap = A.tiles[tk].out_cap;
+  uint32_t* st = A.scratch + A.scr_base[g];
+  const uint8_t* lut = A.luts + (uint32_t)G.lut * 256u;
+  const uint16_t* tab = A.dtab + G.dt_off;
+  const uint32_t k = G.k;
+  const uint32_t kX = 1u, INF = 0xFFFFFFFFu;
+  const uint32_t* in[kWideK];
+  uint32_t n[kWideK], r[kWideK], h[kWideK];
+  uint32_t idx = 0;
+#pragma unroll
+  for (uint32_t j = 0; j < kWideK; ++j) {
+    if (j < k) {
+      in[j] = A.pool + A.woff[G.in_net[j]];
+      n[j] = A.wlen[G.in_net[j]];
+      idx |= (in[j][0] & 3u) << (2 * j);  // the state entry
+      r[j] = 1;
+      h[j] = n[j] > 1 ? in[j][1] : INF;
+    } else {
+      in[j] = nullptr;
+      n[j] = r[j] = 0;
+      h[j] = INF;
+    }
+  }
+  const bool init = A.first != 0;
+  uint32_t fprev = init ? kX : lut[idx];
+  uint32_t nc = 0, nown = 0, bad = 0;
+  if (init) st[nc++] = (1u << 2) | kX;  // (0, X) at rel 1; cap >= 1
+  for (;;) {
+    uint32_t w = h[0];
+#pragma unroll
+    for (uint32_t j = 1; j < kWideK; ++j) w = min(w, h[j]);
+    const uint32_t t = w >> 2;
+    if (t >= 0x3FFFFFFFu) break;  // every input at its end
+    const uint32_t lim = w | 3u;
+    uint32_t mask = 0, nidx = idx;
+#pragma unroll
+    for (uint32_t j = 0; j < kWideK; ++j) {
+      if (h[j] <= lim) {
+        mask |= 1u << j;
+        nidx = (nidx & ~(3u << (2 * j))) | ((h[j] & 3u) << (2 * j));
+        ++r[j];
+        h[j] = r[j] < n[j] ? in[j][r[j]] : INF;
+      }
+    }
+    const uint32_t f = lut[nidx];
+    const uint32_t sc = edge_cls(fprev, f);
+    if (sc != 0u) {
+      uint32_t d;
+      if ((mask & (mask - 1u)) == 0u) {
+        const uint32_t j = __builtin_ctz(mask);
+        const uint32_t p = (idx >> (2 * j)) & 3u, c = (nidx >> (2 * j)) & 3u;
+        const uint32_t fell = ((c == 0u && p != 0u) || (p == 3u && (c == 1u || c == 2u))) ? 1u : 0u;
+        d = tab[wide_dt_index(j, sc, fell, 1)];
+      } else {
+        uint32_t m = 0xFFFFu;
+#pragma unroll
+        for (uint32_t j = 0; j < kWideK; ++j) {
+          if (!((mask >> j) & 1u)) continue;
+          const uint32_t p = (idx >> (2 * j)) & 3u, c = (nidx >> (2 * j)) & 3u;
+          const uint32_t fell = ((c == 0u && p != 0u) || (p == 3u && (c == 1u || c == 2u))) ? 1u : 0u;
+          m = min(m, (uint32_t)tab[wide_dt_index(j, sc, fell, 0)]);
+        }
+        d = m == 0xFFFFu ? 1u : m;
+      }
+      if (nc < cap) st[nc++] = ((t + d) << 2) | f;
+      else bad = 1;
+    }
+    nown += t >= A.lo_rel ? 1u : 0u;
+    idx = nidx;
+    fprev = f;
+  }
+  // keep the candidates strictly earlier than every later one (and inside
+  // [lo_rel, hi_rel)), compacted in time order to the end of the list
+  uint32_t runmin = A.hi_rel, jj = nc, cnt = 0, top = 0, lastv = 4u, has = 0;
+  for (uint32_t q = nc; q-- > 0;) {
+    const uint32_t e = st[q];
+    const uint32_t o = e >> 2;
+    const bool keep = o >= A.lo_rel && o < runmin;
+    runmin = min(runmin, o);
+    if (keep) {
+      st[--jj] = e;
+      cnt += (e & 3u) != lastv ? 1u : 0u;
+      lastv = e & 3u;
+      if (!has) top = e;
+      has = 1;
+    }
+  }
+  // entries (deduplicated) to the front of the region, SAIF durations
+  // between them, summary
+  const ChunkWin W = A.cwin[G.job];
+  const uint32_t s32 = (uint32_t)min(W.lo, 0x7FFFFFFFull);
+  const uint32_t e32 = (uint32_t)min(W.hi, 0x7FFFFFFFull);
+  const uint32_t sp = nc - jj;
+  const uint32_t e_first = sp ? st[jj] : 0u;
+  uint32_t pvv = 4u, pct = max(min(e_first >> 2, e32), s32);
+  const uint32_t ct0 = pct;
+  uint32_t a0 = 0, a1 = 0, wpos = 0;
+  for (uint32_t q = 0; q < sp; ++q) {
+    const uint32_t e = st[jj + q];
+    const uint32_t v = e & 3u;
+    const uint32_t ct = max(min(e >> 2, e32), s32);
+    const uint32_t len = ct - pct;
+    a0 += pvv == 0u ? len : 0u;
+    a1 += pvv == 3u ? len : 0u;
+    if (v != pvv) st[wpos++] = e;  // wpos <= q <= jj + q: read before written
+    pvv = v;
+    pct = ct;
+  }
+  TileOut o;
+  o.cnt = min(cnt, cap);
+  o.ft = sp ? e_first >> 2 : 0u;
+  o.lt = has ? top >> 2 : 0u;
+  o.flags = has ? ((e_first & 3u) | ((top & 3u) << 2) | (1u << 4)) : 0u;
+  o.t0 = a0;
+  o.t1 = a1;
+  o.tx = pct - ct0 - a0 - a1;
+  o.tc = 0;
+  A.tout[tk] = o;
+  if (cnt > cap || bad) atomicOr(A.err, ERR_ORDER);
+  if (nown) atomicAdd(A.ev_count, (unsigned long long)nown);
+}
+
+hipError_t launch_wide(const WideArgs& a, hipStream_t st) {
+  if (a.nwide == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_wide, dim3((a.nwide + 63) / 64), dim3(64), 0, st, a);
+  return hipGetLastError();
+}
+
 // Start-of-run scalars: bump pointer past the stimulus words, all else 0.
 __global__ void k_set_scal(uint64_t* scal, uint64_t bump) {
   if (blockIdx.x == 0 && threadIdx.x < 16) scal[threadIdx.x] = threadIdx.x == 0 ? bump : 0;
@@ -927,6 +1089,17 @@ __global__ void k_plan2(const GateRec* __restrict__ gates, uint32_t ng,
   }
   // a tile's offset inside its gate's waveform travels in 31 bits
   if (N + 1 >= (1ull << 30)) atomicOr(err, ERR_RANGE);
+  if (G.k > kTileK) {  // wide gate: one tile, walked by k_wide
+    GatePlan P;
+    P.nseg = 1;
+    P.stride = nl > 0 ? nl : 1u;
+    P.lead = lead;
+    P.pad = 0;
+    plan[g] = P;
+    ntile[g] = 1;
+    bound[g] = N + 2 + tail_len[G.out_net];
+    return;
+  }
   // leader events per segment so that a segment holds ~seg_target input
   // events; then the gate's tiles are filled evenly (the stride that spreads
   // the leader's events over all 64 segments of each of those tiles, never
@@ -994,7 +1167,15 @@ __global__ void k_tile_desc(const GateRec* __restrict__ gates, uint32_t ng,
   // over the tile's four threads: its output region in the level scratch
   // holds that many entries plus one
   uint32_t span = 0;
-  if (j >= G.k) {
+  if (G.k > kTileK) {
+    // wide gate (one tile): no staged ranges; the output region holds all
+    // input words + 1 (threads 0 and 1 also count inputs 4 and 5)
+    D.L[j] = 0;
+    D.E[j] = 0;
+    D.in_off[j] = 0;
+    span = wlen[G.in_net[j]];
+    if ((uint32_t)j + 4 < G.k) span += wlen[G.in_net[j + 4]];
+  } else if (j >= G.k) {
     D.L[j] = 0;
     D.E[j] = 0;
     D.in_off[j] = 0;

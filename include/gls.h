@@ -50,12 +50,18 @@ int gls_design_from_texts_threads(const char* vlib_text, const char* gv_text,
 /* 64-bit digest of the parsed design: nets and their flags, instances with
  * their pins and every delay bit for bit.  Equal digests simulate alike. */
 int gls_design_digest(const gls_design* d, uint64_t* digest);
-/* Seeded synthetic netlist of one benchmark configuration (0, 1, 2) with the
- * generic cell library; n_cells/n_levels/n_pis override when > 0. */
+/* Seeded synthetic netlist of one benchmark configuration (0, 1, 2; 3 is
+ * configuration 1 with the 5- and 6-input cells GEN_AO221_D1 and
+ * GEN_AO222_D1 in the mix) with the generic cell library;
+ * n_cells/n_levels/n_pis override when > 0. */
 int gls_design_generate(int config, uint64_t seed, int n_cells, int n_levels, int n_pis,
                         gls_design** out);
-/* inter.dat writer / reader — replaces the Boost archive hand-off between
- * `process` and `simulate` (src/process.cpp:581-608, src/simulate.cu:1835-1838). */
+/* inter.dat writer / reader: the hand-off between `process` and `simulate`
+ * (src/process.cpp:581-608 writes the Boost text archive of `inter`,
+ * src/simulate.cu:1835-1838 reads it).  The file holds that object in
+ * text-archive syntax and member order, then the library, netlist and
+ * levelized program, which the reader loads without parsing again.  Files of
+ * the earlier three-text layout ("GLS-INTER 1") are still read. */
 int gls_design_save(const gls_design* d, const char* inter_path);
 int gls_design_load(const char* inter_path, gls_design** out);
 /* Write the design as vlib / gv / SDF text files (generator output). */

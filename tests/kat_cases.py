@@ -63,6 +63,32 @@ module GEN_TIE_D1 (a,z);
   output z;
   and (z, a, 1'b1);
 endmodule
+module GEN_AO221_D1 (a1,a2,b1,b2,c,z);
+  input a1;
+  input a2;
+  input b1;
+  input b2;
+  input c;
+  output z;
+  wire net0, net1;
+  and (net0, a1, a2);
+  and (net1, b1, b2);
+  or (z, net0, net1, c);
+endmodule
+module GEN_AO222_D1 (a1,a2,b1,b2,c1,c2,z);
+  input a1;
+  input a2;
+  input b1;
+  input b2;
+  input c1;
+  input c2;
+  output z;
+  wire net0, net1, net2;
+  and (net0, c1, c2);
+  and (net1, a1, a2);
+  and (net2, b1, b2);
+  or (z, net0, net1, net2);
+endmodule
 module GEN_HA_D1 (a,b,s,co);
   input a;
   input b;
@@ -203,6 +229,44 @@ CASES = [
          # t0: a and const both have entries -> num_in=2, f=0 fall: min(20, 2)=2
          # t100: f=1 rise, num_in=1 -> a1 rise 10 -> 110
          expect={"y": [(0, X), (2, 0), (110, 3)]}),
+    # AO221 (5 distinct inputs, beyond the lane walk's fan-in of 4): the
+    # 3-input OR of two ANDs and c.
+    dict(name="ao221_five_inputs",
+         insts=["GEN_AO221_D1 U0 (.a1(a), .a2(b), .b1(c), .b2(d), .c(e), .z(y));"],
+         sdf={"U0": ["(IOPATH a1 z (10:10:10) (11:11:11))", "(IOPATH a2 z (12:12:12) (13:13:13))",
+                     "(IOPATH b1 z (14:14:14) (15:15:15))", "(IOPATH b2 z (16:16:16) (17:17:17))",
+                     "(IOPATH c z (18:18:18) (19:19:19))"]},
+         stim={"a": [(0, 0), (200, 1), (500, 0)], "b": [(0, 0), (300, 1)],
+               "c": [(0, 0), (600, "x")], "d": [(0, 0), (700, 1)], "e": [(0, 0), (100, 1), (400, 0)]},
+         # t0: all five inputs, z = 0 (fall), num_in = 5 -> min of the falls = 11.
+         # t100: e -> 1: z rises via c 18 -> 118.  t200, t300: a1, a2 -> 1: z stays 1.
+         # t400: e -> 0: z = a1 & a2 = 1, no event.  t500: a1 -> 0: z falls via a1 11 -> 511.
+         # t600: b1 -> X with b2 = 0: z stays 0.  t700: b2 -> 1: z = X, 0 -> X is a
+         # rise, via b2 16 -> 716.
+         expect={"y": [(0, X), (11, 0), (118, 3), (511, 0), (716, X)]}),
+    # AO222 (6 distinct inputs) with edge-specific IOPATHs on c1, events on
+    # two inputs at once, and a pre-empted candidate.
+    dict(name="ao222_six_inputs",
+         insts=["GEN_AO222_D1 U0 (.a1(a), .a2(b), .b1(c), .b2(d), .c1(e), .c2(f), .z(y));"],
+         sdf={"U0": ["(IOPATH a1 z (10:10:10) (20:20:20))", "(IOPATH a2 z (11:11:11) (21:21:21))",
+                     "(IOPATH b1 z (12:12:12) (22:22:22))", "(IOPATH b2 z (13:13:13) (23:23:23))",
+                     "(IOPATH (posedge c1) z (5:5:5) (6:6:6))",
+                     "(IOPATH (negedge c1) z (7:7:7) (8:8:8))",
+                     "(IOPATH c2 z (14:14:14) (24:24:24))"]},
+         stim={"a": [(0, 0), (400, 1)], "b": [(0, 0), (400, 1), (405, 0)],
+               "c": [(0, 0), (500, 1), (515, 0)], "d": [(0, 0), (510, 1)],
+               "e": [(0, 0), (100, 1), (200, 0), (300, 1), (601, 0)],
+               "f": [(0, 0), (100, 1), (302, 0), (600, 1)]},
+         # t0: all six inputs, z = 0 (fall); c1 X -> 0 is a negedge: fall 8;
+         # others 20..24 -> 8.  t100: c1, c2 -> 1: rise, min(posedge c1 5, c2 14)
+         # -> 105.  t200: c1 -> 0: fall, negedge c1 8 -> 208.  t300: c1 -> 1:
+         # rise 5 -> 305.  t302: c2 -> 0: fall 24 -> 326.  t400: a1, a2 -> 1:
+         # rise min(10, 11) -> 410.  t405: a2 -> 0: fall 21 -> 426.  t510: b2
+         # -> 1 (b1 = 1): rise 13 -> 523.  t515: b1 -> 0: fall 22 -> 537.
+         # t600: c2 -> 1 (c1 = 1): rise 14 -> 614; t601: c1 -> 0: fall 8 ->
+         # 609 pre-empts 614, and 0 repeats the value from 537: no change.
+         expect={"y": [(0, X), (8, 0), (105, 3), (208, 0), (305, 3), (326, 0), (410, 3), (426, 0),
+                       (523, 3), (537, 0)]}),
     # `assign p = m` (src/PreProcess.cpp:780-863): an instance input on the
     # alias reads the waveform of the net it names, here a gate output, so
     # this is two_levels with the alias between the cells.
