@@ -192,15 +192,6 @@ __device__ __forceinline__ uint32_t event_bits(uint32_t p, uint32_t c, uint32_t 
   return ((to0 | from1) & 0xAAu) | ev;
 }
 
-// The word after index r of an input range of `bound` words: LDS ranges
-// end with an INF sentinel at index bound (read again past the end), HBM
-// ranges are guarded.
-template <bool LDS, typename P>
-__device__ __forceinline__ uint32_t next_word(P src, uint32_t r, uint32_t bound) {
-  if (LDS) return src[min(r + 1u, bound)];
-  return r + 1u < bound ? src[r + 1u] : 0xFFFFFFFFu;
-}
-
 // What one lane's walk over its segment leaves behind: its kept candidates
 // (st[0..sp), in time order), the last of them, the number of value changes
 // among them, and the number of gate events it owned.  SP: pointer to the
@@ -328,7 +319,7 @@ __device__ __forceinline__ Lane<SP> lane_walk(const TileCtx<K, LDS>& X, uint32_t
   const uint32_t c = ti * kLanes + lane;
   const uint32_t kX = 1u;
   const uint32_t INF = 0xFFFFFFFFu;
-  uint32_t r[K], h[K], n1[K], bound[K];
+  uint32_t r[K], h[K], bound[K];
   typename InPtr<LDS>::T srcL = X.src[0];
   uint32_t BL = X.B[0];
 #pragma unroll
@@ -365,7 +356,6 @@ __device__ __forceinline__ Lane<SP> lane_walk(const TileCtx<K, LDS>& X, uint32_t
     idx |= (X.src[j][r[j] - 1] & 3u) << (2 * j);
     // LDS ranges end with an INF sentinel
     h[j] = (LDS || r[j] < bound[j]) ? X.src[j][r[j]] : INF;
-    n1[j] = next_word<LDS>(X.src[j], r[j], bound[j]);
   }
   // guarded regions: the lane's words stay below the buffer end (a lane
   // whose offset is already past it keeps the buffer's last word)
@@ -391,21 +381,15 @@ __device__ __forceinline__ Lane<SP> lane_walk(const TileCtx<K, LDS>& X, uint32_t
     // every input with an event at time t
     const uint32_t lim = w | 3u;
     uint32_t ev = 0, nidx = idx;
-    bool take[K];
 #pragma unroll
     for (int j = 0; j < K; ++j) {
-      take[j] = h[j] <= lim;
-      ev |= take[j] ? (1u << (2 * j)) : 0u;
-      nidx = take[j] ? ((nidx & ~(3u << (2 * j))) | ((h[j] & 3u) << (2 * j))) : nidx;
-      r[j] += take[j] ? 1u : 0u;
+      const bool take = h[j] <= lim;
+      ev |= take ? (1u << (2 * j)) : 0u;
+      nidx = take ? ((nidx & ~(3u << (2 * j))) | ((h[j] & 3u) << (2 * j))) : nidx;
+      r[j] += take ? 1u : 0u;
     }
-    // the next heads come from the look-ahead registers; the look-ahead
-    // loads are issued now and waited for only when an input advances again
 #pragma unroll
-    for (int j = 0; j < K; ++j) {
-      h[j] = take[j] ? n1[j] : h[j];
-      n1[j] = next_word<LDS>(X.src[j], r[j], bound[j]);
-    }
+    for (int j = 0; j < K; ++j) h[j] = (LDS || r[j] < bound[j]) ? X.src[j][r[j]] : INF;
     const uint32_t f = lut_eval<K>(X, s_lut, nidx);
     const uint32_t sc = edge_cls(fprev, f);
     const uint32_t key = (((sc - 1u) & 3u) << (2 * K)) | event_bits(idx, nidx, ev);
