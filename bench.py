@@ -296,14 +296,14 @@ def batched_vs_plain(ctx, n_sets, n_nets, scale):
                          "tuned_on_instances": len(sub)}}
 
 
-def serving_mix(ctx, n_jobs, scale, check_alone=8):
+def serving_mix(ctx, n_jobs, scale):
     """Serving loop on a seeded Poisson job stream: netlists log-uniform in
     [10k, 2M] x `scale` cells (the same generator), windows U[1, 20] Mps x
-    `scale`, 30% of jobs reuse a netlist already submitted; the arrival rate
-    is set from the mean time of the first jobs run alone, for about 75%
-    device utilisation.  Reports events/s and p50/p99 job latency; the SAIF
-    of `check_alone` jobs is compared with the same job run alone (the tests
-    compare every job of their mixes)."""
+    `scale`, 30% of jobs reuse a netlist already submitted.  Every job is
+    first run alone through the same server path (one job at a time); the
+    arrival rate is set from the mean alone time for about 75% device
+    utilisation.  Reports events/s and p50/p99 job latency, and compares
+    every job's SAIF in the mix with its alone run."""
     import gls
     rng = random.Random(0x5EED0004)
     designs, jobs = [], []
@@ -319,16 +319,20 @@ def serving_mix(ctx, n_jobs, scale, check_alone=8):
         s = gls.Stimulus.generate(d, 0x5EED5000 + i, t_end, 500.0, 0.01)
         jobs.append((d, s, 0, t_end + 1))
     log(f"c4: {n_jobs} jobs generated, {len(designs)} netlists")
+    # alone: the same server path one job at a time (each job run twice, the
+    # second time with its netlist resident and the arena warm, as in the mix)
     alone, t_alone = {}, []
-    for k in range(min(check_alone, n_jobs)):
+    solo = gls.Server(ctx, max_running=1)
+    for k in range(n_jobs):
         d, s, on, off = jobs[k]
-        r = ctx.simulate(d, s, on, off)
-        alone[k] = r.saif_text()
-        r.close()
+        j = solo.submit(d, s, on, off)
+        solo.wait(j)
+        alone[k] = solo.saif_text(j)
         t0 = time.perf_counter()
-        r = ctx.simulate(d, s, on, off)  # netlist resident, as for a served job
+        j = solo.submit(d, s, on, off)
+        solo.wait(j)
         t_alone.append(time.perf_counter() - t0)
-        r.close()
+    solo.close()
     mean_s = sum(t_alone) / len(t_alone)
     rate = 0.75 / mean_s
     gaps = [rng.expovariate(rate) for _ in jobs]
@@ -348,6 +352,7 @@ def serving_mix(ctx, n_jobs, scale, check_alone=8):
             "events_per_s": round(st["events_per_s"], 1), "p50_ms": round(st["p50_ms"], 3),
             "p99_ms": round(st["p99_ms"], 3), "mean_alone_ms": round(mean_s * 1e3, 3),
             "peak_running": int(st["peak_running"]), "resident_programs": int(st["resident_programs"]),
+            "device_buffers_reused": int(st["arena_hits"]), "device_buffers_allocated": int(st["arena_misses"]),
             "saif_same_alone_and_in_mix": same, "jobs_checked_alone": len(alone)}
 
 

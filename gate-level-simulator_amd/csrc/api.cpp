@@ -828,6 +828,8 @@ int gls_server_stats_get(gls_server* s, gls_server_stats* st) {
   st->resident_programs = x.resident_programs;
   st->peak_bytes = x.peak_bytes;
   st->cancelled = x.cancelled;
+  st->arena_hits = x.arena_hits;
+  st->arena_misses = x.arena_misses;
   API_END
 }
 

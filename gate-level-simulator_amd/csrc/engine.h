@@ -107,6 +107,37 @@ class ProgramCache {
       map_;
 };
 
+// Caching device allocator shared by the jobs of a serving loop: a job's
+// buffers go back to the arena when it ends and later jobs take them again,
+// so a stream of jobs neither allocates nor frees device memory once the
+// arena is warm (hipFree waits for the whole device, which would stall the
+// other running jobs).  A cached block of at least the asked size and at
+// most 1.5 times it is reused; when the device is out of memory the cached
+// blocks are freed and the allocation is tried again.  The caller returns a
+// block only when no device work that uses it is pending.
+class DevArena {
+ public:
+  explicit DevArena(uint64_t max_cached_bytes) : max_cached_(max_cached_bytes) {}
+  ~DevArena();
+  DevArena(const DevArena&) = delete;
+  DevArena& operator=(const DevArena&) = delete;
+  // A block of at least `bytes` bytes, or false when the device has no room.
+  bool get(size_t bytes, void** p);
+  void put(void* p);
+  void trim();  // frees every cached block
+  uint64_t hits() const { return hits_; }
+  uint64_t misses() const { return misses_; }
+
+ private:
+  static size_t size_class(size_t bytes);
+  void trim_locked();
+  std::mutex mu_;
+  std::multimap<size_t, void*> free_;          // cached blocks by size
+  std::map<void*, size_t> live_;               // blocks handed out, with their sizes
+  uint64_t cached_ = 0, max_cached_;
+  uint64_t hits_ = 0, misses_ = 0;
+};
+
 // A prepared simulation of one or more instances, run level by level: level
 // l of every instance is evaluated by the same launches.
 //
@@ -120,7 +151,7 @@ class ProgramCache {
 // time base.
 class Job {
  public:
-  Job(int device, hipStream_t stream);
+  Job(int device, hipStream_t stream, DevArena* arena = nullptr);
   ~Job();
   Job(const Job&) = delete;
   Job& operator=(const Job&) = delete;
@@ -179,6 +210,16 @@ class Job {
     size_t slot;          // ChunkWin slot of the chunk's windows
   };
   enum ChunkResult { CHUNK_OK = 0, CHUNK_SHORTER = 1 };
+  // one stimulus signal of the resident store: n (time, value) entries
+  struct SigSrc {
+    const uint32_t* t;
+    const uint8_t* v;
+    size_t n;
+  };
+  // words per pinned staging buffer of the stimulus upload (two of them)
+  static constexpr uint64_t kStageWords = 1ull << 24;
+  void upload_store(const std::vector<SigSrc>& src, const std::vector<uint64_t>& soff,
+                    uint64_t total);
   void ensure(void** p, size_t& cap, size_t need);
   bool try_ensure(void** p, size_t& cap, size_t need);
   Chunk make_chunk(size_t k, uint64_t F, uint64_t Fn, uint64_t prev_lb) const;
@@ -192,12 +233,15 @@ class Job {
   void run_two_pass(std::vector<hipEvent_t>& ev_eval);
   void stop_point();
   void* dmalloc(size_t bytes);
+  bool dev_alloc(void** p, size_t bytes);  // arena or hipMalloc; false: no room
+  void dev_free(void* p);                  // arena (after the stream drains) or hipFree
   bool grow_pool(uint64_t need, uint64_t used);
   bool refresh_pool(uint64_t words);
   uint64_t pool_max_words() const;
   void fetch_chunk_waves(const Chunk& c);
   int device_;
   hipStream_t stream_;
+  DevArena* arena_ = nullptr;
   std::shared_ptr<DevProgram> prog_;
   std::vector<uint32_t> net_base_;
   std::vector<uint64_t> win_;  // start, end per instance

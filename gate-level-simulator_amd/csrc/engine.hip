@@ -5,8 +5,10 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <atomic>
 #include <map>
 #include <mutex>
+#include <thread>
 
 #include "../../include/gls_status.h"
 
@@ -411,8 +413,6 @@ static std::shared_ptr<DevProgram> union_image(const std::vector<std::shared_ptr
   return U;
 }
 
-// ------------------------------------------------------------------ job ---
-
 // GLS_TRACE=1: one line per time chunk on stderr (bounds, outcome, pool
 // words used, host time, pool reallocations).
 static bool trace_on() {
@@ -423,16 +423,113 @@ static bool trace_on() {
   return on;
 }
 
-Job::Job(int device, hipStream_t stream) : device_(device), stream_(stream) {}
+// ---------------------------------------------------------------- arena ---
+
+DevArena::~DevArena() {
+  std::lock_guard<std::mutex> lk(mu_);
+  trim_locked();
+}
+
+size_t DevArena::size_class(size_t bytes) {
+  // 4 KiB at least; up to 1 MiB powers of two, above in 1 MiB steps
+  if (bytes <= (1u << 20)) {
+    size_t c = 4096;
+    while (c < bytes) c <<= 1;
+    return c;
+  }
+  return (bytes + (1u << 20) - 1) & ~(size_t)((1u << 20) - 1);
+}
+
+bool DevArena::get(size_t bytes, void** p) {
+  const size_t c = size_class(bytes);
+  std::lock_guard<std::mutex> lk(mu_);
+  auto it = free_.lower_bound(c);
+  if (it != free_.end() && it->first <= c + c / 2) {
+    *p = it->second;
+    live_[*p] = it->first;
+    cached_ -= it->first;
+    free_.erase(it);
+    ++hits_;
+    return true;
+  }
+  ++misses_;
+  for (int attempt = 0; attempt < 2; ++attempt) {
+    void* q = nullptr;
+    if (hipMalloc(&q, c) == hipSuccess) {
+      live_[q] = c;
+      *p = q;
+      return true;
+    }
+    (void)hipGetLastError();
+    if (free_.empty()) break;
+    trim_locked();  // the cached blocks make room
+  }
+  *p = nullptr;
+  return false;
+}
+
+void DevArena::put(void* p) {
+  if (!p) return;
+  std::lock_guard<std::mutex> lk(mu_);
+  auto it = live_.find(p);
+  if (it == live_.end()) return;
+  const size_t c = it->second;
+  live_.erase(it);
+  free_.emplace(c, p);
+  cached_ += c;
+  // over the cap: the largest cached blocks go first
+  while (cached_ > max_cached_ && !free_.empty()) {
+    auto last = std::prev(free_.end());
+    (void)hipFree(last->second);
+    cached_ -= last->first;
+    free_.erase(last);
+  }
+}
+
+void DevArena::trim() {
+  std::lock_guard<std::mutex> lk(mu_);
+  trim_locked();
+}
+
+void DevArena::trim_locked() {
+  for (auto& kv : free_) (void)hipFree(kv.second);
+  free_.clear();
+  cached_ = 0;
+}
+
+// ------------------------------------------------------------------ job ---
+
+Job::Job(int device, hipStream_t stream, DevArena* arena)
+    : device_(device), stream_(stream), arena_(arena) {}
 
 Job::~Job() {
   if (gexec_) (void)hipGraphExecDestroy(gexec_);
+  // blocks go back to the arena only once this job's work has drained
+  if (arena_) (void)hipStreamSynchronize(stream_);
   for (void* p : owned_)
-    if (p) (void)hipFree(p);
+    if (p) arena_ ? arena_->put(p) : (void)hipFree(p);
   void* grown[] = {d_segmap_, d_sums_, d_carry_, d_scratch_, d_tiles_, d_tout_, d_toff_, d_stk_,
                    d_pool_,   d_ovf_,  d_tails_, d_cwin_};
   for (void* p : grown)
-    if (p) (void)hipFree(p);
+    if (p) arena_ ? arena_->put(p) : (void)hipFree(p);
+}
+
+bool Job::dev_alloc(void** p, size_t bytes) {
+  if (arena_) return arena_->get(bytes, p);
+  if (hipMalloc(p, bytes) == hipSuccess) return true;
+  (void)hipGetLastError();
+  *p = nullptr;
+  return false;
+}
+
+void Job::dev_free(void* p) {
+  if (!p) return;
+  if (arena_) {
+    HC(hipStreamSynchronize(stream_));
+    arena_->put(p);
+  } else {
+    HC(hipFree(p));
+  }
 }
 
 void* Job::dmalloc(size_t bytes) {
@@ -440,7 +537,7 @@ void* Job::dmalloc(size_t bytes) {
   bytes = std::max<size_t>(bytes, 256);
   if (static_) throw GlsError(GLS_E_STATE, "device allocation during an allocation-free execution");
   ++allocs_;
-  HC(hipMalloc(&p, bytes));
+  if (!dev_alloc(&p, bytes)) HC(hipErrorOutOfMemory);
   owned_.push_back(p);
   dev_bytes_ += bytes;
   return p;
@@ -464,18 +561,12 @@ bool Job::try_ensure(void** p, size_t& cap, size_t need) {
   ++allocs_;
   size_t n = std::max<size_t>(need, cap + cap / 2);
   n = std::max<size_t>(n, 4096);
-  if (*p) HC(hipFree(*p));
+  dev_free(*p);
   *p = nullptr;
   dev_bytes_ -= cap;
   cap = 0;
-  if (hipMalloc(p, n) != hipSuccess) {
-    (void)hipGetLastError();
-    *p = nullptr;
-    if (n == need || hipMalloc(p, need) != hipSuccess) {
-      (void)hipGetLastError();
-      *p = nullptr;
-      return false;
-    }
+  if (!dev_alloc(p, n)) {
+    if (n == need || !dev_alloc(p, need)) return false;
     n = need;
   }
   dev_bytes_ += n;
@@ -498,15 +589,12 @@ bool Job::grow_pool(uint64_t need, uint64_t used) {
   if (n > room) n = need;
   if (n > room) return false;
   uint32_t* p = nullptr;
-  if (hipMalloc((void**)&p, (n + kPoolPad) * sizeof(uint32_t)) != hipSuccess) {
-    (void)hipGetLastError();
-    return false;
-  }
+  if (!dev_alloc((void**)&p, (n + kPoolPad) * sizeof(uint32_t))) return false;
   ++pool_reallocs_;
   ++allocs_;
   if (used) HC(hipMemcpyAsync(p, d_pool_, used * 4, hipMemcpyDeviceToDevice, stream_));
   HC(hipStreamSynchronize(stream_));
-  HC(hipFree(d_pool_));
+  dev_free(d_pool_);
   d_pool_ = p;
   dev_bytes_ += (n - pool_words_) * sizeof(uint32_t);
   pool_words_ = n;
@@ -537,19 +625,18 @@ bool Job::refresh_pool(uint64_t words) {
   ++pool_reallocs_;
   ++allocs_;
   HC(hipStreamSynchronize(stream_));
-  HC(hipFree(d_pool_));
+  dev_free(d_pool_);
   d_pool_ = nullptr;
   dev_bytes_ -= (pool_words_ + kPoolPad) * sizeof(uint32_t);
   uint64_t n = words;
   while (n > pool_words_) {
-    if (hipMalloc((void**)&d_pool_, (n + kPoolPad) * sizeof(uint32_t)) == hipSuccess) break;
-    (void)hipGetLastError();
+    if (dev_alloc((void**)&d_pool_, (n + kPoolPad) * sizeof(uint32_t))) break;
     d_pool_ = nullptr;
     n = n / 10 * 9;
   }
   if (!d_pool_) {
     n = pool_words_;
-    HC(hipMalloc((void**)&d_pool_, (n + kPoolPad) * sizeof(uint32_t)));
+    if (!dev_alloc((void**)&d_pool_, (n + kPoolPad) * sizeof(uint32_t))) HC(hipErrorOutOfMemory);
   }
   dev_bytes_ += (n + kPoolPad) * sizeof(uint32_t);
   const bool grew = n > pool_words_;
@@ -560,6 +647,71 @@ bool Job::refresh_pool(uint64_t words) {
     graph_sig_.clear();
   }
   return grew;
+}
+
+// The resident stimulus store (64-bit words (time << 2) | value, signal
+// after signal) goes to the device through two pinned staging buffers:
+// host threads convert block b + 1 while block b is copied.  The times of
+// every signal are checked to increase on the way.
+void Job::upload_store(const std::vector<SigSrc>& src, const std::vector<uint64_t>& soff,
+                       uint64_t total) {
+  if (total == 0) return;
+  const uint64_t B = std::min<uint64_t>(total, kStageWords);
+  uint64_t* buf[2] = {nullptr, nullptr};
+  hipEvent_t done[2] = {nullptr, nullptr};
+  bool used[2] = {false, false};
+  std::atomic<bool> unordered{false};
+  auto release = [&]() {
+    for (int i = 0; i < 2; ++i) {
+      if (done[i]) (void)hipEventDestroy(done[i]);
+      if (buf[i]) (void)hipHostFree(buf[i]);
+    }
+  };
+  try {
+    for (int i = 0; i < 2; ++i) {
+      HC(hipHostMalloc((void**)&buf[i], B * 8, hipHostMallocDefault));
+      HC(hipEventCreateWithFlags(&done[i], hipEventDisableTiming));
+    }
+    const int nth = (int)std::max<uint64_t>(1, std::min<uint64_t>(host_threads(0), B / 65536));
+    for (uint64_t b0 = 0, k = 0; b0 < total; b0 += B, ++k) {
+      const uint64_t n = std::min(B, total - b0);
+      uint64_t* dst = buf[k & 1];
+      if (used[k & 1]) HC(hipEventSynchronize(done[k & 1]));
+      // words [b0, b0 + n), cut into one stretch per thread
+      auto fill = [&](uint64_t x0, uint64_t x1) {
+        size_t sig = (size_t)(std::upper_bound(soff.begin(), soff.end(), x0) - soff.begin()) - 1;
+        for (uint64_t x = x0; x < x1;) {
+          while (sig + 1 < soff.size() && soff[sig + 1] <= x) ++sig;
+          const SigSrc& S = src[sig];
+          const uint64_t q0 = x - soff[sig];
+          const uint64_t q1 = std::min<uint64_t>(S.n, q0 + (x1 - x));
+          for (uint64_t q = q0; q < q1; ++q) {
+            dst[x - b0 + (q - q0)] = ((uint64_t)S.t[q] << 2) | (S.v[q] & 3u);
+            if (q > 0 && S.t[q] <= S.t[q - 1]) unordered.store(true, std::memory_order_relaxed);
+          }
+          x += q1 - q0;
+        }
+      };
+      if (nth == 1) {
+        fill(b0, b0 + n);
+      } else {
+        std::vector<std::thread> th;
+        for (int i = 0; i < nth; ++i)
+          th.emplace_back(fill, b0 + n * i / nth, b0 + n * (i + 1) / nth);
+        for (auto& t : th) t.join();
+      }
+      HC(hipMemcpyAsync(d_store_ + b0, dst, n * 8, hipMemcpyHostToDevice, stream_));
+      HC(hipEventRecord(done[k & 1], stream_));
+      used[k & 1] = true;
+    }
+    HC(hipStreamSynchronize(stream_));
+  } catch (...) {
+    (void)hipStreamSynchronize(stream_);
+    release();
+    throw;
+  }
+  release();
+  if (unordered.load()) throw GlsError(GLS_E_FORMAT, "stimulus times must increase");
 }
 
 void Job::prepare(const std::vector<SimInstance>& inst, uint64_t pool_words) {
@@ -601,7 +753,8 @@ void Job::prepare(const std::vector<SimInstance>& inst, uint64_t pool_words) {
   saif_net_list_.clear();
   saif_net_job_.clear();
   std::vector<uint32_t> words;    // two-pass: initial waveforms in the pool
-  std::vector<uint64_t> store;    // fused: resident 64-bit words
+  std::vector<SigSrc> src;        // fused: the signals of the resident store, in order
+  uint64_t store_words = 0;
   std::vector<uint64_t> soff;
   h_slen_.clear();
   h_snet_.clear();
@@ -623,31 +776,34 @@ void Job::prepare(const std::vector<SimInstance>& inst, uint64_t pool_words) {
         words.push_back((t[q] << 2) | (v[q] & 3u));
       }
     } else {
-      soff.push_back(store.size());
+      soff.push_back(store_words);
       h_slen_.push_back((uint32_t)n);
       h_snet_.push_back(net);
       sjob.push_back(job);
-      for (size_t q = 0; q < n; ++q) store.push_back(((uint64_t)t[q] << 2) | (v[q] & 3u));
+      src.push_back({t, v, n});
+      store_words += n;
       if (n) t_last_ = std::max<uint64_t>(t_last_, t[n - 1]);
     }
     saif_net_list_.push_back(net);
     saif_net_job_.push_back(job);
   };
+  // constants: one entry (0, value) each; the four possible entries live here
+  static const uint32_t kT0[1] = {0};
+  static const uint8_t kV[4] = {0, 1, 2, 3};
   for (size_t i = 0; i < inst.size(); ++i) {
     const SimInstance& in = inst[i];
     const uint32_t nb = net_base_[i];
     for (size_t s = 0; s < in.st->waves.size(); ++s) {
       const Wave& w = in.st->waves[s];
-      for (size_t q = 1; q < w.size(); ++q)
-        if (w.t[q] <= w.t[q - 1]) throw GlsError(GLS_E_FORMAT, "stimulus times must increase");
+      if (mode == 1)
+        for (size_t q = 1; q < w.size(); ++q)
+          if (w.t[q] <= w.t[q - 1]) throw GlsError(GLS_E_FORMAT, "stimulus times must increase");
       add_signal(nb + (uint32_t)in.stim_nets[s], (uint32_t)i, w.t.data(), w.v.data(), w.size());
     }
     const Program& pg = *in.pg;
     for (uint32_t n = 0; n < (uint32_t)pg.net_names.size(); ++n) {
       if (pg.role[n] != NR_CONST) continue;
-      const uint32_t t0 = 0;
-      const uint8_t v0 = (uint8_t)(pg.const_val[n] & 3);
-      add_signal(nb + n, (uint32_t)i, &t0, &v0, 1);
+      add_signal(nb + n, (uint32_t)i, kT0, &kV[pg.const_val[n] & 3], 1);
     }
   }
   n_sig_ = (uint32_t)h_snet_.size();
@@ -678,7 +834,7 @@ void Job::prepare(const std::vector<SimInstance>& inst, uint64_t pool_words) {
   ovf_words_ = ovf_words_opt ? ovf_words_opt : (1ull << 24);
   ensure((void**)&d_ovf_, cap_ovf_, ovf_words_ * 4);
   if (mode == 0) {
-    d_store_ = (uint64_t*)dmalloc(store.size() * 8);
+    d_store_ = (uint64_t*)dmalloc(std::max<uint64_t>(store_words, 1) * 8);
     d_soff_ = (uint64_t*)dmalloc(n_sig_ * 8ull);
     d_slen_ = (uint32_t*)dmalloc(n_sig_ * 4ull);
     d_snet_ = (uint32_t*)dmalloc(n_sig_ * 4ull);
@@ -695,7 +851,7 @@ void Job::prepare(const std::vector<SimInstance>& inst, uint64_t pool_words) {
     d_wtot_ = (unsigned long long*)dmalloc(n_nets_ * 8ull);
     ensure((void**)&d_tails_, cap_tails_, std::max<size_t>(n_gates_, 1) * 4);
     ensure((void**)&d_cwin_, cap_cwin_, inst.size() * sizeof(ChunkWin));
-    stim_words_ = store.size() + n_sig_;  // one chunk: every entry and a state entry per signal
+    stim_words_ = store_words + n_sig_;  // one chunk: every entry and a state entry per signal
   } else {
     stim_words_ = words.size();
   }
@@ -713,7 +869,7 @@ void Job::prepare(const std::vector<SimInstance>& inst, uint64_t pool_words) {
   if (mode == 1 && pool_words < stim_words_ + 1024) pool_words = stim_words_ + 1024;
   if (pool_words < 1024) pool_words = 1024;
   pool_words_ = pool_words;
-  HC(hipMalloc((void**)&d_pool_, (pool_words_ + kPoolPad) * sizeof(uint32_t)));
+  if (!dev_alloc((void**)&d_pool_, (pool_words_ + kPoolPad) * sizeof(uint32_t))) HC(hipErrorOutOfMemory);
   dev_bytes_ += (pool_words_ + kPoolPad) * sizeof(uint32_t);
 
   if (mode == 1) {
@@ -726,8 +882,7 @@ void Job::prepare(const std::vector<SimInstance>& inst, uint64_t pool_words) {
                         hipMemcpyHostToDevice, stream_));
     }
   } else {
-    if (!store.empty())
-      HC(hipMemcpyAsync(d_store_, store.data(), store.size() * 8, hipMemcpyHostToDevice, stream_));
+    upload_store(src, soff, store_words);
     if (n_sig_) {
       HC(hipMemcpyAsync(d_soff_, soff.data(), n_sig_ * 8ull, hipMemcpyHostToDevice, stream_));
       HC(hipMemcpyAsync(d_slen_, h_slen_.data(), n_sig_ * 4ull, hipMemcpyHostToDevice, stream_));

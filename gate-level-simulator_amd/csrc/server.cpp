@@ -22,6 +22,8 @@ Server::Server(int device, const ServerOptions& o, ProgramCache* cache)
     hip_check(hipMemGetInfo(&free_b, &total_b), "hipMemGetInfo");
     opt_.device_budget = (uint64_t)(free_b * 0.9);
   }
+  // cached blocks stay within half the budget; the rest is freed as jobs end
+  arena_ = std::make_unique<DevArena>(opt_.device_budget / 2);
   if (opt_.max_running == 0) opt_.max_running = 1;
   if (!(opt_.pool_factor > 0)) opt_.pool_factor = 1.0;
   for (uint32_t i = 0; i < opt_.max_running; ++i) workers_.emplace_back(&Server::worker, this, (int)i);
@@ -179,7 +181,7 @@ void Server::worker(int slot) {
       for (uint32_t attempt = 0; attempt < 2; ++attempt) {
         ++attempts;
         try {
-          Job job(device_, stream);
+          Job job(device_, stream, arena_.get());
           job.seg_target = opt_.seg_events;
           job.should_stop = [this, r] { return expired(*r, Clock::now()); };
           if (job.should_stop()) throw GlsError(GLS_E_CANCELLED, "job cancelled or past its deadline");
@@ -327,6 +329,8 @@ ServerStats Server::stats() {
   s.peak_running = peak_running_;
   s.peak_bytes = peak_bytes_;
   s.resident_programs = (uint32_t)cache_->size();
+  s.arena_hits = arena_->hits();
+  s.arena_misses = arena_->misses();
   return s;
 }
 

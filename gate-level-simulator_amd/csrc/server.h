@@ -59,6 +59,7 @@ struct ServerStats {
   double p50_ms = 0, p99_ms = 0, mean_ms = 0, wall_s = 0, events_per_s = 0;
   uint32_t peak_running = 0, resident_programs = 0;
   uint64_t peak_bytes = 0;
+  uint64_t arena_hits = 0, arena_misses = 0;  // device blocks reused / newly allocated
 };
 
 class Server {
@@ -103,6 +104,7 @@ class Server {
   int device_;
   ServerOptions opt_;
   ProgramCache* cache_;
+  std::unique_ptr<DevArena> arena_;  // the jobs' device buffers, reused from job to job
   std::mutex mu_;
   std::condition_variable cv_work_, cv_done_;
   std::deque<uint64_t> queue_;

@@ -93,7 +93,8 @@ class ServerStats(ctypes.Structure):
                 ("mean_ms", ctypes.c_double), ("wall_s", ctypes.c_double),
                 ("events_per_s", ctypes.c_double), ("peak_running", ctypes.c_uint32),
                 ("resident_programs", ctypes.c_uint32), ("peak_bytes", ctypes.c_uint64),
-                ("cancelled", ctypes.c_uint64)]
+                ("cancelled", ctypes.c_uint64), ("arena_hits", ctypes.c_uint64),
+                ("arena_misses", ctypes.c_uint64)]
 
 
 # Every symbol include/gls.h declares: (name, restype, argtypes)

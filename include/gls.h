@@ -267,6 +267,8 @@ typedef struct {
   uint32_t peak_running, resident_programs;
   uint64_t peak_bytes;
   uint64_t cancelled;     /* failed jobs that were cancelled or missed their deadline */
+  uint64_t arena_hits;    /* device buffers of jobs taken from the server's cache */
+  uint64_t arena_misses;  /* device buffers newly allocated */
 } gls_server_stats;
 int gls_server_create(gls_ctx* c, const gls_server_opts* o, gls_server** out);
 /* The server copies the stimulus; the design must outlive the server. */
