@@ -169,6 +169,7 @@ struct TileCtx {
   typename InPtr<LDS>::T src[K];  // input ranges, index relative to B
   uint32_t B[K], E[K], L[K];
   uint32_t nseg, stride, lead, dmax;
+  uint32_t span;            // the tile's own input words (out_cap - 1)
   uint32_t lut[4];          // packed truth table, fan-in <= 3
 };
 
@@ -316,40 +317,90 @@ __device__ __forceinline__ Lane<SP> lane_walk(const TileCtx<K, LDS>& X, uint32_t
                                               SP stk_base, uint32_t stk_words, uint32_t& bad,
                                               uint32_t lo_rel, uint32_t hi_rel, bool first) {
   const int lane = threadIdx.x;
-  const uint32_t c = ti * kLanes + lane;
+  const uint32_t c0 = ti * kLanes;                          // the tile's first segment
+  const uint32_t m = min((uint32_t)kLanes, X.nseg - c0);   // its segments (>= 1)
   const uint32_t kX = 1u;
   const uint32_t INF = 0xFFFFFFFFu;
   uint32_t r[K], h[K], bound[K];
   typename InPtr<LDS>::T srcL = X.src[0];
-  uint32_t BL = X.B[0];
+  uint32_t BL = X.B[0], EL = X.E[0];
 #pragma unroll
   for (int j = 0; j < K; ++j) {
     bound[j] = X.E[j] - X.B[j];
     if ((uint32_t)j == X.lead) {
       srcL = X.src[j];
       BL = X.B[j];
+      EL = X.E[j];
     }
   }
   constexpr bool kGuard = !std::is_same<SP, lds_u32*>::value;
   Lane<SP> ln;
   ln.st = stk_base;
   ln.sp = ln.top = ln.cnt = ln.nown = 0;
-  if (c >= X.nseg) return ln;
-  // segment bounds from the leader
-  uint32_t tau_c = 0, tau_n = 0x3FFFFFFFu, halo_end = 0x3FFFFFFFu;
-  uint32_t soff = lane, idx = 0;
-  const uint32_t pl = c * X.stride;
-  if (c > 0) tau_c = srcL[pl - BL] >> 2;
-  if (c + 1 < X.nseg) {
-    tau_n = srcL[pl + X.stride - BL] >> 2;
-    halo_end = tau_n + X.dmax;
+  // Lane bounds.  The tile's nominal segments (every `stride` leader events)
+  // carry uneven numbers of input events; each lane instead takes an equal
+  // share of the tile's input words.  Lane q < m first finds the word rank
+  // of nominal boundary q; lane c then places its start at the leader event
+  // interpolated between the nominal boundaries around rank c * span / 64.
+  // Lane 0 keeps the tile's start; segment 0 of the gate starts at index 1
+  // (index 0 is the state entry) at time 0.
+  const bool gate_start = c0 == 0 && lane == 0;
+  uint32_t rq = 0xFFFFFFFFu;  // rank of nominal boundary `lane` (lanes < m)
+  if ((uint32_t)lane < m) {
+    const uint32_t pq = (c0 + lane) * X.stride;
+    if (c0 + lane == 0) {
+      rq = 0;
+#pragma unroll
+      for (int j = 0; j < K; ++j) rq += 1u - X.L[j];
+    } else {
+      const uint32_t tq = srcL[pq - BL] >> 2;
+      rq = 0;
+#pragma unroll
+      for (int j = 0; j < K; ++j)
+        rq += ((uint32_t)j == X.lead ? pq : X.B[j] + lb_words(X.src[j], bound[j], tq << 2)) - X.L[j];
+    }
   }
+  const uint32_t r0 = __shfl(rq, 0);
+  const uint32_t target = r0 + (uint32_t)(((uint64_t)(X.span - r0) * (uint32_t)lane) >> 6);
+  uint32_t q = 0;  // last nominal boundary at or below the target
+#pragma unroll
+  for (uint32_t step = 32; step; step >>= 1) {
+    const uint32_t cand = q + step;
+    const uint32_t rc = __shfl(rq, (int)(cand & 63u));
+    q = (cand < m && rc <= target) ? cand : q;
+  }
+  const uint32_t r_lo = __shfl(rq, (int)q);
+  const uint32_t r_nx = __shfl(rq, (int)min(q + 1, 63u));
+  const uint32_t r_hi = q + 1 < m ? r_nx : X.span;
+  const uint32_t p_lo = (c0 + q) * X.stride;
+  const uint32_t p_hi = min((c0 + q + 1) * X.stride, EL);
+  uint32_t pi = c0 * X.stride;  // lane 0: the tile's start
+  if (lane > 0) {
+    pi = p_lo;
+    if (r_hi > r_lo && p_hi > p_lo) {
+      pi = p_lo + (uint32_t)(((uint64_t)(target - r_lo) * (p_hi - p_lo)) / (r_hi - r_lo));
+      pi = min(pi, p_hi - 1);
+    }
+  }
+  pi = max(pi, 1u);
+  const uint32_t tau_c = gate_start ? 0u : srcL[pi - BL] >> 2;
+  // the segment ends where the next lane's starts; the tile's last lane ends
+  // at the next tile's first leader event (none after the gate's last tile)
+  const uint32_t tau_next = __shfl_down(tau_c, 1);
+  uint32_t tau_n = 0x3FFFFFFFu, halo_end = 0x3FFFFFFFu;
+  if (lane < kLanes - 1) {
+    tau_n = tau_next;
+  } else if (c0 + m < X.nseg) {
+    tau_n = srcL[(c0 + m) * X.stride - BL] >> 2;
+  }
+  if (tau_n != 0x3FFFFFFFu) halo_end = tau_n + X.dmax;
+  uint32_t soff = lane, idx = 0;
 #pragma unroll
   for (int j = 0; j < K; ++j) {
     // pj >= 1: index 0 is the state entry, and every event time is >= 1
     uint32_t pj;
-    if (c == 0) pj = 1;
-    else if ((uint32_t)j == X.lead) pj = pl;
+    if (gate_start) pj = 1;
+    else if ((uint32_t)j == X.lead) pj = pi;
     else pj = X.B[j] + lb_words(X.src[j], bound[j], tau_c << 2);
     soff += pj - X.L[j];
     r[j] = pj - X.B[j];
@@ -362,7 +413,7 @@ __device__ __forceinline__ Lane<SP> lane_walk(const TileCtx<K, LDS>& X, uint32_t
   const uint32_t s0 = kGuard ? min(soff, stk_words - 1) : soff;
   const uint32_t stk_lim = stk_words - s0;
   SP st = stk_base + s0;
-  const bool init = c == 0 && first;
+  const bool init = gate_start && first;
   // f before the first event: X at t = 0 (src/simulate.cpp:933), else the
   // function of the starting values
   uint32_t fprev = init ? kX : lut_eval<K>(X, s_lut, idx);
@@ -455,6 +506,7 @@ __device__ __forceinline__ void walk_k(const LevelArgs& A, const TileDesc& D, bo
     X.stride = D.stride;
     X.lead = D.lead;
     X.dmax = D.dmax;
+    X.span = D.out_cap - 1;
 #pragma unroll
     for (int q = 0; q < 4; ++q) X.lut[q] = D.lutw[q];
 #pragma unroll
@@ -666,14 +718,14 @@ __global__ __launch_bounds__(kLanes) void k_level(LevelArgs A) {
     const TileDesc D = desc_of(dw);
     const bool wide = D.k > kTileK;  // uniform: the tile is k_wide's
     wait_vm();
-    if (fits && !wide) {  // INF sentinel after every input range
+    if (fits && !wide && lane == 0) {  // INF sentinel after every input range
+      // one lane, constant indices: the descriptor stays in registers (an
+      // index chosen by lane would put it in per-lane scratch memory)
       uint32_t P[4];
       range_layout(D, P);
 #pragma unroll
-      for (uint32_t j = 0; j < 4; ++j) {
-        if (j >= D.k) break;
-        if ((uint32_t)lane == j) s_in[P[j] + D.E[j] - range_start(D.L[j])] = 0xFFFFFFFFu;
-      }
+      for (uint32_t j = 0; j < 4; ++j)
+        if (j < D.k) s_in[P[j] + D.E[j] - range_start(D.L[j])] = 0xFFFFFFFFu;
     }
     __syncthreads();
     // the emit's scalars, loaded now so the walk covers their latency
