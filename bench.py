@@ -113,7 +113,9 @@ def profile_traffic(workload):
     this command's workload (counters are read by a profiler run, not from
     inside this process): the dominant kernel per launch and the whole
     pipeline per step, with the file they come from."""
-    p = os.path.join(ROOT, "profiles", f"hbm_counters_{workload}.txt")
+    name = {"c2": "k_level_2M_cells_hbm_counters.txt",
+            "c1": "k_level_200k_cells_hbm_counters.txt"}[workload]
+    p = os.path.join(ROOT, "profiles", name)
     vals = {}
     try:
         for line in open(p):

@@ -1169,21 +1169,53 @@ __global__ void k_tile_desc(const GateRec* __restrict__ gates, uint32_t ng,
                             const uint64_t* __restrict__ woff, const uint32_t* __restrict__ wlen,
                             const uint32_t* __restrict__ pool, const uint32_t* __restrict__ lutp,
                             TileDesc* __restrict__ tiles, const uint64_t* __restrict__ ntiles) {
+  // a tile's first own word of each input is the previous tile's end: the
+  // block's tiles pass them on through LDS (each boundary searched once)
+  __shared__ uint32_t s_first[256];
   const uint64_t x = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const uint64_t t = x >> 2;
   const int j = (int)(x & 3);
-  if (t >= *ntiles) return;
-  // gate of tile t: binary search over tile_base
-  uint32_t lo = 0, hi = ng;
-  while (hi - lo > 1) {
-    uint32_t mid = (lo + hi) >> 1;
-    if (tile_base[mid] <= t) lo = mid;
-    else hi = mid;
+  const uint64_t nt = *ntiles;
+  const bool valid = t < nt;
+  // gate of tile t: binary search over tile_base by the tile's first
+  // thread, passed to the other three
+  uint32_t g = 0;
+  if (valid && j == 0) {
+    uint32_t lo = 0, hi = ng;
+    while (hi - lo > 1) {
+      uint32_t mid = (lo + hi) >> 1;
+      if (tile_base[mid] <= t) lo = mid;
+      else hi = mid;
+    }
+    g = lo;
   }
-  const uint32_t g = lo;
-  const GateRec& G = gates[g];
-  const GatePlan P = plan[g];
-  const uint32_t ti = (uint32_t)(t - tile_base[g]);
+  g = __shfl(g, (int)((threadIdx.x & 63u) & ~3u));
+  GateRec G;
+  GatePlan P;
+  uint32_t ti = 0, L = 0, c0 = 0, clast = 0, n = 0;
+  const uint32_t* lw = nullptr;
+  const uint32_t* iw = nullptr;
+  bool narrow_in = false;  // input j of a gate of fan-in <= 4
+  if (valid) {
+    G = gates[g];
+    P = plan[g];
+    ti = (uint32_t)(t - tile_base[g]);
+    narrow_in = G.k <= kTileK && (uint32_t)j < G.k;
+    if (narrow_in) {
+      lw = pool + woff[G.in_net[P.lead]];
+      iw = pool + woff[G.in_net[j]];
+      n = wlen[G.in_net[j]];
+      c0 = ti * kLanes;
+      clast = min(c0 + kLanes - 1, P.nseg - 1);
+      if (c0 > 0) {
+        const uint32_t tau_lo = lw[c0 * P.stride] >> 2;
+        L = (uint32_t)j == P.lead ? c0 * P.stride : lb_words(iw, n, tau_lo << 2);
+      }
+    }
+  }
+  s_first[threadIdx.x] = L;
+  __syncthreads();
+  if (!valid) return;
   TileDesc& D = tiles[t];
   D.lutw[j] = lutp[(uint32_t)G.lut * 16u + j];
   if (j == 0) {
@@ -1211,24 +1243,19 @@ __global__ void k_tile_desc(const GateRec* __restrict__ gates, uint32_t ng,
     D.in_off[j] = 0;
     span = wlen[G.in_net[j]];
     if ((uint32_t)j + 4 < G.k) span += wlen[G.in_net[j + 4]];
-  } else if (j >= G.k) {
+  } else if (!narrow_in) {
     D.L[j] = 0;
     D.E[j] = 0;
     D.in_off[j] = 0;
   } else {
-    const uint32_t* lw = pool + woff[G.in_net[P.lead]];
-    const uint32_t* iw = pool + woff[G.in_net[j]];
-    const uint32_t n = wlen[G.in_net[j]];
-    const uint32_t c0 = ti * kLanes;
-    const uint32_t clast = min(c0 + kLanes - 1, P.nseg - 1);
-    uint32_t L = 0, E = n, Ln = n;
-    if (c0 > 0) {
-      const uint32_t tau_lo = lw[c0 * P.stride] >> 2;
-      L = (uint32_t)j == P.lead ? c0 * P.stride : lb_words(iw, n, tau_lo << 2);
-    }
+    uint32_t E = n, Ln = n;
     if (clast + 1 < P.nseg) {
+      // the next tile is this gate's: its first words end this one
       const uint32_t tau_nx = lw[(clast + 1) * P.stride] >> 2;
-      Ln = (uint32_t)j == P.lead ? (clast + 1) * P.stride : lb_words(iw, n, tau_nx << 2);
+      if (threadIdx.x + 4 < blockDim.x && t + 1 < nt)
+        Ln = s_first[threadIdx.x + 4];
+      else
+        Ln = (uint32_t)j == P.lead ? (clast + 1) * P.stride : lb_words(iw, n, tau_nx << 2);
       // halo end: the first word at or after tau_nx + dmax lies at or after
       // Ln, usually at it; step forward with doubling strides, then search
       // only inside the last stride (every word before a is below the key)
