@@ -298,14 +298,16 @@ def batched_vs_plain(ctx, n_sets, n_nets, scale):
                          "tuned_on_instances": len(sub)}}
 
 
-def serving_mix(ctx, n_jobs, scale):
+def serving_mix(ctx, n_jobs, scale, max_batch=4):
     """Serving loop on a seeded Poisson job stream: netlists log-uniform in
     [10k, 2M] x `scale` cells (the same generator), windows U[1, 20] Mps x
     `scale`, 30% of jobs reuse a netlist already submitted.  Every job is
     first run alone through the same server path (one job at a time); the
     arrival rate is set from the mean alone time for about 75% device
-    utilisation.  Reports events/s and p50/p99 job latency, and compares
-    every job's SAIF in the mix with its alone run."""
+    utilisation.  In the mix, up to `max_batch` queued jobs admitted
+    together run as one batch (level l of all of them in one launch).
+    Reports events/s and p50/p99 job latency, and compares every job's SAIF
+    in the mix with its alone run."""
     import gls
     rng = random.Random(0x5EED0004)
     designs, jobs = [], []
@@ -338,7 +340,7 @@ def serving_mix(ctx, n_jobs, scale):
     mean_s = sum(t_alone) / len(t_alone)
     rate = 0.75 / mean_s
     gaps = [rng.expovariate(rate) for _ in jobs]
-    srv = gls.Server(ctx, max_running=4)
+    srv = gls.Server(ctx, max_running=4, max_batch=max_batch)
     ids = []
     for (d, s, on, off), g in zip(jobs, gaps):
         time.sleep(g)
@@ -347,7 +349,8 @@ def serving_mix(ctx, n_jobs, scale):
     same = all(srv.saif_text(ids[k]) == alone[k] for k in alone)
     st = srv.stats()
     srv.close()
-    return {"workload": f"{n_jobs} Poisson jobs at {rate:.2f}/s, cells log-uniform in "
+    return {"max_batch": max_batch,
+            "workload": f"{n_jobs} Poisson jobs at {rate:.2f}/s, cells log-uniform in "
                         f"[{int(1e4 * scale)}, {int(2e6 * scale)}], windows U[{int(1e6 * scale)}, "
                         f"{int(2e7 * scale)}] ps, gap 500 ps, 30% netlist reuse",
             "jobs": int(st["done"]), "failed": int(st["failed"]), "all_ok": ok,
@@ -379,6 +382,8 @@ def main():
     ap.add_argument("--c3-scale", type=float, default=1.0)
     ap.add_argument("--c4-jobs", type=int, default=24)
     ap.add_argument("--c4-scale", type=float, default=1.0)
+    ap.add_argument("--c4-batch", type=int, default=4,
+                    help="serving mix: queued jobs admitted together run as one batch of up to this many")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="only run a plain device copy of this many MiB (3 times): the "
                          "reference point that scales FETCH_SIZE/WRITE_SIZE counters to bytes")
@@ -462,7 +467,7 @@ def main():
             out["batched"] = batched_vs_plain(ctx, max(1, int(256 * min(1.0, args.c3_scale))),
                                               max(1, int(64 * min(1.0, args.c3_scale))), args.c3_scale)
         elif x == "c4":
-            out["serving"] = serving_mix(ctx, args.c4_jobs, args.c4_scale)
+            out["serving"] = serving_mix(ctx, args.c4_jobs, args.c4_scale, args.c4_batch)
         log(f"{x}: done in {time.time() - t0:.1f}s")
     if skipped:
         out["extras_skipped_for_time"] = skipped

@@ -418,8 +418,9 @@ int gls_prepare_batch(gls_ctx* c, size_t n, const gls_design* const* designs,
   for (auto& in : r->inst) {
     SimInstance I;
     I.pg = in.pg.get();
-    // one instance: the context's resident image of the program
-    if (n == 1) I.image = c->cache.get(in.pg, (uint32_t)in.timescale_ps);
+    // the context's image of the program: built once per netlist, so the
+    // instances of one netlist share its truth and delay tables in the batch
+    I.image = c->cache.get(in.pg, (uint32_t)in.timescale_ps);
     I.st = in.st;
     I.stim_nets = in.stim_nets;
     I.start = in.start;
@@ -629,6 +630,15 @@ int gls_run_waveform64(const gls_run* r, const char* net, uint64_t* words, size_
 // Per-net FNV-1a over the name bytes then the little-endian 32-bit event
 // words ((time << 2 | value) truncated to 32 bits), finalised with
 // SplitMix64 and summed over gate-output nets.
+int gls_run_instance_events(const gls_run* r, size_t i, uint64_t* n) {
+  API_BEGIN
+  if (!r || !n) throw GlsError(GLS_E_ARG, "null argument");
+  if (!r->executed) throw GlsError(GLS_E_STATE, "run not executed");
+  inst_of(r, i);  // range check
+  *n = r->job->instance_events(i);
+  API_END
+}
+
 int gls_run_instance_waves_hash(const gls_run* r, size_t i, uint64_t* hash) {
   API_BEGIN
   if (!r || !hash) throw GlsError(GLS_E_ARG, "null argument");
@@ -740,6 +750,7 @@ int gls_server_create(gls_ctx* c, const gls_server_opts* o, gls_server** out) {
     so.max_running = o->max_running ? o->max_running : 4;
     so.seg_events = o->seg_events;
     so.pool_factor = o->pool_factor > 0 ? o->pool_factor : 1.0;
+    so.max_batch = o->max_batch ? o->max_batch : 1;
   }
   auto s = std::make_unique<gls_server>();
   s->srv = std::make_unique<Server>(c->device, so, &c->cache);

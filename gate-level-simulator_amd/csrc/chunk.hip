@@ -273,6 +273,19 @@ hipError_t launch_tail_copy(const GateRec* gates, uint32_t ng, const uint64_t* l
   return hipGetLastError();
 }
 
+// dst[i] += src[i]: a chunk's per-instance gate events join the run's.
+__global__ void k_add_u64(unsigned long long* dst, const unsigned long long* src, uint64_t n) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dst[i] += src[i];
+}
+
+hipError_t launch_add_u64(unsigned long long* dst, const unsigned long long* src, uint64_t n,
+                          hipStream_t st) {
+  if (n == 0) return hipSuccess;
+  k_add_u64<<<(unsigned)((n + 255) / 256), 256, 0, st>>>(dst, src, n);
+  return hipGetLastError();
+}
+
 hipError_t launch_saif_stim(const uint64_t* store, const uint64_t* soff, const uint32_t* slen,
                             const uint32_t* snet, const uint32_t* sjob, uint32_t ns,
                             unsigned long long* saif, const unsigned long long* win,

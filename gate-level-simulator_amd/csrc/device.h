@@ -108,7 +108,7 @@ struct alignas(16) TileOut {
   uint32_t ft, lt;      // times of the first and last entries
   uint32_t flags;       // first value | last value << 2 | has entries << 4
   uint32_t t0, t1, tx;  // SAIF durations between the tile's own entries (clipped)
-  uint32_t tc;          // reserved (0); toggle counts inside tiles are added by the compaction
+  uint32_t ev;          // gate events of the tile (input-event times from the chunk start on)
 };
 static_assert(sizeof(TileOut) == 32, "TileOut layout");
 

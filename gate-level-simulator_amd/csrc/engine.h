@@ -175,6 +175,8 @@ class Job {
   size_t n_instances() const { return net_base_.size(); }
   uint32_t net_base(size_t i) const { return net_base_[i]; }
   uint64_t device_bytes() const { return dev_bytes_; }
+  // gate events of instance i in the last run
+  uint64_t instance_events(size_t i) const { return i < h_evjob_.size() ? h_evjob_[i] : 0; }
   // chunk boundaries of the last run (F_1 .. F_K; the last is kChunkEnd)
   const std::vector<uint64_t>& chunks() const { return sched_; }
 
@@ -302,6 +304,9 @@ class Job {
   size_t cap_tails_ = 0;
   uint64_t *d_tlen_g_ = nullptr, *d_tsrc_g_ = nullptr, *d_toff_g_ = nullptr, *d_gpart_ = nullptr;
   unsigned long long* d_saif_chunk_ = nullptr;
+  // batches: gate events per instance (run total and current chunk)
+  unsigned long long *d_evjob_ = nullptr, *d_evjob_chunk_ = nullptr;
+  std::vector<uint64_t> h_evjob_;
   unsigned long long* d_wtot_ = nullptr;
   ChunkWin* d_cwin_ = nullptr;
   size_t cap_cwin_ = 0;

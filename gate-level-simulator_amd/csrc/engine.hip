@@ -57,7 +57,9 @@ hipError_t launch_gate_carry(const GateRec* gates, uint32_t ng, const uint64_t* 
                              const uint64_t* ntile, const TileOut* tout, uint32_t* tile_off,
                              uint64_t* gate_cnt, uint32_t* wlen, unsigned long long* saif,
                              const ChunkWin* cwin, const unsigned int* err, const TailArgs& T,
-                             hipStream_t st);
+                             unsigned long long* ev_job, hipStream_t st);
+hipError_t launch_add_u64(unsigned long long* dst, const unsigned long long* src, uint64_t n,
+                          hipStream_t st);
 hipError_t launch_compact(const TileDesc* tiles, const uint64_t* ntiles_dev, uint64_t ntiles_cap,
                           const TileOut* tout, const uint32_t* tile_off, const uint64_t* scr_base,
                           const uint32_t* scratch, const uint64_t* gate_base, uint32_t* pool,
@@ -849,6 +851,11 @@ void Job::prepare(const std::vector<SimInstance>& inst, uint64_t pool_words) {
     d_gpart_ = (uint64_t*)dmalloc(scan_part_size(n_gates_) * 8ull);
     d_saif_chunk_ = (unsigned long long*)dmalloc((size_t)n_nets_ * kSaifWords * 8);
     d_wtot_ = (unsigned long long*)dmalloc(n_nets_ * 8ull);
+    d_evjob_ = d_evjob_chunk_ = nullptr;
+    if (inst.size() > 1) {  // gate events per instance
+      d_evjob_ = (unsigned long long*)dmalloc(inst.size() * 8ull);
+      d_evjob_chunk_ = (unsigned long long*)dmalloc(inst.size() * 8ull);
+    }
     ensure((void**)&d_tails_, cap_tails_, std::max<size_t>(n_gates_, 1) * 4);
     ensure((void**)&d_cwin_, cap_cwin_, inst.size() * sizeof(ChunkWin));
     stim_words_ = store_words + n_sig_;  // one chunk: every entry and a state entry per signal
@@ -992,6 +999,7 @@ int Job::run_chunk(const Chunk& c, std::vector<hipEvent_t>& ev_eval, bool device
   T.first = c.first;
   *grow_hint = 0;
 
+  if (d_evjob_chunk_) HC(launch_fill_u64(d_evjob_chunk_, net_base_.size(), 0ull, stream_));
   // the chunk's stimulus at the start of the pool
   HC(launch_stim_count(d_store_, d_soff_, d_slen_, n_sig_, c.lb, c.Fn == kChunkEnd ? ~0ull : c.Fn,
                        d_scnt_, d_sfirst_, stream_));
@@ -1177,7 +1185,7 @@ int Job::run_chunk(const Chunk& c, std::vector<hipEvent_t>& ev_eval, bool device
     }
     // gate counts go to d_bound_ (free again once the scratch bases are scanned)
     HC(launch_gate_carry(gates, ng, d_tilebase_, d_ntile_, d_tout_, d_toff_, d_bound_, d_wlen_,
-                         d_saif_chunk_, cw, err, T, stream_));
+                         d_saif_chunk_, cw, err, T, d_evjob_chunk_, stream_));
     HC(launch_scan(d_bound_, d_outoff_, ng, d_part_, d_scal_ + S_TOT_OUT, bump, stream_));
     HC(launch_pool_guard(d_scal_, pool_limit, stream_));
     HC(launch_compact(d_tiles_, d_scal_ + S_TOT_A, ntiles, d_tout_, d_toff_, d_scrbase_, scratch,
@@ -1220,6 +1228,7 @@ void Job::fetch_chunk_waves(const Chunk& c) {
 void Job::end_chunk(const Chunk& c, bool device_driven) {
   HC(launch_saif_accum(prog_->d_grec, n_gates_, d_saif_chunk_, d_saif_, d_wlen_, d_tail_len_, d_wtot_,
                        stream_));
+  if (d_evjob_) HC(launch_add_u64(d_evjob_, d_evjob_chunk_, net_base_.size(), stream_));
   if (!device_driven && keep_waves) fetch_chunk_waves(c);
   if (c.Fn == kChunkEnd || n_gates_ == 0) return;
   const uint32_t key = (uint32_t)((c.Fn - wmax_) - c.lb + 1);  // lb of the next chunk
@@ -1247,6 +1256,7 @@ void Job::run_chunks(std::vector<hipEvent_t>& ev_eval) {
   HC(launch_set_scal(d_scal_, 0, stream_));
   HC(launch_fill_u64(d_saif_, (uint64_t)n_nets_ * kSaifWords, 0ull, stream_));
   HC(launch_fill_u64(d_wtot_, n_nets_, 0ull, stream_));
+  if (d_evjob_) HC(launch_fill_u64(d_evjob_, net_base_.size(), 0ull, stream_));
   HC(launch_saif_stim(d_store_, d_soff_, d_slen_, d_snet_, d_sjob_, n_sig_, d_saif_, d_win_, stream_));
   ensure((void**)&d_tails_, cap_tails_, std::max<size_t>(n_gates_, 1) * 4);
   HC(launch_tails_init(prog_->d_grec, n_gates_, d_tail_len_, d_tail_off_, d_tails_, stream_));
@@ -1629,6 +1639,7 @@ void Job::build_graph() {
     HC(launch_set_scal(d_scal_, 0, stream_));
     HC(launch_fill_u64(d_saif_, (uint64_t)n_nets_ * kSaifWords, 0ull, stream_));
     HC(launch_fill_u64(d_wtot_, n_nets_, 0ull, stream_));
+    if (d_evjob_) HC(launch_fill_u64(d_evjob_, net_base_.size(), 0ull, stream_));
     HC(launch_saif_stim(d_store_, d_soff_, d_slen_, d_snet_, d_sjob_, n_sig_, d_saif_, d_win_,
                         stream_));
     HC(launch_tails_init(prog_->d_grec, n_gates_, d_tail_len_, d_tail_off_, d_tails_, stream_));
@@ -1660,6 +1671,13 @@ void Job::finish_stats(const uint64_t* h, double ms) {
     throw GlsError(GLS_E_STATE, "a gate's input waveform was out of time order; results discarded");
   stats_.kernel_ms = ms;
   stats_.n_events = h[S_EVC];
+  h_evjob_.assign(net_base_.size(), 0);
+  if (d_evjob_) {
+    HC(hipMemcpyAsync(h_evjob_.data(), d_evjob_, h_evjob_.size() * 8, hipMemcpyDeviceToHost, stream_));
+    HC(hipStreamSynchronize(stream_));
+  } else if (!h_evjob_.empty()) {
+    h_evjob_[0] = stats_.n_events;
+  }
   stats_.pool_words_used = mode == 1 ? h[S_BUMP] : max_bump_;
   stats_.tiles = h[S_TILES];
   stats_.tiles_hbm = h[S_TILES_HBM];

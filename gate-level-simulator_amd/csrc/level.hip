@@ -643,7 +643,7 @@ __device__ __forceinline__ void emit_tile(const LevelArgs& A, uint64_t tk, const
     o.t0 = t0;
     o.t1 = t1;
     o.tx = tx;
-    o.tc = 0;  // counted by k_compact when toggle counts are asked for
+    o.ev = ev;
     A.tout[tk] = o;
     wave_ev += ev;
   }
@@ -792,6 +792,8 @@ __global__ __launch_bounds__(kLanes) void k_level(LevelArgs A) {
 // it (a ballot of the non-empty lanes, then a shuffle; the step's first
 // non-empty tile takes them from the previous step), the offsets are an
 // exclusive scan of the joined counts, and the sums are reduced at the end.
+// In a batch (ev_job set) the gate's events are added to its instance's
+// count for the chunk.
 __device__ __forceinline__ unsigned long long wave_sum_u64(unsigned long long x) {
 #pragma unroll
   for (int d = 1; d < 64; d <<= 1) x += __shfl_xor(x, d);
@@ -804,7 +806,7 @@ __global__ void k_gate_carry(const GateRec* __restrict__ gates, uint32_t ng,
                              uint32_t* __restrict__ tile_off, uint64_t* __restrict__ gate_cnt,
                              uint32_t* __restrict__ wlen, unsigned long long* __restrict__ saif,
                              const ChunkWin* __restrict__ cwin, const unsigned int* __restrict__ err,
-                             TailArgs T) {
+                             TailArgs T, unsigned long long* __restrict__ ev_job) {
   const int lane = threadIdx.x & 63;
   const uint32_t g = blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64;
   if (g >= ng) return;  // uniform over the wavefront
@@ -830,7 +832,7 @@ __global__ void k_gate_carry(const GateRec* __restrict__ gates, uint32_t ng,
     lv = T.tails[T.tail_off[G.out_net] + tl - 1] & 3u;
     has = 1;
   }
-  unsigned long long acc[3] = {0, 0, 0}, tc = 0;
+  unsigned long long acc[3] = {0, 0, 0}, tc = 0, gev = 0;
   for (uint64_t c = 0; c < n; c += 64) {
     const bool in = c + (uint64_t)lane < n;
     const uint64_t t = t0 + c + (uint64_t)lane;
@@ -854,7 +856,8 @@ __global__ void k_gate_carry(const GateRec* __restrict__ gates, uint32_t ng,
       acc[1] += S.t1;
       acc[2] += S.tx;
       // the tile's first entry against the previous tile's last value
-      tc += S.tc + ((p_has && is_toggle(p_lv, my_fv) && S.ft > W.tc_lo && S.ft <= W.tc_hi) ? 1u : 0u);
+      tc += (p_has && is_toggle(p_lv, my_fv) && S.ft > W.tc_lo && S.ft <= W.tc_hi) ? 1u : 0u;
+      gev += S.ev;
     }
     // inclusive scan of the joined counts over the step
     uint32_t incl = mine;
@@ -875,6 +878,7 @@ __global__ void k_gate_carry(const GateRec* __restrict__ gates, uint32_t ng,
   if (has && lane == 0) saif_acc(acc, lv, lt, end > lt ? end : (unsigned long long)lt, start, end);
   const unsigned long long s0 = wave_sum_u64(acc[0]), s1 = wave_sum_u64(acc[1]);
   const unsigned long long s2 = wave_sum_u64(acc[2]), s3 = wave_sum_u64(tc);
+  const unsigned long long s4 = ev_job ? wave_sum_u64(gev) : 0ull;
   if (lane == 0) {
     gate_cnt[g] = cnt;
     wlen[G.out_net] = cnt;
@@ -883,6 +887,8 @@ __global__ void k_gate_carry(const GateRec* __restrict__ gates, uint32_t ng,
     sa[1] = s1;
     sa[2] = s2;
     sa[3] = s3;
+    // a level whose tiles ran out of overflow stacks runs again: count it then
+    if (ev_job && s4 && !(*err & ERR_OVERFLOW)) atomicAdd(ev_job + G.job, s4);
   }
 }
 
@@ -1087,7 +1093,7 @@ __global__ void k_wide(WideArgs A) {
   o.t0 = a0;
   o.t1 = a1;
   o.tx = pct - ct0 - a0 - a1;
-  o.tc = 0;
+  o.ev = nown;
   A.tout[tk] = o;
   if (cnt > cap || bad) atomicOr(A.err, ERR_ORDER);
   if (nown) atomicAdd(A.ev_count, (unsigned long long)nown);
@@ -1346,10 +1352,10 @@ hipError_t launch_gate_carry(const GateRec* gates, uint32_t ng, const uint64_t* 
                              const uint64_t* ntile, const TileOut* tout, uint32_t* tile_off,
                              uint64_t* gate_cnt, uint32_t* wlen, unsigned long long* saif,
                              const ChunkWin* cwin, const unsigned int* err, const TailArgs& T,
-                             hipStream_t st) {
+                             unsigned long long* ev_job, hipStream_t st) {
   if (ng == 0) return hipSuccess;
   k_gate_carry<<<(ng + 3) / 4, 256, 0, st>>>(gates, ng, tile_base, ntile, tout, tile_off, gate_cnt,
-                                             wlen, saif, cwin, err, T);  // one wavefront per gate
+                                             wlen, saif, cwin, err, T, ev_job);  // one wavefront per gate
   return hipGetLastError();
 }
 

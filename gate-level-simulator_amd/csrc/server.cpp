@@ -94,7 +94,7 @@ void Server::worker(int slot) {
   hipStream_t stream = nullptr;
   hip_check(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking), "hipStreamCreate");
   for (;;) {
-    Rec* r = nullptr;
+    std::vector<Rec*> group;
     {
       std::unique_lock<std::mutex> lk(mu_);
       for (;;) {
@@ -122,6 +122,9 @@ void Server::worker(int slot) {
         // admission: the oldest queued job whose footprint fits what is free;
         // a job may pass the head of the queue only while the head does not
         // fit.  A job that can never fit fails and the scan goes on past it.
+        // With batching, later queued jobs without a deadline that also fit
+        // join the first one's run.
+        uint64_t taken = 0;
         for (auto it = queue_.begin(); it != queue_.end();) {
           Rec* c = jobs_[*it].get();
           bool resident = false;
@@ -140,93 +143,181 @@ void Server::worker(int slot) {
             cv_done_.notify_all();
             continue;
           }
-          if (in_use_ + need <= opt_.device_budget) {
-            r = c;
-            queue_.erase(it);
-            break;
+          const bool joinable = group.empty() || (!c->has_deadline && !group[0]->has_deadline);
+          if (joinable && in_use_ + taken + need <= opt_.device_budget) {
+            group.push_back(c);
+            taken += need;
+            it = queue_.erase(it);
+            if (group.size() >= std::max<uint32_t>(1, opt_.max_batch)) break;
+            continue;
+          }
+          if (group.empty()) {
+            ++it;
+            continue;
           }
           ++it;
         }
-        if (r) break;
+        if (!group.empty()) break;
         if (any_deadline) cv_work_.wait_until(lk, next_deadline);
         else cv_work_.wait(lk);
       }
-      if (!r) break;
-      in_use_ += r->reserve;
-      peak_bytes_ = std::max(peak_bytes_, in_use_);
-      ++running_;
-      peak_running_ = std::max(peak_running_, running_);
-      r->info.overlap = running_ - 1;
-      r->info.status = 2;
-      r->t_start = Clock::now();
-    }
-    int status = 0;
-    uint64_t events = 0, bytes = 0;
-    uint32_t attempts = 0;
-    std::vector<uint64_t> saif;
-    try {
-      const RunInst& in = r->inst;
-      SimInstance I;
-      I.pg = in.pg.get();
-      I.image = cache_->get(in.pg, (uint32_t)in.timescale_ps);
-      I.st = in.st;
-      I.stim_nets = in.stim_nets;
-      I.start = in.start;
-      I.end = in.end;
-      uint64_t stim = 0;
-      for (auto& w : in.st->waves) stim += w.size();
-      const uint64_t n_in = std::max<uint64_t>(1, in.st->waves.size());
-      uint64_t pool = stim + (uint64_t)(opt_.pool_factor * 10.0 * (double)in.pg->gates.size() *
-                                        (double)(stim / n_in + 1)) + (1u << 20);
-      for (uint32_t attempt = 0; attempt < 2; ++attempt) {
-        ++attempts;
-        try {
-          Job job(device_, stream, arena_.get());
-          job.seg_target = opt_.seg_events;
-          job.should_stop = [this, r] { return expired(*r, Clock::now()); };
-          if (job.should_stop()) throw GlsError(GLS_E_CANCELLED, "job cancelled or past its deadline");
-          job.prepare({I}, pool);
-          job.run();
-          job.fetch_saif(saif);
-          events = job.stats().n_events;
-          bytes = job.device_bytes();
-          status = 0;
-          break;
-        } catch (const GlsError& e) {
-          status = e.code;
-          if (e.code != GLS_E_OOM || attempt == 1) break;
-          pool *= 2;  // the estimate was short: one retry with a larger pool
-          std::lock_guard<std::mutex> lk(mu_);
-          const uint64_t more = pool * 4 / 2;
-          if (in_use_ + more > opt_.device_budget) break;
-          in_use_ += more;
-          r->reserve += more;
-          peak_bytes_ = std::max(peak_bytes_, in_use_);
-        }
+      if (group.empty()) break;
+      const Clock::time_point now = Clock::now();
+      for (Rec* r : group) {
+        in_use_ += r->reserve;
+        ++running_;
+        r->info.status = 2;
+        r->t_start = now;
       }
-    } catch (const GlsError& e) {
-      status = e.code;
-    } catch (const std::exception&) {
-      status = GLS_E_DEVICE;
+      peak_bytes_ = std::max(peak_bytes_, in_use_);
+      peak_running_ = std::max(peak_running_, running_);
+      for (Rec* r : group) r->info.overlap = running_ - 1;
     }
-    {
-      std::lock_guard<std::mutex> lk(mu_);
-      r->t_end = Clock::now();
-      r->info.status = status;
-      r->info.n_events = events;
-      r->info.device_bytes = bytes;
-      r->info.attempts = attempts;
-      r->info.queue_ms = ms_between(r->t_submit, r->t_start);
-      r->info.run_ms = ms_between(r->t_start, r->t_end);
-      r->info.latency_ms = ms_between(r->t_submit, r->t_end);
-      r->saif = std::move(saif);
-      in_use_ -= r->reserve;
-      --running_;
+    if (group.size() > 1 && run_group(group, stream)) {
+      cv_done_.notify_all();
+      cv_work_.notify_all();
+      continue;
     }
-    cv_done_.notify_all();
-    cv_work_.notify_all();
+    for (Rec* r : group) {
+      int status = 0;
+      uint64_t events = 0, bytes = 0;
+      uint32_t attempts = 0;
+      std::vector<uint64_t> saif;
+      run_one(r, stream, status, events, bytes, attempts, saif);
+      finish(r, status, events, bytes, attempts, std::move(saif));
+      cv_done_.notify_all();
+      cv_work_.notify_all();
+    }
   }
   (void)hipStreamDestroy(stream);
+}
+
+// Event-pool words of a job: its stimulus plus an estimate of the level
+// outputs (each gate reserves about ten times its inputs' mean events).
+static uint64_t job_pool_words(const RunInst& in, double pool_factor) {
+  uint64_t stim = 0;
+  for (auto& w : in.st->waves) stim += w.size();
+  const uint64_t n_in = std::max<uint64_t>(1, in.st->waves.size());
+  return stim + (uint64_t)(pool_factor * 10.0 * (double)in.pg->gates.size() * (double)(stim / n_in + 1)) +
+         (1u << 20);
+}
+
+void Server::run_one(Rec* r, hipStream_t stream, int& status, uint64_t& events, uint64_t& bytes,
+                     uint32_t& attempts, std::vector<uint64_t>& saif) {
+  status = 0;
+  try {
+    const RunInst& in = r->inst;
+    SimInstance I;
+    I.pg = in.pg.get();
+    I.image = cache_->get(in.pg, (uint32_t)in.timescale_ps);
+    I.st = in.st;
+    I.stim_nets = in.stim_nets;
+    I.start = in.start;
+    I.end = in.end;
+    uint64_t pool = job_pool_words(in, opt_.pool_factor);
+    for (uint32_t attempt = 0; attempt < 2; ++attempt) {
+      ++attempts;
+      try {
+        Job job(device_, stream, arena_.get());
+        job.seg_target = opt_.seg_events;
+        job.should_stop = [this, r] { return expired(*r, Clock::now()); };
+        if (job.should_stop()) throw GlsError(GLS_E_CANCELLED, "job cancelled or past its deadline");
+        job.prepare({I}, pool);
+        job.run();
+        job.fetch_saif(saif);
+        events = job.stats().n_events;
+        bytes = job.device_bytes();
+        status = 0;
+        break;
+      } catch (const GlsError& e) {
+        status = e.code;
+        if (e.code != GLS_E_OOM || attempt == 1) break;
+        pool *= 2;  // the estimate was short: one retry with a larger pool
+        std::lock_guard<std::mutex> lk(mu_);
+        const uint64_t more = pool * 4 / 2;
+        if (in_use_ + more > opt_.device_budget) break;
+        in_use_ += more;
+        r->reserve += more;
+        peak_bytes_ = std::max(peak_bytes_, in_use_);
+      }
+    }
+  } catch (const GlsError& e) {
+    status = e.code;
+  } catch (const std::exception&) {
+    status = GLS_E_DEVICE;
+  }
+}
+
+bool Server::run_group(const std::vector<Rec*>& group, hipStream_t stream) {
+  std::vector<uint64_t> saif;
+  std::vector<uint64_t> events(group.size()), base(group.size());
+  uint64_t bytes = 0;
+  try {
+    std::vector<SimInstance> I(group.size());
+    uint64_t pool = 0;
+    for (size_t i = 0; i < group.size(); ++i) {
+      const RunInst& in = group[i]->inst;
+      I[i].pg = in.pg.get();
+      I[i].image = cache_->get(in.pg, (uint32_t)in.timescale_ps);
+      I[i].st = in.st;
+      I[i].stim_nets = in.stim_nets;
+      I[i].start = in.start;
+      I[i].end = in.end;
+      pool += job_pool_words(in, opt_.pool_factor);
+    }
+    Job job(device_, stream, arena_.get());
+    job.seg_target = opt_.seg_events;
+    // the batch stops early only when every job in it was cancelled
+    job.should_stop = [this, &group] {
+      const Clock::time_point now = Clock::now();
+      for (Rec* r : group)
+        if (!expired(*r, now)) return false;
+      return true;
+    };
+    job.prepare(I, pool);
+    job.run();
+    job.fetch_saif(saif);
+    for (size_t i = 0; i < group.size(); ++i) {
+      events[i] = job.instance_events(i);
+      base[i] = job.net_base(i);
+    }
+    bytes = job.device_bytes();
+  } catch (const GlsError& e) {
+    if (e.code == GLS_E_CANCELLED) {
+      for (Rec* r : group) finish(r, GLS_E_CANCELLED, 0, 0, 1, {});
+      return true;
+    }
+    return false;  // run the jobs one at a time
+  } catch (const std::exception&) {
+    return false;
+  }
+  const Clock::time_point now = Clock::now();
+  for (size_t i = 0; i < group.size(); ++i) {
+    Rec* r = group[i];
+    const size_t n = r->inst.names.size();
+    std::vector<uint64_t> mine(saif.begin() + (size_t)kSaifWords * base[i],
+                               saif.begin() + (size_t)kSaifWords * (base[i] + n));
+    // a job cancelled (or past its deadline) while its batch ran keeps no result
+    if (expired(*r, now)) finish(r, GLS_E_CANCELLED, 0, 0, 1, {});
+    else finish(r, 0, events[i], bytes / group.size(), 1, std::move(mine));
+  }
+  return true;
+}
+
+void Server::finish(Rec* r, int status, uint64_t events, uint64_t bytes, uint32_t attempts,
+                    std::vector<uint64_t> saif) {
+  std::lock_guard<std::mutex> lk(mu_);
+  r->t_end = Clock::now();
+  r->info.status = status;
+  r->info.n_events = events;
+  r->info.device_bytes = bytes;
+  r->info.attempts = attempts;
+  r->info.queue_ms = ms_between(r->t_submit, r->t_start);
+  r->info.run_ms = ms_between(r->t_start, r->t_end);
+  r->info.latency_ms = ms_between(r->t_submit, r->t_end);
+  r->saif = std::move(saif);
+  in_use_ -= r->reserve;
+  --running_;
 }
 
 void Server::fail_queued(Rec* r) {

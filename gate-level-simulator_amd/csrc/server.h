@@ -45,6 +45,9 @@ struct ServerOptions {
   uint32_t max_running = 4;    // jobs on the device at once
   uint32_t seg_events = 0;     // fused-kernel segment size (0: default)
   double pool_factor = 1.0;    // scales the event-pool estimate of a job
+  // queued jobs admitted together run as one batch of up to this many
+  // instances (level l of all of them in one launch); 0 or 1: one job per run
+  uint32_t max_batch = 1;
 };
 
 struct ServerJobInfo {
@@ -100,6 +103,15 @@ class Server {
   }
   void fail_queued(Rec* r);  // caller holds mu_
   void worker(int slot);
+  // Runs one job (one retry with twice the pool when its estimate was
+  // short); fills status, events, device bytes, attempts and SAIF.
+  void run_one(Rec* r, hipStream_t stream, int& status, uint64_t& events, uint64_t& bytes,
+               uint32_t& attempts, std::vector<uint64_t>& saif);
+  // Runs jobs admitted together as one batch; false when the batch failed
+  // (its jobs are then run one at a time).
+  bool run_group(const std::vector<Rec*>& group, hipStream_t stream);
+  void finish(Rec* r, int status, uint64_t events, uint64_t bytes, uint32_t attempts,
+              std::vector<uint64_t> saif);
   uint64_t estimate(const Rec& r, bool* resident) const;
   int device_;
   ServerOptions opt_;

@@ -75,7 +75,8 @@ class TuneReport(ctypes.Structure):
 
 class ServerOpts(ctypes.Structure):
     _fields_ = [("device_budget", ctypes.c_uint64), ("max_running", ctypes.c_uint32),
-                ("seg_events", ctypes.c_uint32), ("pool_factor", ctypes.c_double)]
+                ("seg_events", ctypes.c_uint32), ("pool_factor", ctypes.c_double),
+                ("max_batch", ctypes.c_uint32), ("reserved0", ctypes.c_uint32)]
 
 
 class JobInfo(ctypes.Structure):
@@ -143,6 +144,7 @@ SIGNATURES = [
     ("gls_prepare_batch", _I, [_P, ctypes.c_size_t, _PP, _PP, ctypes.POINTER(_U64),
                                ctypes.POINTER(_U64), ctypes.POINTER(SimOpts), _PP]),
     ("gls_run_instances", _I, [_P, ctypes.POINTER(ctypes.c_size_t)]),
+    ("gls_run_instance_events", _I, [_P, ctypes.c_size_t, ctypes.POINTER(_U64)]),
     ("gls_run_instance_saif_text", _I, [_P, ctypes.c_size_t, ctypes.c_char_p, ctypes.c_size_t,
                                         ctypes.POINTER(ctypes.c_size_t)]),
     ("gls_run_instance_waves_hash", _I, [_P, ctypes.c_size_t, ctypes.POINTER(_U64)]),
@@ -372,6 +374,11 @@ class Run(_Handle):
         _check(load().gls_run_instance_saif_text(self.ptr, i, buf, n.value, ctypes.byref(n)))
         return buf.raw[:n.value].decode()
 
+    def instance_events(self, i):
+        n = _U64()
+        _check(load().gls_run_instance_events(self.ptr, i, ctypes.byref(n)))
+        return n.value
+
     def instance_waves_hash(self, i):
         h = _U64()
         _check(load().gls_run_instance_waves_hash(self.ptr, i, ctypes.byref(h)))
@@ -390,9 +397,10 @@ class Server(_Handle):
     """Serving loop over one device (see gls_server_* in include/gls.h)."""
     _free = "gls_server_destroy"
 
-    def __init__(self, ctx, device_budget=0, max_running=4, seg_events=0, pool_factor=1.0):
+    def __init__(self, ctx, device_budget=0, max_running=4, seg_events=0, pool_factor=1.0,
+                 max_batch=1):
         h = ctypes.c_void_p()
-        o = ServerOpts(device_budget, max_running, seg_events, pool_factor)
+        o = ServerOpts(device_budget, max_running, seg_events, pool_factor, max_batch, 0)
         _check(load().gls_server_create(ctx.ptr, ctypes.byref(o), ctypes.byref(h)))
         self._keep = [ctx]
         super().__init__(h.value)

@@ -225,6 +225,9 @@ int gls_prepare_batch(gls_ctx* c, size_t n, const gls_design* const* designs,
 int gls_run_instances(const gls_run* r, size_t* n);
 int gls_run_instance_saif_text(const gls_run* r, size_t i, char* buf, size_t cap, size_t* len);
 int gls_run_instance_waves_hash(const gls_run* r, size_t i, uint64_t* hash);
+/* Gate events of instance i in the last execution (the run's total for a
+ * one-instance run). */
+int gls_run_instance_events(const gls_run* r, size_t i, uint64_t* n);
 
 /* Autotuner: executes the prepared run once per candidate of the fused
  * kernel's events per segment (LDS tile size), workgroups per launch and
@@ -253,6 +256,9 @@ typedef struct {
   uint32_t max_running;   /* jobs on the device at once (0: 4) */
   uint32_t seg_events;    /* fused-kernel events per segment (0: default) */
   double pool_factor;     /* scales a job's event-pool estimate (0: 1) */
+  uint32_t max_batch;     /* queued jobs admitted together run as one batch of up
+                             to this many (level l of all in one launch); 0, 1: none */
+  uint32_t reserved0;
 } gls_server_opts;
 typedef struct {
   int status;             /* 1 queued, 2 running, 0 done, < 0 GLS_E_* failure */

@@ -65,6 +65,7 @@ def test_batch_matches_plain_runs(ctx):
         plain = ctx.simulate(d, s, on, off, keep_waves=True)
         assert run.instance_saif_text(k) == plain.saif_text(), k
         assert run.instance_waves_hash(k) == plain.waves_hash(), k
+        assert run.instance_events(k) == plain.stats()["n_events"], k
         if name:
             g = GOLDEN[name]
             assert "%016x" % run.instance_waves_hash(k) == g["waves_hash"]
@@ -141,6 +142,29 @@ def test_server_results_same_alone_and_in_mix(ctx):
     infos = [srv.info(j) for j in jobs]
     assert any(i["resident"] for i in infos[len(insts):])
     assert any(i["overlap"] for i in infos)
+    srv.close()
+
+
+def test_server_batches_queued_jobs(ctx):
+    # one worker, up to 8 queued jobs admitted together run as one batch
+    # (level l of every job in one launch): each job's SAIF and gate events
+    # equal the job run alone
+    insts = mixed_instances()
+    alone = []
+    for d, s, on, off, _ in insts:
+        r = ctx.simulate(d, s, on, off)
+        alone.append((r.saif_text(), r.stats()["n_events"]))
+        r.close()
+    srv = gls.Server(ctx, max_running=1, max_batch=8)
+    jobs = [srv.submit(d, s, on, off) for rep in range(2) for d, s, on, off, _ in insts]
+    for k, j in enumerate(jobs):
+        assert srv.wait(j) == 0, k
+        assert srv.saif_text(j) == alone[k % len(insts)][0], k
+        assert srv.info(j)["n_events"] == alone[k % len(insts)][1], k
+    st = srv.stats()
+    say("server batches", st)
+    assert st["done"] == len(jobs) and st["failed"] == 0
+    assert st["peak_running"] >= 2  # queued jobs ran together
     srv.close()
 
 

@@ -121,6 +121,7 @@ def test_c3_mix_batched_equals_plain_and_oracle(ctx, tmp_path):
     for k in range(n):
         assert b.instance_waves_hash(k) == plain[k][1], k
         assert b.instance_saif_text(k) == plain[k][2], k
+        assert b.instance_events(k) == plain[k][0], k
     assert b.stats()["n_events"] == sum(p[0] for p in plain)
     b.close()
     g = ctx.prepare_batch([x[0] for x in insts], [x[1] for x in insts], on, off, graph=True)
@@ -128,6 +129,7 @@ def test_c3_mix_batched_equals_plain_and_oracle(ctx, tmp_path):
         g.execute()
         for k in range(n):
             assert g.instance_saif_text(k) == plain[k][2], (rep, k)
+            assert g.instance_events(k) == plain[k][0], (rep, k)
     assert g.stats()["eval_launches"] == 1  # the second execution was a replay
     g.close()
     say("c3 batch == plain")
