@@ -215,8 +215,9 @@ def batched_vs_plain(ctx, n_sets, n_nets, scale):
     mean gap U[100, 5000] ps), plus `n_nets` netlists with log-uniform cell
     counts (5k-500k x scale) and depths U{10..60}, one stimulus set each.
     Batched: every instance in one prepared run (level l of all instances in
-    one launch, time chunks as HBM requires), the chunk and level sequence
-    captured as one hipGraph, kernel settings autotuned on the batch.
+    one launch; instance groups that each fit the event pool, time chunks
+    inside a group only as HBM requires), the whole group, chunk and level
+    sequence captured as one hipGraph, kernel settings autotuned on the batch.
     Plain: every instance prepared and executed alone, one after the other,
     with the same tuned settings (and, separately, the default settings).
     Every instance's SAIF is compared with its plain run."""
@@ -292,8 +293,8 @@ def batched_vs_plain(ctx, n_sets, n_nets, scale):
             "speedup_vs_default_plain": round(dflt_s / batch_s, 3),
             "saif_bit_exact_vs_plain": batch_saif == tuned_saif,
             "events_same_as_plain": bst["n_events"] == tuned_ev,
-            "form": "one captured hipGraph of all chunks and levels" if bst["eval_launches"] == 1
-                    else "level-synchronous launches",
+            "form": "one captured hipGraph of every instance group's chunks and levels"
+                    if bst["eval_launches"] == 1 else "level-synchronous launches",
             "autotune": {**{k: tune[k] for k in ("seg_events", "level_blocks", "prefetch", "best_ms")},
                          "tuned_on_instances": len(sub)}}
 

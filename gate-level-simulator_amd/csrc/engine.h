@@ -212,6 +212,13 @@ class Job {
     size_t slot;          // ChunkWin slot of the chunk's windows
   };
   enum ChunkResult { CHUNK_OK = 0, CHUNK_SHORTER = 1 };
+  // An instance group: instances [i0, i1), their gates [g0, g1) in levels
+  // [L0, L1) of the level table, their stimulus signals [s0, s1), and the
+  // time after which none of their output entries lies.
+  struct Group {
+    uint32_t i0, i1, L0, L1, g0, g1, s0, s1;
+    uint64_t horizon;
+  };
   // one stimulus signal of the resident store: n (time, value) entries
   struct SigSrc {
     const uint32_t* t;
@@ -226,10 +233,11 @@ class Job {
   bool try_ensure(void** p, size_t& cap, size_t need);
   Chunk make_chunk(size_t k, uint64_t F, uint64_t Fn, uint64_t prev_lb) const;
   void put_chunk_windows(const Chunk& c, bool capture);
-  int run_chunk(const Chunk& c, std::vector<hipEvent_t>& ev_eval, bool device_driven,
-                double* grow_hint);
-  void end_chunk(const Chunk& c, bool device_driven);
+  int run_chunk(const Chunk& c, const Group& G, std::vector<hipEvent_t>& ev_eval,
+                bool device_driven, double* grow_hint);
+  void end_chunk(const Chunk& c, const Group& G, bool device_driven);
   void run_chunks(std::vector<hipEvent_t>& ev_eval);
+  void run_group_chunks(const Group& G, std::vector<hipEvent_t>& ev_eval);
   void build_graph();
   void finish_stats(const uint64_t* scal, double ms);
   void run_two_pass(std::vector<hipEvent_t>& ev_eval);
@@ -240,7 +248,7 @@ class Job {
   bool grow_pool(uint64_t need, uint64_t used);
   bool refresh_pool(uint64_t words);
   uint64_t pool_max_words() const;
-  void fetch_chunk_waves(const Chunk& c);
+  void fetch_chunk_waves(const Chunk& c, const Group& G);
   int device_;
   hipStream_t stream_;
   DevArena* arena_ = nullptr;
@@ -310,7 +318,10 @@ class Job {
   unsigned long long* d_wtot_ = nullptr;
   ChunkWin* d_cwin_ = nullptr;
   size_t cap_cwin_ = 0;
-  std::vector<uint64_t> sched_;  // F_1 .. F_K of the last run
+  std::vector<uint64_t> sched_;  // F_1 .. F_K of the last run, group after group
+  std::vector<Group> groups_;
+  std::vector<std::vector<uint64_t>> gsched_;  // per group: its F_1 .. F_K
+  size_t gsched_count_ = 0;                     // chunks of the group being run
   uint64_t max_tiles_ = 0, tiles_cap_ = 0, max_scratch_ = 0, scratch_cap_ = 0;
   uint64_t max_tails_ = 0, max_bump_ = 0;
   uint64_t pool_reallocs_ = 0;  // pool allocations made while running

@@ -367,8 +367,12 @@ void ProgramCache::clear() {
 // Union of several instances' images: nets of instance i are shifted by
 // net_base[i], gates are merged level by level, each gate records its
 // instance; truth and delay tables are shared per distinct image.
+// Gates are ordered group by group (instance groups [groups[q], groups[q+1])),
+// each group level by level and each level instance by instance; the
+// level table is flattened: group q's level l is level q * nlev + l.
 static std::shared_ptr<DevProgram> union_image(const std::vector<std::shared_ptr<DevProgram>>& parts,
-                                               const std::vector<uint32_t>& net_base) {
+                                               const std::vector<uint32_t>& net_base,
+                                               const std::vector<uint32_t>& groups) {
   auto U = std::make_shared<DevProgram>();
   std::map<const DevProgram*, std::pair<uint32_t, uint32_t>> bases;  // dtab, lut
   for (auto& p : parts) {
@@ -384,8 +388,9 @@ static std::shared_ptr<DevProgram> union_image(const std::vector<std::shared_ptr
   size_t nlev = 0;
   for (auto& p : parts) nlev = std::max(nlev, p->level_start.size() - 1);
   U->level_start.assign(1, 0);
+  for (size_t q = 0; q + 1 < groups.size(); ++q)
   for (size_t l = 0; l < nlev; ++l) {
-    for (size_t i = 0; i < parts.size(); ++i) {
+    for (size_t i = groups[q]; i < groups[q + 1]; ++i) {
       const DevProgram& p = *parts[i];
       if (l + 1 >= p.level_start.size()) continue;
       const auto b = bases[&p];
@@ -741,7 +746,32 @@ void Job::prepare(const std::vector<SimInstance>& inst, uint64_t pool_words) {
     win_.push_back(in.start);
     win_.push_back(in.end);
   }
-  prog_ = parts.size() == 1 ? parts[0] : union_image(parts, net_base_);
+  // Instance groups of a batch: consecutive instances whose estimated
+  // waveforms (stimulus, and each gate about ten times its inputs' mean
+  // events) fill at most half the HBM that can hold the event pool.  Each
+  // group runs its own sequence of time chunks (usually one), so per-gate
+  // work is not repeated once per time chunk of the whole batch.
+  std::vector<uint32_t> gsplit{0};
+  if (mode == 0 && inst.size() > 1) {
+    size_t free_b = 0, total_b = 0;
+    HC(hipMemGetInfo(&free_b, &total_b));
+    const double target =
+        pool_words ? 0.5 * (double)pool_words : 0.4 * (double)free_b / 4.0;
+    double acc = 0;
+    for (size_t i = 0; i < inst.size(); ++i) {
+      double stim = 0;
+      for (const Wave& w : inst[i].st->waves) stim += (double)w.size();
+      const double per_input = stim / (double)std::max<size_t>(1, inst[i].st->waves.size()) + 1;
+      const double est = stim + 10.0 * (double)parts[i]->n_gates * per_input;
+      if (i > gsplit.back() && acc + est > target) {
+        gsplit.push_back((uint32_t)i);
+        acc = 0;
+      }
+      acc += est;
+    }
+  }
+  gsplit.push_back((uint32_t)inst.size());
+  prog_ = parts.size() == 1 ? parts[0] : union_image(parts, net_base_, gsplit);
   if (mode == 1 && prog_->max_k > kTileK)
     throw GlsError(GLS_E_UNSUPPORTED, "the two-pass path takes gates of at most 4 distinct inputs");
   prog_->upload(device_, stream_);
@@ -792,9 +822,14 @@ void Job::prepare(const std::vector<SimInstance>& inst, uint64_t pool_words) {
   // constants: one entry (0, value) each; the four possible entries live here
   static const uint32_t kT0[1] = {0};
   static const uint8_t kV[4] = {0, 1, 2, 3};
+  std::vector<uint32_t> sig_start(inst.size() + 1, 0);
+  std::vector<uint64_t> inst_t_last(inst.size(), 0);
   for (size_t i = 0; i < inst.size(); ++i) {
     const SimInstance& in = inst[i];
     const uint32_t nb = net_base_[i];
+    sig_start[i] = (uint32_t)h_snet_.size();
+    for (const Wave& w : in.st->waves)
+      if (w.size()) inst_t_last[i] = std::max<uint64_t>(inst_t_last[i], w.t[w.size() - 1]);
     for (size_t s = 0; s < in.st->waves.size(); ++s) {
       const Wave& w = in.st->waves[s];
       if (mode == 1)
@@ -809,8 +844,26 @@ void Job::prepare(const std::vector<SimInstance>& inst, uint64_t pool_words) {
     }
   }
   n_sig_ = (uint32_t)h_snet_.size();
-  size_t nlev = level_start_.size() - 1;
+  sig_start[inst.size()] = n_sig_;
+  const size_t ngroups = gsplit.size() - 1;
+  const size_t nlev = (level_start_.size() - 1) / ngroups;
   horizon_ = t_last_ + (uint64_t)(nlev + 1) * wmax_ + 2;
+  groups_.clear();
+  for (size_t q = 0; q < ngroups; ++q) {
+    Group G;
+    G.i0 = gsplit[q];
+    G.i1 = gsplit[q + 1];
+    G.L0 = (uint32_t)(q * nlev);
+    G.L1 = (uint32_t)((q + 1) * nlev);
+    G.g0 = level_start_[G.L0];
+    G.g1 = level_start_[G.L1];
+    G.s0 = mode == 0 ? sig_start[G.i0] : 0;
+    G.s1 = mode == 0 ? sig_start[G.i1] : n_sig_;
+    uint64_t tl = 0;
+    for (uint32_t i = G.i0; i < G.i1; ++i) tl = std::max(tl, inst_t_last[i]);
+    G.horizon = tl + (uint64_t)(nlev + 1) * wmax_ + 2;
+    groups_.push_back(G);
+  }
 
   d_woff_ = (uint64_t*)dmalloc(n_nets_ * sizeof(uint64_t));
   d_wlen_ = (uint32_t*)dmalloc(n_nets_ * sizeof(uint32_t));
@@ -982,14 +1035,14 @@ void Job::put_chunk_windows(const Chunk& c, bool capture) {
 // rewritten).  A plain chunk that cannot fit the pool (or, with automatic
 // chunks, is projected not to) returns CHUNK_SHORTER before its results
 // are used; *grow_hint then suggests a length factor (0: none).
-int Job::run_chunk(const Chunk& c, std::vector<hipEvent_t>& ev_eval, bool device_driven,
-                   double* grow_hint) {
+int Job::run_chunk(const Chunk& c, const Group& G, std::vector<hipEvent_t>& ev_eval,
+                   bool device_driven, double* grow_hint) {
   uint64_t* bump = d_scal_ + S_BUMP;
   unsigned long long* evc = (unsigned long long*)(d_scal_ + S_EVC);
   unsigned int* err = (unsigned int*)(d_scal_ + S_ERR);
   const uint32_t seg = seg_target ? seg_target : kDefaultSeg;
   const bool retry = !device_driven && !static_ && (ovf_words_opt == 0 || pool_grow);
-  const size_t nlev = level_start_.size() - 1;
+  const uint32_t ns = G.s1 - G.s0;  // the group's stimulus signals
   const ChunkWin* cw = d_cwin_ + c.slot * net_base_.size();
   TailArgs T;
   T.tail_len = d_tail_len_;
@@ -1001,9 +1054,9 @@ int Job::run_chunk(const Chunk& c, std::vector<hipEvent_t>& ev_eval, bool device
 
   if (d_evjob_chunk_) HC(launch_fill_u64(d_evjob_chunk_, net_base_.size(), 0ull, stream_));
   // the chunk's stimulus at the start of the pool
-  HC(launch_stim_count(d_store_, d_soff_, d_slen_, n_sig_, c.lb, c.Fn == kChunkEnd ? ~0ull : c.Fn,
-                       d_scnt_, d_sfirst_, stream_));
-  if (n_sig_) HC(launch_scan(d_scnt_, d_sbase_, n_sig_, d_spart_, bump, nullptr, stream_));
+  HC(launch_stim_count(d_store_, d_soff_ + G.s0, d_slen_ + G.s0, ns, c.lb,
+                       c.Fn == kChunkEnd ? ~0ull : c.Fn, d_scnt_, d_sfirst_, stream_));
+  if (ns) HC(launch_scan(d_scnt_, d_sbase_, ns, d_spart_, bump, nullptr, stream_));
   else HC(launch_fill_u64((unsigned long long*)bump, 1, 0ull, stream_));
   uint64_t stim_chunk = 0;
   if (!device_driven) {
@@ -1015,8 +1068,8 @@ int Job::run_chunk(const Chunk& c, std::vector<hipEvent_t>& ev_eval, bool device
       if (stim_chunk + 1024 > pool_words_) return CHUNK_SHORTER;
     }
   }
-  HC(launch_stim_copy(d_store_, d_soff_, d_snet_, n_sig_, c.lb, d_scnt_, d_sfirst_, d_sbase_, d_pool_,
-                      pool_words_, d_woff_, d_wlen_, err, stream_));
+  HC(launch_stim_copy(d_store_, d_soff_ + G.s0, d_snet_ + G.s0, ns, c.lb, d_scnt_, d_sfirst_,
+                      d_sbase_, d_pool_, pool_words_, d_woff_, d_wlen_, err, stream_));
 
   size_t prev = SIZE_MAX;  // last level run
   uint64_t prev_bump = 0, prev_evc = 0, prev_err = 0;
@@ -1040,11 +1093,12 @@ int Job::run_chunk(const Chunk& c, std::vector<hipEvent_t>& ev_eval, bool device
     return true;
   };
   const bool projected = !device_driven && chunk_ps == 0;
-  size_t next = 0;  // level to run next
-  while (next <= nlev) {
+  const uint64_t group_gates = G.g1 - G.g0;
+  size_t next = G.L0;  // level to run next
+  while (next <= G.L1) {
     const size_t l = next++;
     if (!device_driven) stop_point();
-    if (l == nlev) {  // after the last level: did it overflow?
+    if (l == G.L1) {  // after the last level: did it overflow?
       if (!retry || prev == SIZE_MAX) break;
       uint64_t h[8];
       HC(hipMemcpyAsync(h, d_scal_, sizeof h, hipMemcpyDeviceToHost, stream_));
@@ -1085,8 +1139,9 @@ int Job::run_chunk(const Chunk& c, std::vector<hipEvent_t>& ev_eval, bool device
       if (ntiles >= 0xFFFFFFF0ull) throw GlsError(GLS_E_RANGE, "too many tiles in one level");
       // automatic chunks: a chunk projected (by its gates so far) not to fit
       // the largest pool is cut short before it runs further
-      if (projected && g0 >= std::max<uint32_t>(64, n_gates_ / 32) && used > stim_chunk) {
-        const double proj = (double)(used - stim_chunk) * n_gates_ / g0;
+      const uint64_t done_g = g0 - G.g0;
+      if (projected && done_g >= std::max<uint64_t>(64, group_gates / 32) && used > stim_chunk) {
+        const double proj = (double)(used - stim_chunk) * group_gates / done_g;
         const double cap = (double)(pool_auto_ && !static_ ? pool_max_words() : pool_words_);
         if (((double)stim_chunk + proj) * 1.35 > cap) {
           *grow_hint = std::max(0.05, 0.85 * (cap / 1.35 - stim_chunk) / proj);
@@ -1198,7 +1253,7 @@ int Job::run_chunk(const Chunk& c, std::vector<hipEvent_t>& ev_eval, bool device
 
 // Gate-output waveforms of a finished chunk to the host: each gate net's
 // entries after its tail, in absolute time.
-void Job::fetch_chunk_waves(const Chunk& c) {
+void Job::fetch_chunk_waves(const Chunk& c, const Group& G) {
   std::vector<uint64_t> off(n_nets_);
   std::vector<uint32_t> len(n_nets_), tl(n_nets_);
   uint64_t used = 0;
@@ -1211,8 +1266,8 @@ void Job::fetch_chunk_waves(const Chunk& c) {
   if (used) HC(hipMemcpyAsync(pool.data(), d_pool_, used * 4, hipMemcpyDeviceToHost, stream_));
   HC(hipStreamSynchronize(stream_));
   if (waves_.size() != n_nets_) waves_.assign(n_nets_, {});
-  for (const GateRec& g : prog_->h_grec) {
-    const uint32_t n = g.out_net;
+  for (uint32_t gi = G.g0; gi < G.g1; ++gi) {
+    const uint32_t n = prog_->h_grec[gi].out_net;
     if (off[n] + len[n] > used) throw GlsError(GLS_E_STATE, "waveform outside the event pool");
     for (uint32_t q = tl[n]; q < len[n]; ++q) {
       const uint32_t w = pool[off[n] + q];
@@ -1225,16 +1280,16 @@ void Job::fetch_chunk_waves(const Chunk& c) {
 // After a chunk: its SAIF shares join the totals, its waveforms go to the
 // host (keep_waves, plain runs), and the gate outputs' tails for the next
 // chunk are cut from their waveforms.
-void Job::end_chunk(const Chunk& c, bool device_driven) {
-  HC(launch_saif_accum(prog_->d_grec, n_gates_, d_saif_chunk_, d_saif_, d_wlen_, d_tail_len_, d_wtot_,
-                       stream_));
+void Job::end_chunk(const Chunk& c, const Group& G, bool device_driven) {
+  const GateRec* gates = prog_->d_grec + G.g0;
+  const uint32_t ng = G.g1 - G.g0;
+  HC(launch_saif_accum(gates, ng, d_saif_chunk_, d_saif_, d_wlen_, d_tail_len_, d_wtot_, stream_));
   if (d_evjob_) HC(launch_add_u64(d_evjob_, d_evjob_chunk_, net_base_.size(), stream_));
-  if (!device_driven && keep_waves) fetch_chunk_waves(c);
-  if (c.Fn == kChunkEnd || n_gates_ == 0) return;
+  if (!device_driven && keep_waves) fetch_chunk_waves(c, G);
+  if (c.Fn == kChunkEnd || ng == 0) return;
   const uint32_t key = (uint32_t)((c.Fn - wmax_) - c.lb + 1);  // lb of the next chunk
-  HC(launch_tail_len(prog_->d_grec, n_gates_, d_woff_, d_wlen_, d_pool_, key, d_tlen_g_, d_tsrc_g_,
-                     stream_));
-  HC(launch_scan(d_tlen_g_, d_toff_g_, n_gates_, d_gpart_, d_scal_ + S_TAILS, nullptr, stream_));
+  HC(launch_tail_len(gates, ng, d_woff_, d_wlen_, d_pool_, key, d_tlen_g_, d_tsrc_g_, stream_));
+  HC(launch_scan(d_tlen_g_, d_toff_g_, ng, d_gpart_, d_scal_ + S_TAILS, nullptr, stream_));
   if (!device_driven) {
     uint64_t total = 0;
     HC(hipMemcpyAsync(&total, d_scal_ + S_TAILS, 8, hipMemcpyDeviceToHost, stream_));
@@ -1242,9 +1297,8 @@ void Job::end_chunk(const Chunk& c, bool device_driven) {
     ensure((void**)&d_tails_, cap_tails_, std::max<uint64_t>(total, 1) * 4);
     max_tails_ = std::max(max_tails_, total);
   }
-  HC(launch_tail_copy(prog_->d_grec, n_gates_, d_tlen_g_, d_tsrc_g_, d_toff_g_, d_pool_, d_tails_,
-                      cap_tails_ / 4, d_tail_len_, d_tail_off_, (unsigned int*)(d_scal_ + S_ERR),
-                      stream_));
+  HC(launch_tail_copy(gates, ng, d_tlen_g_, d_tsrc_g_, d_toff_g_, d_pool_, d_tails_, cap_tails_ / 4,
+                      d_tail_len_, d_tail_off_, (unsigned int*)(d_scal_ + S_ERR), stream_));
 }
 
 // A plain run of every chunk.  Automatic chunks: the first chunk is the
@@ -1258,26 +1312,41 @@ void Job::run_chunks(std::vector<hipEvent_t>& ev_eval) {
   HC(launch_fill_u64(d_wtot_, n_nets_, 0ull, stream_));
   if (d_evjob_) HC(launch_fill_u64(d_evjob_, net_base_.size(), 0ull, stream_));
   HC(launch_saif_stim(d_store_, d_soff_, d_slen_, d_snet_, d_sjob_, n_sig_, d_saif_, d_win_, stream_));
-  ensure((void**)&d_tails_, cap_tails_, std::max<size_t>(n_gates_, 1) * 4);
-  HC(launch_tails_init(prog_->d_grec, n_gates_, d_tail_len_, d_tail_off_, d_tails_, stream_));
   waves_.clear();
   if (keep_waves) waves_.assign(n_nets_, {});
   sched_.clear();
+  gsched_.assign(groups_.size(), {});
   stats_.chunk_retries = 0;
   max_bump_ = 0;
-  max_tails_ = n_gates_;
+  max_tails_ = 0;
+  for (size_t q = 0; q < groups_.size(); ++q) {
+    run_group_chunks(groups_[q], ev_eval);
+    gsched_[q].assign(sched_.end() - (std::ptrdiff_t)gsched_count_, sched_.end());
+  }
+  stats_.chunks = sched_.size();
+}
+
+// The time chunks of one instance group, from its own time 0: the group's
+// gate outputs start from X tails.
+void Job::run_group_chunks(const Group& G, std::vector<hipEvent_t>& ev_eval) {
+  const uint32_t ng = G.g1 - G.g0;
+  ensure((void**)&d_tails_, cap_tails_, std::max<size_t>(ng, 1) * 4);
+  HC(launch_tails_init(prog_->d_grec + G.g0, ng, d_tail_len_, d_tail_off_, d_tails_, stream_));
+  max_tails_ = std::max<uint64_t>(max_tails_, ng);
+  gsched_count_ = 0;
+  const uint64_t horizon = G.horizon;
   const uint64_t lmax = kRelSpan - wmax_ - 8;  // relative times of a chunk stay in the event word
   const uint64_t lmin = wmax_ + 1;             // the look-back of chunk 1 starts after t = 0
   uint64_t L;
   if (chunk_ps == kNoChunks) {
-    if (horizon_ >= lmax) throw GlsError(GLS_E_RANGE, "times reach 2^30 - 2^18: the run needs time chunks");
-    L = horizon_;
+    if (horizon >= lmax) throw GlsError(GLS_E_RANGE, "times reach 2^30 - 2^18: the run needs time chunks");
+    L = horizon;
   } else if (chunk_ps) {
     if (chunk_ps < lmin)
       throw GlsError(GLS_E_ARG, "chunk length must exceed the largest gate delay");
     L = std::min<uint64_t>(chunk_ps, lmax);
   } else {
-    L = std::min<uint64_t>(horizon_, lmax);
+    L = std::min<uint64_t>(horizon, lmax);
   }
   // a chunk that did not fit caps every later chunk at the length it was
   // run again with: chunks after an empty stretch cannot grow back into a
@@ -1286,7 +1355,7 @@ void Job::run_chunks(std::vector<hipEvent_t>& ev_eval) {
   uint64_t F = 0, prev_lb = 0;
   size_t k = 0;
   for (;;) {
-    const uint64_t Fn = (F + L >= horizon_) ? kChunkEnd : F + L;
+    const uint64_t Fn = (F + L >= horizon) ? kChunkEnd : F + L;
     const Chunk c = make_chunk(k, F, Fn, prev_lb);
     uint64_t snap[16];
     HC(hipMemcpyAsync(snap, d_scal_, sizeof snap, hipMemcpyDeviceToHost, stream_));
@@ -1295,7 +1364,7 @@ void Job::run_chunks(std::vector<hipEvent_t>& ev_eval) {
     double hint = 0;
     const auto tc0 = std::chrono::steady_clock::now();
     const uint64_t re0 = pool_reallocs_;
-    const int res = run_chunk(c, ev_eval, false, &hint);
+    const int res = run_chunk(c, G, ev_eval, false, &hint);
     if (trace_on()) {
       uint64_t b = 0;
       HC(hipMemcpyAsync(&b, d_scal_ + S_BUMP, 8, hipMemcpyDeviceToHost, stream_));
@@ -1323,8 +1392,9 @@ void Job::run_chunks(std::vector<hipEvent_t>& ev_eval) {
     HC(hipMemcpyAsync(&used, d_scal_ + S_BUMP, 8, hipMemcpyDeviceToHost, stream_));
     HC(hipStreamSynchronize(stream_));
     max_bump_ = std::max(max_bump_, used);
-    end_chunk(c, false);
+    end_chunk(c, G, false);
     sched_.push_back(Fn);
+    ++gsched_count_;
     if (Fn == kChunkEnd) break;
     if (chunk_ps == 0) {
       // next length: half of the largest pool at this chunk's output rate
@@ -1339,7 +1409,6 @@ void Job::run_chunks(std::vector<hipEvent_t>& ev_eval) {
     F = Fn;
     ++k;
   }
-  stats_.chunks = sched_.size();
 }
 
 void Job::run_two_pass(std::vector<hipEvent_t>& ev_eval) {
@@ -1418,7 +1487,7 @@ void Job::run() {
     finish_stats(h, ms);
     if (keep_waves) {  // one chunk (checked at capture): the pool holds every waveform
       waves_.assign(n_nets_, {});
-      fetch_chunk_waves(make_chunk(0, 0, kChunkEnd, 0));
+      fetch_chunk_waves(make_chunk(0, 0, kChunkEnd, 0), groups_[0]);
     }
     stats_.wall_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - w0).count();
     return;
@@ -1620,18 +1689,21 @@ void Job::build_graph() {
   // stays clear of every level's outputs when it fits above the largest bump
   graph_scratch_in_pool_ = max_bump_ <= pool_words_ && scratch_cap_ <= pool_words_ - max_bump_;
   if (!graph_scratch_in_pool_) ensure((void**)&d_scratch_, cap_scratch_, scratch_cap_ * 4);
-  // every chunk's windows in a slot of its own
+  // every chunk's windows in a slot of its own (the groups' chunks in turn)
   const size_t K = sched_.size();
   ensure((void**)&d_cwin_, cap_cwin_, std::max<size_t>(K, 1) * net_base_.size() * sizeof(ChunkWin));
-  std::vector<Chunk> chunks;
-  uint64_t F = 0, prev_lb = 0;
-  for (size_t k = 0; k < K; ++k) {
-    Chunk c = make_chunk(k, F, sched_[k], prev_lb);
-    c.slot = k;
-    put_chunk_windows(c, true);
-    chunks.push_back(c);
-    prev_lb = c.lb;
-    F = sched_[k];
+  std::vector<std::vector<Chunk>> chunks(groups_.size());
+  size_t slot = 0;
+  for (size_t q = 0; q < groups_.size(); ++q) {
+    uint64_t F = 0, prev_lb = 0;
+    for (size_t k = 0; k < gsched_[q].size(); ++k) {
+      Chunk c = make_chunk(k, F, gsched_[q][k], prev_lb);
+      c.slot = slot++;
+      put_chunk_windows(c, true);
+      chunks[q].push_back(c);
+      prev_lb = c.lb;
+      F = gsched_[q][k];
+    }
   }
   hipGraph_t g = nullptr;
   HC(hipStreamBeginCapture(stream_, hipStreamCaptureModeRelaxed));
@@ -1642,12 +1714,16 @@ void Job::build_graph() {
     if (d_evjob_) HC(launch_fill_u64(d_evjob_, net_base_.size(), 0ull, stream_));
     HC(launch_saif_stim(d_store_, d_soff_, d_slen_, d_snet_, d_sjob_, n_sig_, d_saif_, d_win_,
                         stream_));
-    HC(launch_tails_init(prog_->d_grec, n_gates_, d_tail_len_, d_tail_off_, d_tails_, stream_));
     std::vector<hipEvent_t> none;
-    for (const Chunk& c : chunks) {
-      double hint = 0;
-      run_chunk(c, none, true, &hint);
-      end_chunk(c, true);
+    for (size_t q = 0; q < groups_.size(); ++q) {
+      const Group& G = groups_[q];
+      HC(launch_tails_init(prog_->d_grec + G.g0, G.g1 - G.g0, d_tail_len_, d_tail_off_, d_tails_,
+                           stream_));
+      for (const Chunk& c : chunks[q]) {
+        double hint = 0;
+        run_chunk(c, G, none, true, &hint);
+        end_chunk(c, G, true);
+      }
     }
   } catch (...) {
     (void)hipStreamEndCapture(stream_, &g);

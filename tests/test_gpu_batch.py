@@ -75,6 +75,36 @@ def test_batch_matches_plain_runs(ctx):
     assert run.stats()["n_events"] == total
 
 
+def test_batch_in_instance_groups(ctx):
+    # a fixed 64 MiB pool splits the batch into instance groups (each runs
+    # its own time chunks from time 0): every instance still equals its
+    # plain run, eagerly (waveforms too) and replayed from a captured graph
+    insts = mixed_instances()
+    args = ([i[0] for i in insts], [i[1] for i in insts], [i[2] for i in insts],
+            [i[3] for i in insts])
+    plain = []
+    for d, s, on, off, _ in insts:
+        r = ctx.simulate(d, s, on, off, keep_waves=True)
+        plain.append((r.waves_hash(), r.saif_text(), r.stats()["n_events"]))
+        r.close()
+    b = ctx.prepare_batch(*args, keep_waves=True, pool_bytes=64 << 20).execute()
+    say("instance groups: chunks", b.stats()["chunks"])
+    assert b.stats()["chunks"] >= 3
+    for k in range(len(insts)):
+        assert b.instance_waves_hash(k) == plain[k][0], k
+        assert b.instance_saif_text(k) == plain[k][1], k
+        assert b.instance_events(k) == plain[k][2], k
+    b.close()
+    g = ctx.prepare_batch(*args, graph=True, pool_bytes=64 << 20)
+    for rep in range(3):
+        g.execute()
+        for k in range(len(insts)):
+            assert g.instance_saif_text(k) == plain[k][1], (rep, k)
+            assert g.instance_events(k) == plain[k][2], (rep, k)
+    assert g.stats()["eval_launches"] == 1
+    g.close()
+
+
 def test_batch_segment_and_ticket_variants(ctx):
     insts = mixed_instances()[:4]
     args = ([i[0] for i in insts], [i[1] for i in insts], [i[2] for i in insts],
