@@ -748,15 +748,15 @@ void Job::prepare(const std::vector<SimInstance>& inst, uint64_t pool_words) {
   }
   // Instance groups of a batch: consecutive instances whose estimated
   // waveforms (stimulus, and each gate about ten times its inputs' mean
-  // events) fill at most half the HBM that can hold the event pool.  Each
-  // group runs its own sequence of time chunks (usually one), so per-gate
-  // work is not repeated once per time chunk of the whole batch.
+  // events: on the generator's netlists about three times what they make)
+  // fit the HBM that can hold the event pool.  Each group runs its own
+  // sequence of time chunks (usually one), so per-gate work is not repeated
+  // once per time chunk of the whole batch.
   std::vector<uint32_t> gsplit{0};
   if (mode == 0 && inst.size() > 1) {
     size_t free_b = 0, total_b = 0;
     HC(hipMemGetInfo(&free_b, &total_b));
-    const double target =
-        pool_words ? 0.5 * (double)pool_words : 0.4 * (double)free_b / 4.0;
+    const double target = pool_words ? (double)pool_words : 0.8 * (double)free_b / 4.0;
     double acc = 0;
     for (size_t i = 0; i < inst.size(); ++i) {
       double stim = 0;
