@@ -5,15 +5,17 @@ whole window, stimulus already resident in HBM): the 2,000,000-cell
 configuration -- generic 12-cell library, 4,096 primary inputs, 60 levels
 with log-normal widths, Pareto(1.5) fanout capped at 256, SDF delays
 U{5..50} ps, Poisson stimulus with mean gap 200 ps over 0..20,000,000 ps,
-1% X toggles (seed 0x5EED0002).  Its waveforms (about 2 TB) do not fit one
-GPU: the run streams them through time chunks.  Synthetic data from the
+1% X toggles (seed 0x5EED0002).  Its waveforms (1.15e12 gate events, about
+4.6 TB) do not fit one GPU: the run streams them through time chunks.  Synthetic data from the
 seeded generator.
 
 Extras on the same JSON line, while the time budget lasts: the 200,000-cell
-configuration (plain and one captured graph), the batching configuration
-(256 stimulus sets on a 200k-cell netlist + 64 netlists of 5k-500k cells,
-batched vs plain with the same tuned kernel settings), and the serving mix
-(Poisson jobs of 10k-2M cells, 1-20 Mps windows, 30% netlist reuse).
+configuration (plain and one captured graph), the serving mix (Poisson jobs
+of 10k-2M cells, 1-20 Mps windows, 30% netlist reuse), the stimulus ingest
+of the 2M-cell configuration (its VCD read by the parallel reader, then the
+pinned upload), and the batching configuration (256 stimulus sets on a
+200k-cell netlist + 64 netlists of 5k-500k cells, batched vs plain with the
+same tuned kernel settings).
 
 The CPU baseline (the oracle, a serial restatement of the reference CPU
 simulator) simulates the 2,000-cell configuration in full on one pinned
@@ -209,7 +211,7 @@ def plain_steps(ctx, name, steps, warmup, args, graph_steps=0):
     return res
 
 
-def batched_vs_plain(ctx, n_sets, n_nets, scale):
+def batched_vs_plain(ctx, n_sets, n_nets, scale, plain_default=False):
     """Batching workload: one netlist from the 200k-cell generator at `scale`
     of its size driven by `n_sets` seeded stimulus sets (0-2 Mps windows,
     mean gap U[100, 5000] ps), plus `n_nets` netlists with log-uniform cell
@@ -219,7 +221,8 @@ def batched_vs_plain(ctx, n_sets, n_nets, scale):
     inside a group only as HBM requires), the whole group, chunk and level
     sequence captured as one hipGraph, kernel settings autotuned on the batch.
     Plain: every instance prepared and executed alone, one after the other,
-    with the same tuned settings (and, separately, the default settings).
+    with the same tuned settings (and, with plain_default, separately with
+    the default settings).
     Every instance's SAIF is compared with its plain run."""
     import gls
     rng = random.Random(0x5EED0003)
@@ -278,9 +281,11 @@ def batched_vs_plain(ctx, n_sets, n_nets, scale):
     default = {"seg_events": 32, "level_blocks": 0, "prefetch": True}
     if knobs == default or tune["level_blocks"] == 0:
         dflt_s = tuned_s
-    else:
+    elif plain_default:
         dflt_s, _, _ = plain({})
         log(f"c3: plain (default settings) {dflt_s * 1e3:.1f} ms")
+    else:
+        dflt_s = None
     return {"workload": f"{n_sets} stimulus sets on a {cells}-cell netlist + {n_nets} netlists "
                         f"(log-uniform {int(5000 * scale)}-{int(500000 * scale)} cells, depth 10-60), "
                         f"0-2 Mps windows, gap U[100,5000] ps",
@@ -288,9 +293,9 @@ def batched_vs_plain(ctx, n_sets, n_nets, scale):
             "batched_s": round(batch_s, 4), "batched_events_per_s": round(bst["n_events"] / batch_s, 1),
             "plain_tuned_s": round(tuned_s, 4),
             "plain_tuned_events_per_s": round(tuned_ev / tuned_s, 1),
-            "plain_default_s": round(dflt_s, 4),
+            "plain_default_s": round(dflt_s, 4) if dflt_s else None,
             "speedup": round(tuned_s / batch_s, 3),
-            "speedup_vs_default_plain": round(dflt_s / batch_s, 3),
+            "speedup_vs_default_plain": round(dflt_s / batch_s, 3) if dflt_s else None,
             "saif_bit_exact_vs_plain": batch_saif == tuned_saif,
             "events_same_as_plain": bst["n_events"] == tuned_ev,
             "form": "one captured hipGraph of every instance group's chunks and levels"
@@ -330,13 +335,16 @@ def serving_mix(ctx, n_jobs, scale, max_batch=4):
     solo = gls.Server(ctx, max_running=1)
     for k in range(n_jobs):
         d, s, on, off = jobs[k]
-        j = solo.submit(d, s, on, off)
-        solo.wait(j)
-        alone[k] = solo.saif_text(j)
         t0 = time.perf_counter()
         j = solo.submit(d, s, on, off)
         solo.wait(j)
-        t_alone.append(time.perf_counter() - t0)
+        alone[k] = solo.saif_text(j)
+        t1 = time.perf_counter()
+        j = solo.submit(d, s, on, off)
+        solo.wait(j)
+        t_alone.append(time.perf_counter() - t1)
+        log(f"c4: alone {k}: {d.info()['n_instances']} cells, 0-{off - 1} ps: first {1e3 * (t1 - t0):.0f} ms, "
+            f"resident {1e3 * t_alone[-1]:.0f} ms")
     solo.close()
     mean_s = sum(t_alone) / len(t_alone)
     rate = 0.75 / mean_s
@@ -346,7 +354,10 @@ def serving_mix(ctx, n_jobs, scale, max_batch=4):
     for (d, s, on, off), g in zip(jobs, gaps):
         time.sleep(g)
         ids.append(srv.submit(d, s, on, off))
-    ok = all(srv.wait(j) == 0 for j in ids)
+    ok = True
+    for k, j in enumerate(ids):
+        ok = srv.wait(j) == 0 and ok
+        log(f"c4: mix job {k} done")
     same = all(srv.saif_text(ids[k]) == alone[k] for k in alone)
     st = srv.stats()
     srv.close()
@@ -360,6 +371,46 @@ def serving_mix(ctx, n_jobs, scale, max_batch=4):
             "peak_running": int(st["peak_running"]), "resident_programs": int(st["resident_programs"]),
             "device_buffers_reused": int(st["arena_hits"]), "device_buffers_allocated": int(st["arena_misses"]),
             "saif_same_alone_and_in_mix": same, "jobs_checked_alone": len(alone)}
+
+
+def vcd_ingest(ctx, tmp):
+    """Stimulus ingest of the 2M-cell configuration: its VCD written to a
+    file (4,096 inputs, 0-20 Mps, gap 200 ps), read back by the parallel
+    reader (line rate), and uploaded through the pinned staging buffers
+    (the prepare of a netlist already resident).  Falls back to the
+    200k-cell stimulus when the scratch disk is short of room."""
+    import shutil
+    import gls
+    name = "c2"
+    if shutil.disk_usage(tmp).free < (12 << 30):
+        name = "c1"
+    W = WORKLOADS[name]
+    d = gls.Design.generate(W["config"], W["seed"], W["cells"], W["levels"], W["pis"])
+    s = gls.Stimulus.generate(d, W["seed"], W["t_end"], W["gap"], 0.01)
+    path = os.path.join(tmp, "stim.vcd")
+    t0 = time.perf_counter()
+    s.write_vcd(path)
+    t_write = time.perf_counter() - t0
+    size = os.path.getsize(path)
+    s.close()
+    t0 = time.perf_counter()
+    s2 = gls.Stimulus.from_vcd(path)
+    t_parse = time.perf_counter() - t0
+    os.remove(path)
+    r = ctx.prepare(d, s2, 0, W["t_end"] + 1)  # program image built and made resident
+    r.close()
+    t0 = time.perf_counter()
+    r = ctx.prepare(d, s2, 0, W["t_end"] + 1)
+    t_prep = time.perf_counter() - t0
+    r.close()
+    n = s2.info()["n_events"]
+    log(f"ingest: {size / 1e9:.2f} GB VCD read in {t_parse:.2f} s, prepare {t_prep:.2f} s")
+    return {"workload": f"{W['cells']}-cell configuration's stimulus, {W['pis']} inputs, "
+                        f"0-{W['t_end']} ps, gap {W['gap']} ps",
+            "vcd_bytes": size, "stimulus_events": n, "write_s": round(t_write, 3),
+            "parse_s": round(t_parse, 3), "parse_gb_per_s": round(size / t_parse / 1e9, 3),
+            "prepare_s": round(t_prep, 3),
+            "upload_gb_per_s": round(8 * n / t_prep / 1e9, 3)}
 
 
 def main():
@@ -376,11 +427,14 @@ def main():
     ap.add_argument("--no-prefetch", action="store_true")
     ap.add_argument("--hbm-stacks", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--extras", default="c1,c3,c4",
-                    help="comma list of extra measurements after the headline (c1, c3, c4; '' none)")
-    ap.add_argument("--budget-s", type=float, default=540.0,
+    ap.add_argument("--extras", default="c1,c4,ingest,c3",
+                    help="comma list of extra measurements after the headline (c1, c3, c4, "
+                         "ingest; '' none)")
+    ap.add_argument("--budget-s", type=float, default=600.0,
                     help="extras start only while the elapsed time leaves room for them")
     ap.add_argument("--c3-scale", type=float, default=1.0)
+    ap.add_argument("--c3-plain-default", action="store_true",
+                    help="batching mix: also time the plain runs with the default kernel settings")
     ap.add_argument("--c4-jobs", type=int, default=24)
     ap.add_argument("--c4-scale", type=float, default=1.0)
     ap.add_argument("--c4-batch", type=int, default=4,
@@ -453,7 +507,8 @@ def main():
         out["pipeline_counter_over_alg"] = tr["pipeline_over_alg"]
     extras = [x for x in args.extras.split(",") if x]
     # rough costs of each extra (s), to stay inside the budget
-    cost = {"c1": 25.0, "c3": 150.0 * args.c3_scale + 20, "c4": 5.0 * args.c4_jobs * args.c4_scale + 20}
+    cost = {"c1": 35.0, "c3": 280.0 * args.c3_scale + 20, "c4": 2.0 * args.c4_jobs * args.c4_scale + 10,
+            "ingest": 60.0}
     skipped = []
     for x in extras:
         if time.time() - T0 + cost.get(x, 0) > args.budget_s:
@@ -466,9 +521,12 @@ def main():
                                                                         "t_end": None}), graph_steps=1)
         elif x == "c3":
             out["batched"] = batched_vs_plain(ctx, max(1, int(256 * min(1.0, args.c3_scale))),
-                                              max(1, int(64 * min(1.0, args.c3_scale))), args.c3_scale)
+                                              max(1, int(64 * min(1.0, args.c3_scale))), args.c3_scale,
+                                              args.c3_plain_default)
         elif x == "c4":
             out["serving"] = serving_mix(ctx, args.c4_jobs, args.c4_scale, args.c4_batch)
+        elif x == "ingest":
+            out["ingest"] = vcd_ingest(ctx, tmp)
         log(f"{x}: done in {time.time() - t0:.1f}s")
     if skipped:
         out["extras_skipped_for_time"] = skipped

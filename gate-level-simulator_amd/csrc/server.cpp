@@ -39,23 +39,38 @@ Server::~Server() {
   for (auto& t : workers_) t.join();
 }
 
-// Device bytes a job needs: its event pool (stimulus plus an estimate of the
-// per-level output reservations: each gate reserves its input event count),
-// per-net arrays, level scratch, overflow stacks, and the program image when
-// it is not resident yet.
-uint64_t Server::estimate(const Rec& r, bool* resident) const {
-  const RunInst& in = r.inst;
+// Event-pool words of a job: its stimulus plus an estimate of the level
+// outputs (internal nets toggle a few times more often than the inputs, and
+// a gate reserves about fan-in times its inputs' events: ten times the
+// inputs' mean events per gate), at most cap_words.  A job whose waveforms
+// need more runs in time chunks that fit its pool.
+static uint64_t job_pool_words(const RunInst& in, double pool_factor, uint64_t cap_words) {
   uint64_t stim = 0;
   for (auto& w : in.st->waves) stim += w.size();
   const uint64_t n_in = std::max<uint64_t>(1, in.st->waves.size());
+  const double est = (double)stim + pool_factor * 10.0 * (double)in.pg->gates.size() *
+                                        (double)(stim / n_in + 1) + (double)(1u << 20);
+  return std::max<uint64_t>(1u << 22, (uint64_t)std::min(est, (double)cap_words));
+}
+
+// Largest event pool of one job: 60% of the budget shared by the jobs that
+// may run at once (the rest is per-net arrays, level scratch and stacks).
+uint64_t Server::pool_cap_words() const {
+  const double share = 0.6 * (double)opt_.device_budget / std::max<uint32_t>(1, opt_.max_running);
+  return std::max<uint64_t>(1ull << 24, (uint64_t)(share / 4));
+}
+
+// Device bytes a job needs: its event pool, its resident stimulus (8 bytes
+// an event), per-net arrays, level scratch, overflow stacks, and the program
+// image when it is not resident yet.
+uint64_t Server::estimate(const Rec& r, bool* resident) const {
+  const RunInst& in = r.inst;
   const uint64_t gates = in.pg->gates.size(), nets = in.names.size();
-  const uint64_t per_input = stim / n_in + 1;
-  // internal nets toggle a few times more often than the inputs (glitches);
-  // a gate reserves about fan-in times its inputs' events
-  uint64_t pool = stim + (uint64_t)(opt_.pool_factor * 10.0 * (double)gates * (double)per_input) +
-                  (1u << 20);
+  uint64_t stim = 0;
+  for (auto& w : in.st->waves) stim += w.size();
+  const uint64_t pool = job_pool_words(in, opt_.pool_factor, pool_cap_words());
   const uint64_t ovf = std::min<uint64_t>(1ull << 24, std::max<uint64_t>(1ull << 18, pool / 16));
-  uint64_t bytes = pool * 4 + ovf * 4 + nets * 48 + gates * 64 + (64ull << 20);
+  uint64_t bytes = pool * 4 + stim * 8 + ovf * 4 + nets * 48 + gates * 64 + (64ull << 20);
   *resident = cache_->resident(in.pg.get(), (uint32_t)in.timescale_ps);
   if (!*resident) bytes += gates * (sizeof(GateRec) + sizeof(GateDev) + 512);
   return bytes;
@@ -192,16 +207,6 @@ void Server::worker(int slot) {
   (void)hipStreamDestroy(stream);
 }
 
-// Event-pool words of a job: its stimulus plus an estimate of the level
-// outputs (each gate reserves about ten times its inputs' mean events).
-static uint64_t job_pool_words(const RunInst& in, double pool_factor) {
-  uint64_t stim = 0;
-  for (auto& w : in.st->waves) stim += w.size();
-  const uint64_t n_in = std::max<uint64_t>(1, in.st->waves.size());
-  return stim + (uint64_t)(pool_factor * 10.0 * (double)in.pg->gates.size() * (double)(stim / n_in + 1)) +
-         (1u << 20);
-}
-
 void Server::run_one(Rec* r, hipStream_t stream, int& status, uint64_t& events, uint64_t& bytes,
                      uint32_t& attempts, std::vector<uint64_t>& saif) {
   status = 0;
@@ -214,7 +219,7 @@ void Server::run_one(Rec* r, hipStream_t stream, int& status, uint64_t& events, 
     I.stim_nets = in.stim_nets;
     I.start = in.start;
     I.end = in.end;
-    uint64_t pool = job_pool_words(in, opt_.pool_factor);
+    uint64_t pool = job_pool_words(in, opt_.pool_factor, pool_cap_words());
     for (uint32_t attempt = 0; attempt < 2; ++attempt) {
       ++attempts;
       try {
@@ -263,7 +268,7 @@ bool Server::run_group(const std::vector<Rec*>& group, hipStream_t stream) {
       I[i].stim_nets = in.stim_nets;
       I[i].start = in.start;
       I[i].end = in.end;
-      pool += job_pool_words(in, opt_.pool_factor);
+      pool += job_pool_words(in, opt_.pool_factor, pool_cap_words());
     }
     Job job(device_, stream, arena_.get());
     job.seg_target = opt_.seg_events;

@@ -113,6 +113,7 @@ class Server {
   void finish(Rec* r, int status, uint64_t events, uint64_t bytes, uint32_t attempts,
               std::vector<uint64_t> saif);
   uint64_t estimate(const Rec& r, bool* resident) const;
+  uint64_t pool_cap_words() const;
   int device_;
   ServerOptions opt_;
   ProgramCache* cache_;
