@@ -40,8 +40,12 @@ struct gls_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
   ProgramCache cache;  // resident program images of this device
+  // device buffers of finished runs, reused by later runs and servers of
+  // this context (large device allocations and frees can take seconds)
+  std::shared_ptr<DevArena> arena;
 };
 struct gls_run {
+  std::shared_ptr<DevArena> arena;  // outlives the job, which returns its buffers to it
   std::unique_ptr<Job> job;
   std::vector<RunInst> inst;
   bool keep_waves = false;
@@ -357,7 +361,7 @@ int gls_ctx_mem_info(gls_ctx* c, uint64_t* free_bytes, uint64_t* total_bytes) {
   hip_check(hipSetDevice(c->device), "hipSetDevice");
   size_t f = 0, t = 0;
   hip_check(hipMemGetInfo(&f, &t), "hipMemGetInfo");
-  *free_bytes = f;
+  *free_bytes = f + (c->arena ? c->arena->cached_bytes() : 0);  // cached blocks are reusable
   *total_bytes = t;
   API_END
 }
@@ -373,6 +377,9 @@ int gls_ctx_create(int device, gls_ctx** out) {
   c->device = device;
   hip_check(hipSetDevice(device), "hipSetDevice");
   hip_check(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking), "hipStreamCreate");
+  size_t free_b = 0, total_b = 0;
+  hip_check(hipMemGetInfo(&free_b, &total_b), "hipMemGetInfo");
+  c->arena = std::make_shared<DevArena>((uint64_t)total_b);
   *out = c.release();
   API_END
 }
@@ -429,7 +436,8 @@ int gls_prepare_batch(gls_ctx* c, size_t n, const gls_design* const* designs,
   }
   r->keep_waves = opts && opts->keep_waves;
   r->saif_tc = opts && opts->saif_tc;
-  r->job = std::make_unique<Job>(c->device, c->stream);
+  r->arena = c->arena;
+  r->job = std::make_unique<Job>(c->device, c->stream, c->arena.get());
   apply_opts(*r->job, opts);
   r->job->prepare(inst, opts ? opts->pool_bytes / 4 : 0);
   // sized once here: executions only copy into it
@@ -753,7 +761,7 @@ int gls_server_create(gls_ctx* c, const gls_server_opts* o, gls_server** out) {
     so.max_batch = o->max_batch ? o->max_batch : 1;
   }
   auto s = std::make_unique<gls_server>();
-  s->srv = std::make_unique<Server>(c->device, so, &c->cache);
+  s->srv = std::make_unique<Server>(c->device, so, &c->cache, c->arena);
   *out = s.release();
   API_END
 }

@@ -123,8 +123,14 @@ class DevArena {
   DevArena& operator=(const DevArena&) = delete;
   // A block of at least `bytes` bytes, or false when the device has no room.
   bool get(size_t bytes, void** p);
+  // For a buffer that uses whatever size it gets (an automatically sized
+  // event pool): the largest cached block when it holds at least min_bytes,
+  // else a new block of `want` bytes, or of less (down to min_bytes) when the
+  // device has no room; *got is the block's size.
+  bool get_large(size_t want, size_t min_bytes, void** p, size_t* got);
   void put(void* p);
   void trim();  // frees every cached block
+  uint64_t cached_bytes();
   uint64_t hits() const { return hits_; }
   uint64_t misses() const { return misses_; }
 

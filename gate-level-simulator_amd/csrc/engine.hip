@@ -475,6 +475,51 @@ bool DevArena::get(size_t bytes, void** p) {
   return false;
 }
 
+bool DevArena::get_large(size_t want, size_t min_bytes, void** p, size_t* got) {
+  min_bytes = size_class(std::max<size_t>(min_bytes, 1));
+  std::lock_guard<std::mutex> lk(mu_);
+  if (!free_.empty()) {
+    auto last = std::prev(free_.end());
+    if (last->first >= min_bytes) {
+      *p = last->second;
+      *got = last->first;
+      live_[*p] = last->first;
+      cached_ -= last->first;
+      free_.erase(last);
+      ++hits_;
+      return true;
+    }
+  }
+  ++misses_;
+  const size_t first = size_class(std::max(want, min_bytes));
+  for (int attempt = 0; attempt < 2; ++attempt) {
+    // 10% smaller after each refusal, down to min_bytes
+    for (size_t m = first;;) {
+      void* q = nullptr;
+      if (hipMalloc(&q, m) == hipSuccess) {
+        live_[q] = m;
+        *p = q;
+        *got = m;
+        return true;
+      }
+      (void)hipGetLastError();
+      const size_t next = size_class(m / 10 * 9);
+      if (m <= min_bytes || next >= m) break;
+      m = std::max(next, min_bytes);
+    }
+    if (free_.empty()) break;
+    trim_locked();  // the cached blocks make room
+  }
+  *p = nullptr;
+  *got = 0;
+  return false;
+}
+
+uint64_t DevArena::cached_bytes() {
+  std::lock_guard<std::mutex> lk(mu_);
+  return cached_;
+}
+
 void DevArena::put(void* p) {
   if (!p) return;
   std::lock_guard<std::mutex> lk(mu_);
@@ -510,6 +555,7 @@ Job::Job(int device, hipStream_t stream, DevArena* arena)
     : device_(device), stream_(stream), arena_(arena) {}
 
 Job::~Job() {
+  const auto t0 = std::chrono::steady_clock::now();
   if (gexec_) (void)hipGraphExecDestroy(gexec_);
   // blocks go back to the arena only once this job's work has drained
   if (arena_) (void)hipStreamSynchronize(stream_);
@@ -519,24 +565,42 @@ Job::~Job() {
                    d_pool_,   d_ovf_,  d_tails_, d_cwin_};
   for (void* p : grown)
     if (p) arena_ ? arena_->put(p) : (void)hipFree(p);
+  if (trace_on())
+    std::fprintf(stderr, "gls: job buffers released in %.1f ms\n",
+                 std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count());
 }
 
 bool Job::dev_alloc(void** p, size_t bytes) {
-  if (arena_) return arena_->get(bytes, p);
-  if (hipMalloc(p, bytes) == hipSuccess) return true;
-  (void)hipGetLastError();
-  *p = nullptr;
-  return false;
+  const auto t0 = std::chrono::steady_clock::now();
+  bool ok;
+  if (arena_) {
+    ok = arena_->get(bytes, p);
+  } else {
+    ok = hipMalloc(p, bytes) == hipSuccess;
+    if (!ok) {
+      (void)hipGetLastError();
+      *p = nullptr;
+    }
+  }
+  if (trace_on() && bytes >= (1ull << 30))
+    std::fprintf(stderr, "gls: device allocation of %.1f GiB %s in %.1f ms\n", bytes / 1073741824.0,
+                 ok ? "done" : "failed",
+                 std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count());
+  return ok;
 }
 
 void Job::dev_free(void* p) {
   if (!p) return;
+  const auto t0 = std::chrono::steady_clock::now();
   if (arena_) {
     HC(hipStreamSynchronize(stream_));
     arena_->put(p);
   } else {
     HC(hipFree(p));
   }
+  if (trace_on())
+    std::fprintf(stderr, "gls: device free in %.1f ms\n",
+                 std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count());
 }
 
 void* Job::dmalloc(size_t bytes) {
@@ -591,7 +655,8 @@ bool Job::grow_pool(uint64_t need, uint64_t used) {
   if (!pool_auto_ || static_) return false;
   size_t free_b = 0, total_b = 0;
   HC(hipMemGetInfo(&free_b, &total_b));
-  const uint64_t room = (uint64_t)(free_b * 0.9) / 4;  // the old pool stays during the copy
+  // the old pool stays during the copy; cached blocks make room when needed
+  const uint64_t room = (uint64_t)((free_b + (arena_ ? arena_->cached_bytes() : 0)) * 0.9) / 4;
   uint64_t n = std::max<uint64_t>(need, 2 * pool_words_);
   if (n > room) n = need;
   if (n > room) return false;
@@ -614,11 +679,13 @@ bool Job::grow_pool(uint64_t need, uint64_t used) {
 }
 
 // Largest pool an automatically sized pool may take between chunks (when
-// nothing in it needs keeping): 80% of the free HBM plus the current pool.
+// nothing in it needs keeping): 80% of the free HBM and the arena's cached
+// blocks, plus the current pool.
 uint64_t Job::pool_max_words() const {
   size_t free_b = 0, total_b = 0;
   hip_check(hipMemGetInfo(&free_b, &total_b), "hipMemGetInfo");
-  return (uint64_t)((free_b + pool_words_ * 4.0) * 0.8) / 4;
+  const double cached = arena_ ? (double)arena_->cached_bytes() : 0.0;
+  return (uint64_t)((free_b + cached + pool_words_ * 4.0) * 0.8) / 4;
 }
 
 // Between chunks (the pool holds nothing to keep): replaces an automatically
@@ -722,6 +789,7 @@ void Job::upload_store(const std::vector<SigSrc>& src, const std::vector<uint64_
 }
 
 void Job::prepare(const std::vector<SimInstance>& inst, uint64_t pool_words) {
+  const auto t_prep = std::chrono::steady_clock::now();
   HC(hipSetDevice(device_));
   if (inst.empty()) throw GlsError(GLS_E_ARG, "no instance to simulate");
   if (inst.size() > 65535) throw GlsError(GLS_E_ARG, "at most 65535 instances per batch");
@@ -756,7 +824,8 @@ void Job::prepare(const std::vector<SimInstance>& inst, uint64_t pool_words) {
   if (mode == 0 && inst.size() > 1) {
     size_t free_b = 0, total_b = 0;
     HC(hipMemGetInfo(&free_b, &total_b));
-    const double target = pool_words ? (double)pool_words : 0.8 * (double)free_b / 4.0;
+    const double cached = arena_ ? (double)arena_->cached_bytes() : 0.0;  // reusable too
+    const double target = pool_words ? (double)pool_words : 0.8 * ((double)free_b + cached) / 4.0;
     double acc = 0;
     for (size_t i = 0; i < inst.size(); ++i) {
       double stim = 0;
@@ -916,20 +985,48 @@ void Job::prepare(const std::vector<SimInstance>& inst, uint64_t pool_words) {
     stim_words_ = words.size();
   }
 
-  // event pool: a caller-given size is fixed unless pool_grow is set; size 0
-  // starts at four times the stimulus (at least 64 MiB); a growable pool is
-  // enlarged between levels and chunks as needed
+  // event pool: a caller-given size is fixed unless pool_grow is set; a
+  // growable pool is enlarged between levels and chunks as needed.  Size 0:
+  // sized from the waveforms expected (the stimulus plus ten times each
+  // instance's mean input events per gate, at least four times the
+  // stimulus and 64 MiB), at most 85% of the HBM free or cached; with an
+  // arena the largest cached block is taken whole when it holds the least
+  // size, so a run reuses the last run's pool.  (Large device allocations
+  // and frees can take seconds each: growing a pool step by step from a
+  // small start cost more than the simulation on large netlists.)
   pool_auto_ = pool_words == 0 || pool_grow;
+  bool pool_taken = false;
   if (pool_words == 0) {
     size_t free_b = 0, total_b = 0;
     HC(hipMemGetInfo(&free_b, &total_b));
-    pool_words = std::min<uint64_t>((uint64_t)(free_b * 0.85) / 4,
-                                    std::max<uint64_t>(1ull << 24, 4 * stim_words_));
+    const double room = 0.85 * ((double)free_b + (arena_ ? (double)arena_->cached_bytes() : 0.0)) / 4;
+    double est = 0;
+    for (const SimInstance& I : inst) {
+      uint64_t stim = 0;
+      for (auto& w : I.st->waves) stim += w.size();
+      const uint64_t n_in = std::max<uint64_t>(1, I.st->waves.size());
+      est += (double)stim + 10.0 * (double)I.pg->gates.size() * (double)(stim / n_in + 1);
+    }
+    uint64_t least = std::max<uint64_t>(1ull << 24, 4 * stim_words_);
+    if (mode == 1) least = std::max<uint64_t>(least, stim_words_ + 1024);
+    least = std::min<uint64_t>(least, (uint64_t)room);
+    const uint64_t want = std::max<uint64_t>(least, (uint64_t)std::min(est, room));
+    if (arena_) {
+      size_t got = 0;
+      if (!arena_->get_large((want + kPoolPad) * sizeof(uint32_t), (least + kPoolPad) * sizeof(uint32_t),
+                             (void**)&d_pool_, &got))
+        HC(hipErrorOutOfMemory);
+      pool_words = got / sizeof(uint32_t) - kPoolPad;
+      pool_taken = true;
+    } else {
+      pool_words = want;
+    }
   }
   if (mode == 1 && pool_words < stim_words_ + 1024) pool_words = stim_words_ + 1024;
   if (pool_words < 1024) pool_words = 1024;
   pool_words_ = pool_words;
-  if (!dev_alloc((void**)&d_pool_, (pool_words_ + kPoolPad) * sizeof(uint32_t))) HC(hipErrorOutOfMemory);
+  if (!pool_taken && !dev_alloc((void**)&d_pool_, (pool_words_ + kPoolPad) * sizeof(uint32_t)))
+    HC(hipErrorOutOfMemory);
   dev_bytes_ += (pool_words_ + kPoolPad) * sizeof(uint32_t);
 
   if (mode == 1) {
@@ -959,6 +1056,10 @@ void Job::prepare(const std::vector<SimInstance>& inst, uint64_t pool_words) {
   stats_.n_nets = n_nets_;
   stats_.n_gates = n_gates_;
   stats_.n_pins = prog_->n_pins;
+  if (trace_on())
+    std::fprintf(stderr, "gls: prepared %zu instance(s), pool %.1f GiB, in %.1f ms\n", inst.size(),
+                 pool_words_ * 4.0 / 1073741824.0,
+                 std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_prep).count());
 }
 
 void Job::stop_point() {

@@ -14,16 +14,22 @@ static double ms_between(Clock::time_point a, Clock::time_point b) {
   return std::chrono::duration<double, std::milli>(b - a).count();
 }
 
-Server::Server(int device, const ServerOptions& o, ProgramCache* cache)
-    : device_(device), opt_(o), cache_(cache) {
+Server::Server(int device, const ServerOptions& o, ProgramCache* cache,
+               std::shared_ptr<DevArena> arena)
+    : device_(device), opt_(o), cache_(cache), arena_(std::move(arena)) {
   hip_check(hipSetDevice(device_), "hipSetDevice");
   if (opt_.device_budget == 0) {
+    // free HBM plus the shared arena's cached blocks (freed when an
+    // allocation needs their room; freeing them now would make the first
+    // jobs allocate again, and large allocations are slow)
     size_t free_b = 0, total_b = 0;
     hip_check(hipMemGetInfo(&free_b, &total_b), "hipMemGetInfo");
-    opt_.device_budget = (uint64_t)(free_b * 0.9);
+    const uint64_t cached = arena_ ? arena_->cached_bytes() : 0;
+    opt_.device_budget = (uint64_t)((free_b + cached) * 0.9);
   }
-  // cached blocks stay within half the budget; the rest is freed as jobs end
-  arena_ = std::make_unique<DevArena>(opt_.device_budget / 2);
+  // an own arena keeps cached blocks within the budget; a cached block is
+  // freed when an allocation fails
+  if (!arena_) arena_ = std::make_shared<DevArena>(opt_.device_budget);
   if (opt_.max_running == 0) opt_.max_running = 1;
   if (!(opt_.pool_factor > 0)) opt_.pool_factor = 1.0;
   for (uint32_t i = 0; i < opt_.max_running; ++i) workers_.emplace_back(&Server::worker, this, (int)i);
@@ -50,7 +56,12 @@ static uint64_t job_pool_words(const RunInst& in, double pool_factor, uint64_t c
   const uint64_t n_in = std::max<uint64_t>(1, in.st->waves.size());
   const double est = (double)stim + pool_factor * 10.0 * (double)in.pg->gates.size() *
                                         (double)(stim / n_in + 1) + (double)(1u << 20);
-  return std::max<uint64_t>(1u << 22, (uint64_t)std::min(est, (double)cap_words));
+  // rounded up to a power of two (at most the cap): jobs of similar sizes
+  // then ask for the same block sizes, which the arena hands back cached
+  const uint64_t w = std::max<uint64_t>(1u << 22, (uint64_t)std::min(est, (double)cap_words));
+  uint64_t c = 1u << 22;
+  while (c < w) c <<= 1;
+  return std::min(c, std::max<uint64_t>(w, cap_words));
 }
 
 // Largest event pool of one job: 60% of the budget shared by the jobs that

@@ -67,7 +67,10 @@ struct ServerStats {
 
 class Server {
  public:
-  Server(int device, const ServerOptions& o, ProgramCache* cache);
+  // arena: the device-buffer cache to share (null: the server's own); the
+  // default budget counts its cached blocks as free
+  Server(int device, const ServerOptions& o, ProgramCache* cache,
+         std::shared_ptr<DevArena> arena = nullptr);
   ~Server();  // finishes every submitted job first
   // deadline_s > 0: the job fails with GLS_E_CANCELLED when it has not
   // finished that many seconds after submission (checked while queued and
@@ -117,7 +120,7 @@ class Server {
   int device_;
   ServerOptions opt_;
   ProgramCache* cache_;
-  std::unique_ptr<DevArena> arena_;  // the jobs' device buffers, reused from job to job
+  std::shared_ptr<DevArena> arena_;  // the jobs' device buffers, reused from job to job
   std::mutex mu_;
   std::condition_variable cv_work_, cv_done_;
   std::deque<uint64_t> queue_;

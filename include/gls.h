@@ -98,15 +98,20 @@ int gls_stim_info(const gls_stim* s, uint64_t* n_signals, uint64_t* n_events);
 void gls_stim_free(gls_stim* s);
 
 /* ------------------------------------------------------------ device --- */
+/* A context keeps the device buffers of finished runs cached for its later
+   runs and servers; they are freed when an allocation fails and by
+   gls_ctx_destroy (a server's default budget counts them as free). */
 int gls_ctx_create(int device, gls_ctx** out);
 void gls_ctx_destroy(gls_ctx* c);
-/* Free and total device memory of the context's device (hipMemGetInfo), for
- * callers that size batches to fit. */
+/* Free and total device memory of the context's device (hipMemGetInfo; free
+ * includes the context's cached blocks), for callers that size batches to
+ * fit. */
 int gls_ctx_mem_info(gls_ctx* c, uint64_t* free_bytes, uint64_t* total_bytes);
 
 typedef struct {
-  uint64_t pool_bytes;   /* event pool size, fixed (0: sized from the stimulus and grown
-                            between levels as needed, up to the free HBM) */
+  uint64_t pool_bytes;   /* event pool size, fixed (0: sized from the expected waveforms,
+                            up to 85% of the free HBM, taking the context's cached pool
+                            when there is one, and grown between levels as needed) */
   uint32_t seg_events;   /* target input events per time segment (0: path default) */
   uint32_t keep_waves;   /* 1: keep waveforms for gls_run_waveform */
   uint32_t path;         /* GLS_PATH_FUSED (default) or GLS_PATH_TWO_PASS */
