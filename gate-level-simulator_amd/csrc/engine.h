@@ -130,6 +130,9 @@ class DevArena {
   bool get_large(size_t want, size_t min_bytes, void** p, size_t* got);
   void put(void* p);
   void trim();  // frees every cached block
+  // frees every cached block when the device has less than 4 GiB (or 1.5%
+  // of its memory) free: the HIP runtime needs room of its own
+  void keep_headroom();
   uint64_t cached_bytes();
   uint64_t hits() const { return hits_; }
   uint64_t misses() const { return misses_; }
@@ -137,6 +140,7 @@ class DevArena {
  private:
   static size_t size_class(size_t bytes);
   void trim_locked();
+  void keep_headroom_locked();
   std::mutex mu_;
   std::multimap<size_t, void*> free_;          // cached blocks by size
   std::map<void*, size_t> live_;               // blocks handed out, with their sizes

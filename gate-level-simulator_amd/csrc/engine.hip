@@ -108,6 +108,10 @@ static uint32_t tile_cap(uint32_t seg) {
 
 void hip_check(hipError_t e, const char* what) {
   if (e == hipSuccess) return;
+  // the error travels in the exception: clear HIP's last-error state, which
+  // the launch wrappers read (hipGetLastError) and would otherwise report
+  // again for a later, good launch once the caller has handled this one
+  (void)hipGetLastError();
   int code = (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation) ? GLS_E_OOM : GLS_E_DEVICE;
   throw GlsError(code, std::string(what) + ": " + hipGetErrorString(e));
 }
@@ -465,6 +469,7 @@ bool DevArena::get(size_t bytes, void** p) {
     if (hipMalloc(&q, c) == hipSuccess) {
       live_[q] = c;
       *p = q;
+      keep_headroom_locked();
       return true;
     }
     (void)hipGetLastError();
@@ -500,6 +505,7 @@ bool DevArena::get_large(size_t want, size_t min_bytes, void** p, size_t* got) {
         live_[q] = m;
         *p = q;
         *got = m;
+        keep_headroom_locked();
         return true;
       }
       (void)hipGetLastError();
@@ -513,6 +519,24 @@ bool DevArena::get_large(size_t want, size_t min_bytes, void** p, size_t* got) {
   *p = nullptr;
   *got = 0;
   return false;
+}
+
+// The HIP runtime needs free device memory of its own (code objects loaded
+// at a kernel's first launch, launch arguments): when an allocation leaves
+// less than 4 GiB (or 1.5% of the device) free, the cached blocks go.
+void DevArena::keep_headroom_locked() {
+  if (free_.empty()) return;
+  size_t f = 0, t = 0;
+  if (hipMemGetInfo(&f, &t) != hipSuccess) {
+    (void)hipGetLastError();
+    return;
+  }
+  if (f < std::max<size_t>(4ull << 30, t / 64)) trim_locked();
+}
+
+void DevArena::keep_headroom() {
+  std::lock_guard<std::mutex> lk(mu_);
+  keep_headroom_locked();
 }
 
 uint64_t DevArena::cached_bytes() {
@@ -843,7 +867,15 @@ void Job::prepare(const std::vector<SimInstance>& inst, uint64_t pool_words) {
   prog_ = parts.size() == 1 ? parts[0] : union_image(parts, net_base_, gsplit);
   if (mode == 1 && prog_->max_k > kTileK)
     throw GlsError(GLS_E_UNSUPPORTED, "the two-pass path takes gates of at most 4 distinct inputs");
-  prog_->upload(device_, stream_);
+  try {
+    prog_->upload(device_, stream_);
+  } catch (const GlsError& e) {
+    // cached blocks of finished runs may hold the room the image needs
+    if (e.code != GLS_E_OOM || !arena_) throw;
+    arena_->trim();
+    prog_->upload(device_, stream_);
+  }
+  if (arena_) arena_->keep_headroom();
   n_gates_ = prog_->n_gates;
   n_nets_ = (uint32_t)nets;
   level_start_ = prog_->level_start;

@@ -303,3 +303,39 @@ def test_batch_toggle_counts_match_plain(ctx):
         plain = ctx.simulate(d, s, on, off, saif_tc=True).saif_text()
         assert "(TC " in plain
         assert b.instance_saif_text(k) == plain, k
+
+
+def test_cached_buffers_make_room_for_new_netlists():
+    # a finished run's pool stays cached by its context; a later batch of a
+    # netlist not yet resident needs more room for its program image than
+    # is left free, so the upload fails once, the cache is freed and the
+    # upload runs again (the failure must not be reported again by the
+    # batch's first kernel launch); results equal those on a fresh context
+    c = gls.Context(0)
+    try:
+        free, total = c.mem_info()
+        d0 = gls.Design.generate(0, 91, 300, 8, 16)
+        s0 = gls.Stimulus.generate(d0, 92, 20000, 200.0, 0.01)
+        big = (free - (512 << 20)) // 4096 * 4096
+        say("cached pool", big >> 30, "GiB of", free >> 30)
+        r = c.simulate(d0, s0, 0, 20001, pool_bytes=big)
+        assert r.stats()["pool_bytes"] == big
+        r.close()
+        f2, _ = c.mem_info()
+        assert f2 >= big  # cached blocks count as free
+        d1 = gls.Design.generate(1, 93, 500000, 40, 2560)
+        stims = [gls.Stimulus.generate(d1, 97 + k, 4000, 150.0, 0.01) for k in range(24)]
+        say("batch of 24 x 500k cells")
+        b = c.prepare_batch([d1] * 24, stims, [0] * 24, [4001] * 24).execute()
+        got = [b.instance_saif_text(k) for k in (0, 11, 23)]
+        b.close()
+    finally:
+        c.close()
+    fresh = gls.Context(0)
+    try:
+        for k, i in enumerate((0, 11, 23)):
+            r = fresh.simulate(d1, stims[i], 0, 4001)
+            assert r.saif_text() == got[k], i
+            r.close()
+    finally:
+        fresh.close()
