@@ -228,6 +228,7 @@ class Job {
   struct Group {
     uint32_t i0, i1, L0, L1, g0, g1, s0, s1;
     uint64_t horizon;
+    double est_words;  // estimated waveform words of the group's whole window
   };
   // one stimulus signal of the resident store: n (time, value) entries
   struct SigSrc {

@@ -845,6 +845,13 @@ void Job::prepare(const std::vector<SimInstance>& inst, uint64_t pool_words) {
   // sequence of time chunks (usually one), so per-gate work is not repeated
   // once per time chunk of the whole batch.
   std::vector<uint32_t> gsplit{0};
+  std::vector<double> inst_est(inst.size());
+  for (size_t i = 0; i < inst.size(); ++i) {
+    double stim = 0;
+    for (const Wave& w : inst[i].st->waves) stim += (double)w.size();
+    const double per_input = stim / (double)std::max<size_t>(1, inst[i].st->waves.size()) + 1;
+    inst_est[i] = stim + 10.0 * (double)parts[i]->n_gates * per_input;
+  }
   if (mode == 0 && inst.size() > 1) {
     size_t free_b = 0, total_b = 0;
     HC(hipMemGetInfo(&free_b, &total_b));
@@ -852,10 +859,7 @@ void Job::prepare(const std::vector<SimInstance>& inst, uint64_t pool_words) {
     const double target = pool_words ? (double)pool_words : 0.8 * ((double)free_b + cached) / 4.0;
     double acc = 0;
     for (size_t i = 0; i < inst.size(); ++i) {
-      double stim = 0;
-      for (const Wave& w : inst[i].st->waves) stim += (double)w.size();
-      const double per_input = stim / (double)std::max<size_t>(1, inst[i].st->waves.size()) + 1;
-      const double est = stim + 10.0 * (double)parts[i]->n_gates * per_input;
+      const double est = inst_est[i];
       if (i > gsplit.back() && acc + est > target) {
         gsplit.push_back((uint32_t)i);
         acc = 0;
@@ -963,6 +967,8 @@ void Job::prepare(const std::vector<SimInstance>& inst, uint64_t pool_words) {
     uint64_t tl = 0;
     for (uint32_t i = G.i0; i < G.i1; ++i) tl = std::max(tl, inst_t_last[i]);
     G.horizon = tl + (uint64_t)(nlev + 1) * wmax_ + 2;
+    G.est_words = 0;
+    for (uint32_t i = G.i0; i < G.i1; ++i) G.est_words += inst_est[i];
     groups_.push_back(G);
   }
 
@@ -1435,10 +1441,11 @@ void Job::end_chunk(const Chunk& c, const Group& G, bool device_driven) {
 }
 
 // A plain run of every chunk.  Automatic chunks: the first chunk is the
-// whole window; a chunk projected not to fit the largest pool (or that does
-// not fit) is run again shorter, the pool taking the largest size free HBM
-// allows, and each next chunk aims at half of that pool by the output rate
-// of the one before.
+// whole window, or shorter when the waveform estimate says it would not fit
+// the pool; a chunk projected not to fit the largest pool (or that does not
+// fit) is run again shorter, the pool taking the largest size free HBM
+// allows, and each next chunk aims at 60% of that pool by the output rate of
+// the one before.
 void Job::run_chunks(std::vector<hipEvent_t>& ev_eval) {
   HC(launch_set_scal(d_scal_, 0, stream_));
   HC(launch_fill_u64(d_saif_, (uint64_t)n_nets_ * kSaifWords, 0ull, stream_));
@@ -1480,6 +1487,16 @@ void Job::run_group_chunks(const Group& G, std::vector<hipEvent_t>& ev_eval) {
     L = std::min<uint64_t>(chunk_ps, lmax);
   } else {
     L = std::min<uint64_t>(horizon, lmax);
+    // first chunk from the waveform estimate (it runs one to three times
+    // high on the generator's netlists): about the pool's worth at 1.2x the
+    // estimated rate, the whole window when that covers most of it.  A first
+    // chunk that does not fit is found out only after part of it has run
+    // (projected from its first levels, then run again shorter): on the
+    // 2M-cell workload two such attempts cost 3.6% of the run.
+    if (G.est_words > 0) {
+      const double l0 = 1.2 * (double)pool_words_ * (double)horizon / G.est_words;
+      if (l0 < 0.8 * (double)L) L = std::max<uint64_t>(lmin, (uint64_t)l0);
+    }
   }
   // a chunk that did not fit caps every later chunk at the length it was
   // run again with: chunks after an empty stretch cannot grow back into a
@@ -1530,8 +1547,9 @@ void Job::run_group_chunks(const Group& G, std::vector<hipEvent_t>& ev_eval) {
     ++gsched_count_;
     if (Fn == kChunkEnd) break;
     if (chunk_ps == 0) {
-      // next length: half of the largest pool at this chunk's output rate
-      const double target = 0.5 * (double)(pool_auto_ && !static_ ? pool_max_words() : pool_words_);
+      // next length: 60% of the largest pool at this chunk's output rate
+      // (the level scratch and the projection's margin need the rest)
+      const double target = 0.6 * (double)(pool_auto_ && !static_ ? pool_max_words() : pool_words_);
       const double rate = (double)std::max<uint64_t>(used, 1) / (double)L;
       const double want = target / rate;
       L = (uint64_t)std::min<double>(std::max<double>(want, (double)L / 4), (double)L * 4);
