@@ -193,6 +193,8 @@ def plain_steps(ctx, name, steps, warmup, args, graph_steps=0):
         "tiles_per_step": st["tiles"], "hbm_path_tile_frac": round(st["tiles_hbm"] / max(1, st["tiles"]), 4),
         "saif_durations_consistent": saif_consistent(run.saif_text(), t_end + 1),
     }
+    plain_saif = run.saif_text() if graph_steps else None
+    run.close()  # its buffers (the event pool) go back to the context for the graph run
     if graph_steps:
         g = ctx.prepare(d, s, 0, t_end + 1, seg_events=args.seg_events, level_blocks=args.level_blocks,
                         prefetch=not args.no_prefetch, graph=True, hbm_stacks=args.hbm_stacks)
@@ -205,10 +207,9 @@ def plain_steps(ctx, name, steps, warmup, args, graph_steps=0):
         gst = g.stats()
         res["graph"] = {"ms_per_step": round(sum(gw) / len(gw) * 1e3, 3),
                         "events_per_s": round(gst["n_events"] / (sum(gw) / len(gw)), 1),
-                        "replayed": gst["eval_launches"] == 1,
-                        "saif_same_as_plain": g.saif_text() == run.saif_text()}
+                        "time_chunks": gst["chunks"], "replayed": gst["eval_launches"] == 1,
+                        "saif_same_as_plain": g.saif_text() == plain_saif}
         g.close()
-    run.close()
     return res
 
 

@@ -1494,7 +1494,10 @@ void Job::run_group_chunks(const Group& G, std::vector<hipEvent_t>& ev_eval) {
     // (projected from its first levels, then run again shorter): on the
     // 2M-cell workload two such attempts cost 3.6% of the run.
     if (G.est_words > 0) {
-      const double l0 = 1.2 * (double)pool_words_ * (double)horizon / G.est_words;
+      // a growable pool counts at the largest size it may take
+      const double pw = (double)(pool_auto_ && !static_ ? std::max(pool_words_, pool_max_words())
+                                                        : pool_words_);
+      const double l0 = 1.2 * pw * (double)horizon / G.est_words;
       if (l0 < 0.8 * (double)L) L = std::max<uint64_t>(lmin, (uint64_t)l0);
     }
   }
