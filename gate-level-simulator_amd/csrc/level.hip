@@ -422,14 +422,10 @@ __device__ __forceinline__ Lane<SP> lane_walk(const TileCtx<K, LDS>& X, uint32_t
     st[0] = (1u << 2) | kX;
     nc = 1;
   }
-  // pass 1: merge and evaluate
-  for (;;) {
-    uint32_t w = h[0];
-#pragma unroll
-    for (int j = 1; j < K; ++j) w = min(w, h[j]);
-    const uint32_t t = w >> 2;
-    if (t >= halo_end) break;  // also the end of every input (INF)
-    // every input with an event at time t
+  // pass 1: merge and evaluate.  One step takes every input event at the
+  // next event time t (w: its smallest word) and returns the gate's value
+  // after it, its output time o and whether it is a candidate (a change).
+  auto step = [&](uint32_t w, uint32_t t, uint32_t& o, bool& cand) -> uint32_t {
     const uint32_t lim = w | 3u;
     uint32_t ev = 0, nidx = idx;
 #pragma unroll
@@ -446,17 +442,42 @@ __device__ __forceinline__ Lane<SP> lane_walk(const TileCtx<K, LDS>& X, uint32_t
     const uint32_t key = (((sc - 1u) & 3u) << (2 * K)) | event_bits(idx, nidx, ev);
     idx = nidx;
     fprev = f;
-    const uint32_t o = t + s_dt[key];  // sc = 0 reads a valid entry that is not used
-    const bool own = t < tau_n;
-    nown += (own && t >= lo_rel) ? 1u : 0u;
-    const bool cand = sc != 0u;
-    if (own) {
-      const bool room = !kGuard || nc < stk_lim;
-      bad |= room ? 0u : 1u;
-      st[room ? nc : stk_lim - 1] = (o << 2) | f;
-      nc += (cand && room) ? 1u : 0u;
-    }
-    hmin = (cand && !own) ? min(hmin, o) : hmin;
+    o = t + s_dt[key];  // sc = 0 reads a valid entry that is not used
+    cand = sc != 0u;
+    return f;
+  };
+  auto next_word = [&]() -> uint32_t {
+    uint32_t w = h[0];
+#pragma unroll
+    for (int j = 1; j < K; ++j) w = min(w, h[j]);
+    return w;
+  };
+  // the lane's own events (t < tau_n): candidates go to the stack.  The
+  // halo comes after them in time, so it gets a loop of its own and the
+  // owned steps carry none of its bookkeeping.
+  for (;;) {
+    const uint32_t w = next_word();
+    const uint32_t t = w >> 2;
+    if (t >= tau_n) break;  // also the end of every input (INF)
+    uint32_t o;
+    bool cand;
+    const uint32_t f = step(w, t, o, cand);
+    nown += t >= lo_rel ? 1u : 0u;
+    const bool room = !kGuard || nc < stk_lim;
+    bad |= room ? 0u : 1u;
+    st[room ? nc : stk_lim - 1] = (o << 2) | f;
+    nc += (cand && room) ? 1u : 0u;
+  }
+  // the halo (tau_n <= t < halo_end): only the smallest output time of its
+  // candidates is kept
+  for (;;) {
+    const uint32_t w = next_word();
+    const uint32_t t = w >> 2;
+    if (t >= halo_end) break;
+    uint32_t o;
+    bool cand;
+    step(w, t, o, cand);
+    hmin = cand ? min(hmin, o) : hmin;
   }
   // pass 2: keep the candidates strictly earlier than every later one,
   // compacted (in time order) to the end of the region
