@@ -481,7 +481,7 @@ __device__ __forceinline__ Lane<SP> lane_walk(const TileCtx<K, LDS>& X, uint32_t
   }
   // pass 2: keep the candidates strictly earlier than every later one,
   // compacted (in time order) to the end of the region
-  uint32_t runmin = min(hmin, hi_rel), j = nc, cnt = 0, top = 0, lastv = 4u, has = 0;
+  uint32_t runmin = min(hmin, hi_rel), j = nc, cnt = 0, lastv = 4u;
   for (uint32_t i = nc; i-- > 0;) {
     const uint32_t e = st[i];
     const uint32_t o = e >> 2;
@@ -492,12 +492,12 @@ __device__ __forceinline__ Lane<SP> lane_walk(const TileCtx<K, LDS>& X, uint32_t
     const uint32_t v = e & 3u;
     cnt += (keep && v != lastv) ? 1u : 0u;
     lastv = keep ? v : lastv;
-    top = (keep && !has) ? e : top;
-    has |= keep ? 1u : 0u;
   }
   ln.st = st + j;
   ln.sp = nc - j;
-  ln.top = top;
+  // the last kept candidate: the first one kept going backwards, now at the
+  // region's last slot
+  ln.top = j < nc ? st[nc - 1] : 0u;
   ln.cnt = cnt;
   ln.nown = nown;
   return ln;
@@ -645,7 +645,9 @@ __device__ __forceinline__ void emit_tile(const LevelArgs& A, uint64_t tk, const
       a0 += pvv == 0u ? len : 0u;
       a1 += pvv == 3u ? len : 0u;
       if (v != pvv) {
-        if (w < out_cap) dst[w] = e;
+        // past out_cap (inputs out of time order: the run fails with
+        // ERR_ORDER) stores land on the region's last word
+        dst[min(w, out_cap - 1u)] = e;
         ++w;
       }
       pvv = v;
