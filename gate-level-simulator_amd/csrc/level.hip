@@ -455,10 +455,11 @@ __device__ __forceinline__ Lane<SP> lane_walk(const TileCtx<K, LDS>& X, uint32_t
   // the lane's own events (t < tau_n): candidates go to the stack.  The
   // halo comes after them in time, so it gets a loop of its own and the
   // owned steps carry none of its bookkeeping.
-  for (;;) {
-    const uint32_t w = next_word();
-    const uint32_t t = w >> 2;
-    if (t >= tau_n) break;  // also the end of every input (INF)
+  // (Both loops test at the bottom: one compare a step decides both the
+  // lanes that go on and the loop's exit.)
+  uint32_t w = next_word();
+  uint32_t t = w >> 2;
+  while (t < tau_n) {  // the end of every input (INF) is never below tau_n
     uint32_t o;
     bool cand;
     const uint32_t f = step(w, t, o, cand);
@@ -467,17 +468,18 @@ __device__ __forceinline__ Lane<SP> lane_walk(const TileCtx<K, LDS>& X, uint32_t
     bad |= room ? 0u : 1u;
     st[room ? nc : stk_lim - 1] = (o << 2) | f;
     nc += (cand && room) ? 1u : 0u;
+    w = next_word();
+    t = w >> 2;
   }
   // the halo (tau_n <= t < halo_end): only the smallest output time of its
   // candidates is kept
-  for (;;) {
-    const uint32_t w = next_word();
-    const uint32_t t = w >> 2;
-    if (t >= halo_end) break;
+  while (t < halo_end) {
     uint32_t o;
     bool cand;
     step(w, t, o, cand);
     hmin = cand ? min(hmin, o) : hmin;
+    w = next_word();
+    t = w >> 2;
   }
   // pass 2: keep the candidates strictly earlier than every later one,
   // compacted (in time order) to the end of the region
