@@ -632,14 +632,15 @@ __device__ __forceinline__ void emit_tile(const LevelArgs& A, uint64_t tk, const
   const uint32_t ct0 = pct;
   uint32_t a0 = 0, a1 = 0;
   uint32_t w = carry.has ? carry.cnt : 0u;  // lane's first slot in the tile output
-  // four stack reads in flight per step (indices clamped into the stack)
+  // four stack reads in flight per step; indices past the last entry read
+  // the last entry again, which adds nothing (same value: no store; same
+  // time: no duration), so every step takes all four
   for (uint32_t i = 0; i < sp; i += 4) {
     uint32_t e4[4];
 #pragma unroll
     for (int q = 0; q < 4; ++q) e4[q] = st[min(i + q, sp - 1)];
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
-      if (i + q >= sp) break;
       const uint32_t e = e4[q];
       const uint32_t v = e & 3u;
       const uint32_t ct = max(min(e >> 2, e32), s32);
