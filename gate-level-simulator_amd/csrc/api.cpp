@@ -36,16 +36,27 @@ struct gls_design {
 struct gls_stim {
   Stimulus st;
 };
+// A HIP stream destroyed with the last of its holders: a context and the
+// runs prepared on it (a run may be freed after its context, and its buffers
+// go back to the arena only once the stream has drained).
+struct StreamRef {
+  hipStream_t s = nullptr;
+  ~StreamRef() {
+    if (s) (void)hipStreamDestroy(s);
+  }
+};
 struct gls_ctx {
   int device = 0;
-  hipStream_t stream = nullptr;
+  hipStream_t stream = nullptr;  // stream_ref->s
+  std::shared_ptr<StreamRef> stream_ref;
   ProgramCache cache;  // resident program images of this device
   // device buffers of finished runs, reused by later runs and servers of
   // this context (large device allocations and frees can take seconds)
   std::shared_ptr<DevArena> arena;
 };
 struct gls_run {
-  std::shared_ptr<DevArena> arena;  // outlives the job, which returns its buffers to it
+  std::shared_ptr<StreamRef> stream_ref;  // outlive the job, which syncs the stream and
+  std::shared_ptr<DevArena> arena;        // returns its buffers to the arena
   std::unique_ptr<Job> job;
   std::vector<RunInst> inst;
   bool keep_waves = false;
@@ -376,7 +387,9 @@ int gls_ctx_create(int device, gls_ctx** out) {
   auto c = std::make_unique<gls_ctx>();
   c->device = device;
   hip_check(hipSetDevice(device), "hipSetDevice");
-  hip_check(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking), "hipStreamCreate");
+  c->stream_ref = std::make_shared<StreamRef>();
+  hip_check(hipStreamCreateWithFlags(&c->stream_ref->s, hipStreamNonBlocking), "hipStreamCreate");
+  c->stream = c->stream_ref->s;
   size_t free_b = 0, total_b = 0;
   hip_check(hipMemGetInfo(&free_b, &total_b), "hipMemGetInfo");
   c->arena = std::make_shared<DevArena>((uint64_t)total_b);
@@ -386,7 +399,7 @@ int gls_ctx_create(int device, gls_ctx** out) {
 
 void gls_ctx_destroy(gls_ctx* c) {
   if (!c) return;
-  if (c->stream) (void)hipStreamDestroy(c->stream);
+  // the stream goes with the last run prepared on this context
   delete c;
 }
 
@@ -436,6 +449,7 @@ int gls_prepare_batch(gls_ctx* c, size_t n, const gls_design* const* designs,
   }
   r->keep_waves = opts && opts->keep_waves;
   r->saif_tc = opts && opts->saif_tc;
+  r->stream_ref = c->stream_ref;
   r->arena = c->arena;
   r->job = std::make_unique<Job>(c->device, c->stream, c->arena.get());
   apply_opts(*r->job, opts);

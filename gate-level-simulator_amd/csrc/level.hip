@@ -457,17 +457,31 @@ __device__ __forceinline__ Lane<SP> lane_walk(const TileCtx<K, LDS>& X, uint32_t
   // owned steps carry none of its bookkeeping.
   // (Both loops test at the bottom: one compare a step decides both the
   // lanes that go on and the loop's exit.)
-  uint32_t w = next_word();
-  uint32_t t = w >> 2;
-  while (t < tau_n) {  // the end of every input (INF) is never below tau_n
-    uint32_t o;
-    bool cand;
-    const uint32_t f = step(w, t, o, cand);
-    nown += t >= lo_rel ? 1u : 0u;
+  auto push = [&](uint32_t o, uint32_t f, bool cand) {
     const bool room = !kGuard || nc < stk_lim;
     bad |= room ? 0u : 1u;
     st[room ? nc : stk_lim - 1] = (o << 2) | f;
     nc += (cand && room) ? 1u : 0u;
+  };
+  uint32_t w = next_word();
+  uint32_t t = w >> 2;
+  // look-back events (t < lo_rel, chunks after the first) come first: they
+  // make candidates but are not this chunk's gate events
+  const uint32_t t_own = min(lo_rel, tau_n);
+  while (t < t_own) {
+    uint32_t o;
+    bool cand;
+    const uint32_t f = step(w, t, o, cand);
+    push(o, f, cand);
+    w = next_word();
+    t = w >> 2;
+  }
+  while (t < tau_n) {  // the end of every input (INF) is never below tau_n
+    uint32_t o;
+    bool cand;
+    const uint32_t f = step(w, t, o, cand);
+    ++nown;
+    push(o, f, cand);
     w = next_word();
     t = w >> 2;
   }

@@ -305,6 +305,21 @@ def test_batch_toggle_counts_match_plain(ctx):
         assert b.instance_saif_text(k) == plain, k
 
 
+def test_run_outlives_its_context():
+    # a run freed after its context (as a garbage-collected handle can be)
+    # still reads its results and frees its buffers: the context's stream
+    # and buffer cache live until the run is gone
+    c = gls.Context(0)
+    d, s = golden_cases.make("c0_small")
+    start, end = golden_cases.CASES["c0_small"]["window"]
+    r = c.simulate(d, s, start, end)
+    text = r.saif_text()
+    c.close()
+    assert r.saif_text() == text
+    assert oracle_bind.sha256(text) == GOLDEN["c0_small"]["saif_sha256"]
+    r.close()
+
+
 def test_cached_buffers_make_room_for_new_netlists():
     # a finished run's pool stays cached by its context; a later batch of a
     # netlist not yet resident needs more room for its program image than
