@@ -100,7 +100,9 @@ void gls_stim_free(gls_stim* s);
 /* ------------------------------------------------------------ device --- */
 /* A context keeps the device buffers of finished runs cached for its later
    runs and servers; they are freed when an allocation fails and by
-   gls_ctx_destroy (a server's default budget counts them as free). */
+   gls_ctx_destroy (a server's default budget counts them as free).  Runs
+   may be freed before or after their context; servers must be destroyed
+   before it. */
 int gls_ctx_create(int device, gls_ctx** out);
 void gls_ctx_destroy(gls_ctx* c);
 /* Free and total device memory of the context's device (hipMemGetInfo; free
@@ -278,9 +280,11 @@ typedef struct {
   uint32_t peak_running, resident_programs;
   uint64_t peak_bytes;
   uint64_t cancelled;     /* failed jobs that were cancelled or missed their deadline */
-  uint64_t arena_hits;    /* device buffers of jobs taken from the server's cache */
+  uint64_t arena_hits;    /* device buffers taken from the context's cache */
   uint64_t arena_misses;  /* device buffers newly allocated */
 } gls_server_stats;
+/* The server uses the context's program cache and device-buffer cache:
+   destroy it before the context. */
 int gls_server_create(gls_ctx* c, const gls_server_opts* o, gls_server** out);
 /* The server copies the stimulus; the design must outlive the server. */
 int gls_server_submit(gls_server* s, const gls_design* d, const gls_stim* st, uint64_t dumpon,

@@ -1,5 +1,11 @@
 """The measured configurations as parity cases on the GPU.
 
+* 200k cells (config 1: 40 levels, 1024 inputs, gap 500 ps) over its full
+  0-20 Mps window: T0 + T1 + TX = DURATION on every record; one chunk,
+  fixed 5 Mps chunks and a captured graph's replays give the same SAIF
+  text and gate events (a 6,000-cell netlist of the same generator is
+  pinned to the oracle's digests in test_gpu_parity.py).
+
 * 2M cells (config 2: log-normal level widths, 60 levels, 4096 inputs, gap
   200 ps) over its full 0-20 Mps window: too large for the oracle, so it is
   checked through properties that hold at any size: T0 + T1 + TX = DURATION
@@ -83,6 +89,34 @@ def test_c2_full_window_properties(ctx):
     assert len(recs) >= info["n_instances"]
     bad = [r for r in recs if r[1] + r[2] + r[3] != duration]
     assert not bad, bad[:5]
+
+
+def test_c1_full_window_properties(ctx):
+    t0 = time.time()
+    d = gls.Design.generate(1, 0x5EED0001, 200000, 40, 1024)
+    s = gls.Stimulus.generate(d, 0x5EED0001, 20000000, 500.0, 0.01)
+    say("c1 generated", "%.0fs" % (time.time() - t0))
+    r = ctx.simulate(d, s, 0, 20000001)
+    st = r.stats()
+    text = r.saif_text()
+    r.close()
+    say("c1", st["n_events"], "events", st["chunks"], "chunks")
+    assert st["n_events"] > 5 * 10 ** 10
+    recs, duration = saif_records(text)
+    assert duration == 20000001 and len(recs) >= 200000
+    bad = [x for x in recs if x[1] + x[2] + x[3] != duration]
+    assert not bad, bad[:5]
+    r = ctx.simulate(d, s, 0, 20000001, chunk_ps=5000000)
+    assert r.stats()["chunks"] >= 4
+    assert r.stats()["n_events"] == st["n_events"] and r.saif_text() == text
+    r.close()
+    g = ctx.prepare(d, s, 0, 20000001, graph=True)
+    for rep_ in range(3):
+        g.execute()
+        assert g.stats()["n_events"] == st["n_events"], rep_
+        assert g.saif_text() == text, rep_
+    assert g.stats()["eval_launches"] == 1
+    g.close()
 
 
 def c3_mix():
