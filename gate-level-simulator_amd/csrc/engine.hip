@@ -1045,16 +1045,18 @@ void Job::prepare(const std::vector<SimInstance>& inst, uint64_t pool_words) {
       const uint64_t n_in = std::max<uint64_t>(1, I.st->waves.size());
       est += (double)stim + 10.0 * (double)I.pg->gates.size() * (double)(stim / n_in + 1);
     }
-    uint64_t least = std::max<uint64_t>(1ull << 24, 4 * stim_words_);
-    if (mode == 1) least = std::max<uint64_t>(least, stim_words_ + 1024);
-    least = std::min<uint64_t>(least, (uint64_t)room);
+    // the hard minimum (the two-pass path keeps the stimulus in the pool)
+    // applies after the clamp to the room: a taken block is never enlarged
+    const uint64_t need = mode == 1 ? stim_words_ + 1024 : 1024;
+    uint64_t least = std::min<uint64_t>(std::max<uint64_t>(1ull << 24, 4 * stim_words_), (uint64_t)room);
+    least = std::max(least, need);
     const uint64_t want = std::max<uint64_t>(least, (uint64_t)std::min(est, room));
     if (arena_) {
       size_t got = 0;
       if (!arena_->get_large((want + kPoolPad) * sizeof(uint32_t), (least + kPoolPad) * sizeof(uint32_t),
                              (void**)&d_pool_, &got))
         HC(hipErrorOutOfMemory);
-      pool_words = got / sizeof(uint32_t) - kPoolPad;
+      pool_words = got / sizeof(uint32_t) - kPoolPad;  // >= least: got >= (least + kPoolPad) * 4
       pool_taken = true;
     } else {
       pool_words = want;
