@@ -98,18 +98,19 @@ def test_udp_mux2_matches_library_table():
 
 
 def test_check_delay_rule():
-    for pre, cur in itertools.product(range(4), range(4)):
-        if (pre == V0 and cur != V0) or (cur == V1 and pre != V1):
-            want = 1
-        elif (pre == V1 and cur != V1) or (cur == V0 and pre != V0):
-            want = 2
-        elif (pre, cur) == (VX, VZ):
-            want = 3
-        elif (pre, cur) == (VZ, VX):
-            want = 4
-        else:
-            want = 0
-        assert oracle_bind.check_delay(pre, cur) == want
+    # CheckDelay's edge classes written out pair by pair (1 rise: leaving 0 or
+    # reaching 1; 2 fall: leaving 1 or reaching 0; 3: X -> Z takes the larger
+    # of rise and fall; 4: Z -> X the smaller; 0: no change), rows previous
+    # value, columns current value, in the order 0, X, Z, 1
+    table = {
+        V0: {V0: 0, VX: 1, VZ: 1, V1: 1},
+        VX: {V0: 2, VX: 0, VZ: 3, V1: 1},
+        VZ: {V0: 2, VX: 4, VZ: 0, V1: 1},
+        V1: {V0: 2, VX: 2, VZ: 2, V1: 0},
+    }
+    for pre, row in table.items():
+        for cur, want in row.items():
+            assert oracle_bind.check_delay(pre, cur) == want, (pre, cur)
 
 
 @pytest.mark.parametrize("case", kat_cases.CASES, ids=[c["name"] for c in kat_cases.CASES])
