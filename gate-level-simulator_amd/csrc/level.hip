@@ -286,7 +286,9 @@ __device__ __forceinline__ void stage_tile(const LevelArgs& A, const TileDesc& D
 // folds AddDelay, src/simulate.cpp:1012-1176) and candidate output time
 // o = t + d; each candidate of an owned event is appended as (o << 2 | f).
 // Events in the halo (the next segment's first D_max) only record the
-// smallest output time they make.
+// smallest output time they make.  Look-back events, owned events and halo
+// events come in that order in time, so each has a loop of its own, tested
+// at the bottom: a step carries only its own kind's bookkeeping.
 //
 // Pass 2 applies the preemption rule of src/simulate.cpp:990-1009 in its
 // closed form: a candidate is kept iff its output time is strictly below
